@@ -1,0 +1,102 @@
+"""ctypes binding of ``libcovertreex_hip.so`` (C-ABI in ``include/covertreex_hip.h``).
+
+The library is required: there is no CPU fallback.  Loading fails with a clear
+error when it has not been built (``python -c "import __graft_entry__ as g;
+g.build()"``) and every entry point raises :class:`HipError` on a non-zero
+status, carrying the library's thread-local error text.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcovertreex_hip.so")
+
+CX_OK = 0
+CX_ERR_INVALID = 1
+CX_ERR_HIP = 2
+CX_ERR_NO_DEVICE = 3
+CX_ERR_UNSUPPORTED = 4
+
+CX_ORDER_HILBERT = 0
+CX_ORDER_NATURAL = 1
+
+CX_KNN_REFERENCE_F32 = 0
+CX_KNN_MIXED = 1
+
+_i64 = C.c_int64
+_i32 = C.c_int32
+_f32 = C.c_float
+_p = C.c_void_p
+
+# name -> (restype, argtypes); every symbol declared in include/covertreex_hip.h
+SIGNATURES = {
+    "cx_last_error": (C.c_char_p, []),
+    "cx_device_count": (_i32, [_p]),
+    "cx_tree_build_residual": (_i32, [_p, _p, _p, _i64, _i32, _i32, _f32, _p, _i32, _i32, _i32,
+                                      _i32, _f32, _p]),
+    "cx_tree_knn_residual": (_i32, [_p, _p, _i64, _i32, _i32, _i32, _p, _p, _p]),
+    "cx_tree_knn_residual_device": (_i32, [_p, _p, _i64, _i32, _i32, _i32, _p, _p, _p]),
+    "cx_tree_info": (_i32, [_p, _p, _p, _p, _p, _p]),
+    "cx_tree_get_parents": (_i32, [_p, _p]),
+    "cx_tree_get_children": (_i32, [_p, _p]),
+    "cx_tree_get_next": (_i32, [_p, _p]),
+    "cx_tree_get_levels": (_i32, [_p, _p]),
+    "cx_tree_get_si_cache": (_i32, [_p, _p]),
+    "cx_tree_get_order": (_i32, [_p, _p]),
+    "cx_tree_get_subtree_bounds": (_i32, [_p, _p, _p]),
+    "cx_subtree_bounds": (_i32, [_p, _p, _i64, _p, _p]),
+    "cx_morton_order": (_i32, [_p, _i64, _i32, _p]),
+    "cx_nystrom_residual": (_i32, [_p, _i64, _i32, _p, _i32, C.c_double, _p, _i32, _i32,
+                                   C.c_double, _p, _p]),
+    "cx_tree_free": (None, [_p]),
+}
+
+
+class HipError(RuntimeError):
+    """A non-zero cx_status from the HIP library."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"covertreex_hip error {status}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load the HIP library (once).  Raises ``ImportError`` when it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(
+            f"{p} is missing: the HIP engine has no CPU fallback; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != CX_OK:
+        msg = load().cx_last_error()
+        raise HipError(int(status), msg.decode() if msg else "")
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    check(load().cx_device_count(C.byref(n)))
+    return int(n.value)

@@ -1,0 +1,296 @@
+// C-ABI entry points of libcovertreex_hip.so (declared in include/covertreex_hip.h).
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "cx_internal.hpp"
+
+namespace cx {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+TreeState::~TreeState()
+{
+    if (stream)
+        (void)hipStreamSynchronize(stream);
+    // DevBuf members release themselves; the stream goes last
+    v.release();
+    scaled.release();
+    raw.release();
+    norms.release();
+    p.release();
+    order.release();
+    inv_order.release();
+    parents.release();
+    children.release();
+    next.release();
+    levels.release();
+    depth.release();
+    si.release();
+    smin.release();
+    smax.release();
+    roots.release();
+    child_off.release();
+    child_list.release();
+    vhi.release();
+    vlo.release();
+    vnorm.release();
+    xnorm.release();
+    si_acc.release();
+    work.release();
+    if (stream)
+        (void)hipStreamDestroy(stream);
+}
+
+uint8_t *workspace(TreeState &t, size_t bytes)
+{
+    if (t.work.count < bytes) {
+        CX_HIP(hipStreamSynchronize(t.stream));
+        t.work.alloc(bytes);
+    }
+    return t.work.ptr;
+}
+
+template <typename F> static cx_status guarded(F &&f)
+{
+    try {
+        f();
+        return CX_OK;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return CX_ERR_INVALID;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return CX_ERR_INVALID;
+    }
+}
+
+static void require_device()
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0)
+        throw Error(CX_ERR_NO_DEVICE, "no HIP device available");
+}
+
+static void copy_to_host(const TreeState &t, void *dst, const void *src, size_t bytes)
+{
+    CX_REQUIRE(dst != nullptr, "output pointer is NULL");
+    if (bytes == 0)
+        return;
+    CX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, t.stream));
+    CX_HIP(hipStreamSynchronize(t.stream));
+}
+
+} // namespace cx
+
+using namespace cx;
+
+extern "C" {
+
+const char *cx_last_error(void) { return g_last_error.c_str(); }
+
+cx_status cx_device_count(int32_t *out)
+{
+    return guarded([&] {
+        CX_REQUIRE(out != nullptr, "out is NULL");
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess)
+            count = 0;
+        *out = count;
+    });
+}
+
+cx_status cx_tree_build_residual(const float *V, const float *p, const float *coords, int64_t n,
+                                 int32_t m, int32_t d, float var, const float *ls, int32_t n_ls,
+                                 int32_t kernel_type, int32_t chunk, int32_t order_mode,
+                                 float grid_scale, cx_tree **out)
+{
+    return guarded([&] {
+        CX_REQUIRE(out != nullptr, "out is NULL");
+        *out = nullptr;
+        CX_REQUIRE(n >= 0 && n < (int64_t)1 << 31, "n must be in [0, 2^31)");
+        CX_REQUIRE(m >= 1 && d >= 1 && d <= 64, "need m >= 1 and 1 <= d <= 64");
+        CX_REQUIRE(n == 0 || (V && p && coords), "V, p and coords must be non-NULL");
+        CX_REQUIRE(order_mode == CX_ORDER_HILBERT || order_mode == CX_ORDER_NATURAL,
+                   "order_mode must be CX_ORDER_HILBERT or CX_ORDER_NATURAL");
+        require_device();
+        cx_tree *h = new cx_tree();
+        try {
+            TreeState &t = h->s;
+            CX_HIP(hipGetDevice(&t.device));
+            CX_HIP(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
+            t.n = n;
+            t.m = m;
+            t.d = d;
+            t.var = var;
+            t.kernel = kernel_type == CX_KERNEL_MATERN52 ? 1 : 0; // src/metric.rs:50-56
+            // inv_ls[i] = 1 / (|ls| < 1e-6 ? 1e-6 : ls), fallback ls[0] or 1
+            // (ResidualMetric::new, src/metric.rs:93-101)
+            const float fallback = (ls && n_ls > 0) ? ls[0] : 1.0f;
+            t.inv_ls.resize(d);
+            for (int32_t c = 0; c < d; c++) {
+                float raw = (ls && c < n_ls) ? ls[c] : fallback;
+                float safe = std::fabs(raw) < 1e-6f ? 1e-6f : raw;
+                t.inv_ls[c] = 1.0f / safe;
+            }
+            if (n > 0)
+                build_residual_tree(t, V, p, coords, chunk, order_mode, grid_scale);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+cx_status cx_tree_knn_residual_device(cx_tree *t, const int64_t *d_queries, int64_t nq, int32_t k,
+                                      int32_t predecessor, int32_t mode, int64_t *d_out_idx,
+                                      float *d_out_dist, float *d_out_err)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        CX_REQUIRE(k >= 1, "k must be >= 1");
+        CX_REQUIRE(nq >= 0, "nq must be >= 0");
+        CX_REQUIRE(nq == 0 || (d_queries && d_out_idx && d_out_dist), "NULL query/output pointer");
+        std::lock_guard<std::mutex> lock(t->s.mu);
+        CX_HIP(hipSetDevice(t->s.device));
+        if (nq == 0)
+            return;
+        if (mode == CX_KNN_REFERENCE_F32)
+            knn_reference_f32(t->s, d_queries, nq, k, predecessor, d_out_idx, d_out_dist,
+                              d_out_err);
+        else if (mode == CX_KNN_MIXED)
+            knn_mixed(t->s, d_queries, nq, k, predecessor, d_out_idx, d_out_dist, d_out_err);
+        else
+            throw Error(CX_ERR_INVALID, "unknown kNN mode");
+        CX_HIP(hipStreamSynchronize(t->s.stream));
+    });
+}
+
+cx_status cx_tree_knn_residual(cx_tree *t, const int64_t *queries, int64_t nq, int32_t k,
+                               int32_t predecessor, int32_t mode, int64_t *out_idx,
+                               float *out_dist, float *out_err)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        CX_REQUIRE(k >= 1, "k must be >= 1");
+        CX_REQUIRE(nq >= 0, "nq must be >= 0");
+        CX_REQUIRE(nq == 0 || (queries && out_idx && out_dist), "NULL query/output pointer");
+        if (nq == 0)
+            return;
+        DevBuf<int64_t> q(nq), oi((size_t)nq * k);
+        DevBuf<float> od((size_t)nq * k), oe(out_err ? (size_t)nq * k : 0);
+        hipStream_t st = t->s.stream;
+        CX_HIP(hipMemcpyAsync(q.ptr, queries, sizeof(int64_t) * nq, hipMemcpyHostToDevice, st));
+        cx_status s = cx_tree_knn_residual_device(t, q.ptr, nq, k, predecessor, mode, oi.ptr,
+                                                  od.ptr, out_err ? oe.ptr : nullptr);
+        if (s != CX_OK)
+            throw Error(s, g_last_error);
+        CX_HIP(hipMemcpyAsync(out_idx, oi.ptr, sizeof(int64_t) * nq * k, hipMemcpyDeviceToHost,
+                              st));
+        CX_HIP(hipMemcpyAsync(out_dist, od.ptr, sizeof(float) * nq * k, hipMemcpyDeviceToHost,
+                              st));
+        if (out_err)
+            CX_HIP(hipMemcpyAsync(out_err, oe.ptr, sizeof(float) * nq * k,
+                                  hipMemcpyDeviceToHost, st));
+        CX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+cx_status cx_tree_info(const cx_tree *t, int64_t *n, int32_t *m, int32_t *d, int32_t *min_level,
+                       int32_t *max_level)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        if (n)
+            *n = t->s.n;
+        if (m)
+            *m = t->s.m;
+        if (d)
+            *d = t->s.d;
+        if (min_level)
+            *min_level = t->s.min_level;
+        if (max_level)
+            *max_level = t->s.max_level;
+    });
+}
+
+#define CX_GETTER(name, field, type)                                                  \
+    cx_status name(const cx_tree *t, type *out)                                       \
+    {                                                                                 \
+        return guarded([&] {                                                          \
+            CX_REQUIRE(t != nullptr, "tree is NULL");                                 \
+            copy_to_host(t->s, out, t->s.field.ptr, sizeof(type) * (size_t)t->s.n);   \
+        });                                                                           \
+    }
+
+CX_GETTER(cx_tree_get_parents, parents, int64_t)
+CX_GETTER(cx_tree_get_children, children, int64_t)
+CX_GETTER(cx_tree_get_next, next, int64_t)
+CX_GETTER(cx_tree_get_levels, levels, int32_t)
+CX_GETTER(cx_tree_get_si_cache, si, float)
+CX_GETTER(cx_tree_get_order, order, int64_t)
+
+cx_status cx_tree_get_subtree_bounds(const cx_tree *t, int64_t *min_out, int64_t *max_out)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        copy_to_host(t->s, min_out, t->s.smin.ptr, sizeof(int64_t) * (size_t)t->s.n);
+        copy_to_host(t->s, max_out, t->s.smax.ptr, sizeof(int64_t) * (size_t)t->s.n);
+    });
+}
+
+cx_status cx_subtree_bounds(const int64_t *parents, const int64_t *node_to_dataset, int64_t n,
+                            int64_t *min_out, int64_t *max_out)
+{
+    return guarded([&] {
+        CX_REQUIRE(n >= 0 && n < (int64_t)1 << 31, "n must be in [0, 2^31)");
+        if (n == 0)
+            return;
+        CX_REQUIRE(parents && node_to_dataset && min_out && max_out, "NULL pointer");
+        for (int64_t i = 0; i < n; i++)
+            CX_REQUIRE(parents[i] >= -1 && parents[i] < n, "parent index out of range");
+        require_device();
+        subtree_bounds_device(parents, node_to_dataset, n, min_out, max_out);
+    });
+}
+
+cx_status cx_morton_order(const float *coords, int64_t n, int32_t d, int64_t *order_out)
+{
+    return guarded([&] {
+        CX_REQUIRE(n >= 0 && n < (int64_t)1 << 31, "n must be in [0, 2^31)");
+        if (n == 0)
+            return;
+        CX_REQUIRE(coords && order_out && d >= 1, "NULL pointer or d < 1");
+        require_device();
+        morton_order_device(coords, n, d, order_out);
+    });
+}
+
+cx_status cx_nystrom_residual(const double *points, int64_t n, int32_t d,
+                              const int64_t *inducing_idx, int32_t m, double var, const double *ls,
+                              int32_t n_ls, int32_t kernel_type, double jitter, float *v_out,
+                              float *p_out)
+{
+    return guarded([&] {
+        CX_REQUIRE(n >= 1 && n < (int64_t)1 << 31, "n must be in [1, 2^31)");
+        CX_REQUIRE(d >= 1 && m >= 1 && m <= n, "need d >= 1 and 1 <= m <= n");
+        CX_REQUIRE(points && inducing_idx && v_out && p_out, "NULL pointer");
+        for (int32_t j = 0; j < m; j++)
+            CX_REQUIRE(inducing_idx[j] >= 0 && inducing_idx[j] < n,
+                       "inducing index out of range");
+        require_device();
+        nystrom_device(points, n, d, inducing_idx, m, var, ls, n_ls,
+                       kernel_type == CX_KERNEL_MATERN52 ? 1 : 0, jitter, v_out, p_out);
+    });
+}
+
+void cx_tree_free(cx_tree *t) { delete t; }
+
+} // extern "C"

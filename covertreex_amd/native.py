@@ -1,0 +1,169 @@
+"""Python handle over a device-resident tree (``cx_tree*``).
+
+Mirrors the reference's ``CoverTreeWrapper`` surface (src/lib.rs:63-904):
+build via :meth:`HipTree.build_residual` (``build_pcct2_residual_tree``),
+query via :meth:`HipTree.knn_residual` (``knn_query_residual``) and the
+``get_*`` getters.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import check, ptr
+
+MODES = {"reference": _ffi.CX_KNN_REFERENCE_F32, "reference-f32": _ffi.CX_KNN_REFERENCE_F32,
+         "mixed": _ffi.CX_KNN_MIXED}
+
+
+def _f32c(a, ndim):
+    arr = np.asarray(a)
+    if arr.dtype.kind not in "fiu":
+        raise TypeError(f"expected a float32 or float64 {ndim}D array")
+    arr = np.ascontiguousarray(arr, dtype=np.float32)
+    if arr.ndim != ndim:
+        raise TypeError(f"expected a float32 or float64 {ndim}D array")
+    return arr
+
+
+class HipTree:
+    """A cover tree resident in HBM on the current HIP device."""
+
+    def __init__(self, handle: int, n: int):
+        self._h = C.c_void_p(handle)
+        self.n = int(n)
+        self._lock = threading.Lock()
+
+    # -- construction ---------------------------------------------------------
+    @classmethod
+    def build_residual(cls, v_matrix, p_diag, coords, rbf_var: float, rbf_ls, *,
+                       chunk_size: int = 512, batch_order: str = "hilbert",
+                       grid_whiten_scale: float = 1.0, kernel_type: int = 0) -> "HipTree":
+        """Build on the device (build_pcct2_residual_tree, src/lib.rs:1373-1564)."""
+        lib = _ffi.load()
+        v = _f32c(v_matrix, 2)
+        p = _f32c(p_diag, 1)
+        x = _f32c(coords, 2)
+        ls = np.ascontiguousarray(np.atleast_1d(np.asarray(rbf_ls, dtype=np.float32)))
+        n, m = v.shape
+        if p.shape[0] != n or x.shape[0] != n:
+            raise ValueError("v_matrix, p_diag and coords must have the same number of rows")
+        order = (batch_order or "hilbert").lower()
+        if order == "natural":
+            mode = _ffi.CX_ORDER_NATURAL
+        else:  # "hilbert", "hilbert-morton" and anything else -> Morton (src/lib.rs:1404-1411)
+            mode = _ffi.CX_ORDER_HILBERT
+        out = C.c_void_p()
+        check(lib.cx_tree_build_residual(ptr(v), ptr(p), ptr(x), n, m, x.shape[1],
+                                         float(rbf_var), ptr(ls), ls.size, int(kernel_type),
+                                         int(chunk_size or n or 1), mode,
+                                         float(grid_whiten_scale), C.byref(out)))
+        return cls(out.value, n)
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            _ffi.load().cx_tree_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- queries --------------------------------------------------------------
+    def knn_residual(self, query_indices, k: int, *, predecessor_mode: bool = False,
+                     mode: str = "mixed", return_error: bool = False):
+        """kNN for dataset-index queries; (Q,k) int64 indices and float32 distances,
+        ascending, padded with -1 / FLT_MAX (src/lib.rs:907-938)."""
+        q = np.ascontiguousarray(np.asarray(query_indices).reshape(-1), dtype=np.int64)
+        k = int(k)
+        if k < 1:
+            raise ValueError("k must be >= 1")
+        idx = np.empty((q.size, k), np.int64)
+        dist = np.empty((q.size, k), np.float32)
+        err = np.empty((q.size, k), np.float32) if return_error else None
+        with self._lock:
+            check(_ffi.load().cx_tree_knn_residual(self._h, ptr(q), q.size, k,
+                                                    int(bool(predecessor_mode)), MODES[mode],
+                                                    ptr(idx), ptr(dist), ptr(err)))
+        if return_error:
+            return idx, dist, err
+        return idx, dist
+
+    def knn_residual_device(self, d_queries: int, nq: int, k: int, d_idx: int, d_dist: int,
+                            d_err: int | None = None, *, predecessor_mode=False, mode="mixed"):
+        """Device-pointer variant (inputs/outputs already in HBM)."""
+        with self._lock:
+            check(_ffi.load().cx_tree_knn_residual_device(
+                self._h, C.c_void_p(d_queries), int(nq), int(k), int(bool(predecessor_mode)),
+                MODES[mode], C.c_void_p(d_idx), C.c_void_p(d_dist),
+                C.c_void_p(d_err) if d_err else None))
+
+    # -- getters (src/lib.rs:825-904) -------------------------------------------
+    def info(self):
+        n, m, d = C.c_int64(), C.c_int32(), C.c_int32()
+        lo, hi = C.c_int32(), C.c_int32()
+        check(_ffi.load().cx_tree_info(self._h, C.byref(n), C.byref(m), C.byref(d), C.byref(lo),
+                                       C.byref(hi)))
+        return {"n": n.value, "m": m.value, "d": d.value, "min_level": lo.value,
+                "max_level": hi.value}
+
+    def _get(self, fn, dtype):
+        out = np.empty(self.n, dtype)
+        check(getattr(_ffi.load(), fn)(self._h, ptr(out)))
+        return out
+
+    def get_parents(self):
+        return self._get("cx_tree_get_parents", np.int64)
+
+    def get_children(self):
+        return self._get("cx_tree_get_children", np.int64)
+
+    def get_next_node(self):
+        return self._get("cx_tree_get_next", np.int64)
+
+    def get_levels(self):
+        return self._get("cx_tree_get_levels", np.int32)
+
+    def get_si_cache(self):
+        return self._get("cx_tree_get_si_cache", np.float32)
+
+    def get_order(self):
+        return self._get("cx_tree_get_order", np.int64)
+
+    def get_min_level(self):
+        return self.info()["min_level"]
+
+    def get_max_level(self):
+        return self.info()["max_level"]
+
+    def get_subtree_bounds(self):
+        mn = np.empty(self.n, np.int64)
+        mx = np.empty(self.n, np.int64)
+        check(_ffi.load().cx_tree_get_subtree_bounds(self._h, ptr(mn), ptr(mx)))
+        return mn, mx
+
+
+def compute_subtree_bounds(parents, node_to_dataset):
+    """compute_subtree_bounds_py (src/lib.rs:991-1006) on the device."""
+    par = np.ascontiguousarray(parents, dtype=np.int64)
+    n2d = np.ascontiguousarray(node_to_dataset, dtype=np.int64)
+    if par.shape != n2d.shape:
+        raise ValueError("parents and node_to_dataset must have the same length")
+    mn = np.empty(par.size, np.int64)
+    mx = np.empty(par.size, np.int64)
+    check(_ffi.load().cx_subtree_bounds(ptr(par), ptr(n2d), par.size, ptr(mn), ptr(mx)))
+    return mn, mx
+
+
+def hilbert_order(coords):
+    """hilbert_order (src/lib.rs:956-962): Morton permutation for D == 3."""
+    x = _f32c(coords, 2)
+    out = np.empty(x.shape[0], np.int64)
+    check(_ffi.load().cx_morton_order(ptr(x), x.shape[0], x.shape[1], ptr(out)))
+    return out

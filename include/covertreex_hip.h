@@ -1,0 +1,135 @@
+/*
+ * covertreex_hip: MI355X (gfx950) engine for the parallel compressed cover tree
+ * under the residual-correlation metric d = sqrt(1 - |rho|).
+ *
+ * C-ABI boundary.  Plain pointers and sizes only; every entry point returns a
+ * cx_status and never throws.  On failure cx_last_error() returns a
+ * thread-local message.  Host-pointer entry points copy their inputs to HBM;
+ * the tree owns its device-resident, tree-ordered copies until cx_tree_free.
+ * Each tree handle owns one HIP stream and a mutex: calls on one handle are
+ * serialised, different handles may run concurrently.
+ *
+ * Each declaration names the reference interface it replaces (file:line in
+ * the covertreex repository).
+ */
+#ifndef COVERTREEX_HIP_H
+#define COVERTREEX_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t cx_status;
+enum {
+    CX_OK = 0,
+    CX_ERR_INVALID = 1,     /* bad argument (the reference raises PyTypeError / panics) */
+    CX_ERR_HIP = 2,         /* HIP runtime error */
+    CX_ERR_NO_DEVICE = 3,   /* no usable gfx950 device */
+    CX_ERR_UNSUPPORTED = 4, /* shape outside what the kernels were built for */
+};
+
+/* kernel_type, covertreex/kernels.py:59-132 and src/metric.rs:44-57 */
+enum { CX_KERNEL_RBF = 0, CX_KERNEL_MATERN52 = 1 };
+
+/* batch_order, src/lib.rs:1404-1411 */
+enum { CX_ORDER_HILBERT = 0 /* 21-bit Morton on D == 3, identity otherwise */, CX_ORDER_NATURAL = 1 };
+
+/* Query precision modes for cx_tree_knn_residual*. */
+enum {
+    /* The reference's f32 search semantics (single_residual_knn_query,
+     * src/algo.rs:612-1242, empty budget schedule) with its f32 distance
+     * arithmetic: one wavefront per query. */
+    CX_KNN_REFERENCE_F32 = 0,
+    /* Batched mixed-precision engine: query tiles x candidate tiles on MFMA
+     * with BF16 hi/lo V and a certified bound, surviving candidates rescored
+     * in compensated FP32; neighbour sets are exact under the residual metric
+     * evaluated on the stored inputs. */
+    CX_KNN_MIXED = 1,
+};
+
+typedef struct cx_tree cx_tree;
+
+/* Thread-local description of the last failure on this thread. */
+const char *cx_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+cx_status cx_device_count(int32_t *out);
+
+/* Replaces build_pcct2_residual_tree (src/lib.rs:1373-1564), including the
+ * si_cache pass (src/lib.rs:1548-1561) and, when subtree_bounds != 0, the
+ * engine's compute_subtree_bounds_py call (covertreex/engine.py:1038-1043).
+ *   V: n x m float32 row-major (Nystrom factor), p: n residual variances,
+ *   coords: n x d float32, ls: n_ls lengthscales (1 or d values),
+ *   chunk: batch size (reference default 512), order_mode: CX_ORDER_*,
+ *   grid_scale: grid_whiten_scale (reference default 1.0).
+ * The tree is built on the device; node i holds dataset row order[i]. */
+cx_status cx_tree_build_residual(const float *V, const float *p, const float *coords,
+                                 int64_t n, int32_t m, int32_t d, float var,
+                                 const float *ls, int32_t n_ls, int32_t kernel_type,
+                                 int32_t chunk, int32_t order_mode, float grid_scale,
+                                 cx_tree **out);
+
+/* Replaces CoverTreeWrapper.knn_query_residual (src/lib.rs:369-507) +
+ * to_py_arrays (src/lib.rs:907-938).  queries: dataset indices (host).
+ * Outputs (host): out_idx nq x k dataset indices ascending by distance,
+ * padded -1; out_dist nq x k padded FLT_MAX; out_err (optional, may be NULL)
+ * nq x k certified bound on |d - d_exact| for CX_KNN_MIXED (the f32 error
+ * of the reference arithmetic is not bounded: NaN in CX_KNN_REFERENCE_F32).
+ * Out-of-range query indices map to node 0 as the reference does
+ * (src/lib.rs:414-418). */
+cx_status cx_tree_knn_residual(cx_tree *t, const int64_t *queries, int64_t nq, int32_t k,
+                               int32_t predecessor, int32_t mode, int64_t *out_idx,
+                               float *out_dist, float *out_err);
+
+/* Same with device-resident queries and outputs (HBM pointers), enqueued on
+ * the tree's stream and synchronised before returning. */
+cx_status cx_tree_knn_residual_device(cx_tree *t, const int64_t *d_queries, int64_t nq,
+                                      int32_t k, int32_t predecessor, int32_t mode,
+                                      int64_t *d_out_idx, float *d_out_dist,
+                                      float *d_out_err);
+
+/* Tree shape.  Replaces len(node_to_dataset) / get_min_level / get_max_level
+ * (src/lib.rs:889-904). */
+cx_status cx_tree_info(const cx_tree *t, int64_t *n, int32_t *m, int32_t *d,
+                       int32_t *min_level, int32_t *max_level);
+
+/* Host copies of the device tree arrays.  Replace get_parents, get_children,
+ * get_next_node, get_levels (src/lib.rs:825-887) and the node_to_dataset list
+ * returned by build_pcct2_residual_tree (src/lib.rs:1543). */
+cx_status cx_tree_get_parents(const cx_tree *t, int64_t *out);
+cx_status cx_tree_get_children(const cx_tree *t, int64_t *out);
+cx_status cx_tree_get_next(const cx_tree *t, int64_t *out);
+cx_status cx_tree_get_levels(const cx_tree *t, int32_t *out);
+cx_status cx_tree_get_si_cache(const cx_tree *t, float *out);
+cx_status cx_tree_get_order(const cx_tree *t, int64_t *out);
+cx_status cx_tree_get_subtree_bounds(const cx_tree *t, int64_t *min_out, int64_t *max_out);
+
+/* Replaces compute_subtree_bounds_py (src/lib.rs:991-1006 -> src/tree.rs:144-185):
+ * min/max node_to_dataset value over each node's subtree, on the device. */
+cx_status cx_subtree_bounds(const int64_t *parents, const int64_t *node_to_dataset, int64_t n,
+                            int64_t *min_out, int64_t *max_out);
+
+/* Replaces hilbert_order (src/lib.rs:956-962 -> src/pcct.rs:27-67): the 21-bit
+ * Morton permutation for d == 3, identity otherwise, computed on the device. */
+cx_status cx_morton_order(const float *coords, int64_t n, int32_t d, int64_t *order_out);
+
+/* Replaces build_residual_backend's numeric core
+ * (covertreex/metrics/residual/host_backend.py:201-240): FP64 kernel
+ * matrices, Cholesky of K_mm + jitter*var*I, V = (L^-1 K_xm^T)^T and
+ * p = max(var - |V|^2, 1e-9), all on the device, stored as FP32.
+ *   points: n x d float64, inducing_idx: m sorted dataset indices,
+ *   ls: n_ls lengthscales (1 or d), v_out: n x m float32, p_out: n float32.
+ * Returns CX_ERR_INVALID when K_mm + jitter is not positive definite. */
+cx_status cx_nystrom_residual(const double *points, int64_t n, int32_t d,
+                              const int64_t *inducing_idx, int32_t m, double var,
+                              const double *ls, int32_t n_ls, int32_t kernel_type, double jitter,
+                              float *v_out, float *p_out);
+
+void cx_tree_free(cx_tree *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

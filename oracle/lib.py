@@ -1,0 +1,340 @@
+"""ctypes bindings for the C oracle (TEST INFRASTRUCTURE, see ``oracle/__init__``)."""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+EXP_LIBM = 0
+EXP_CR = 1
+
+_p = C.c_void_p
+_i64 = C.c_int64
+_i32 = C.c_int32
+_f32 = C.c_float
+
+
+def build_library(force: bool = False) -> str:
+    """Compile ``liboracle.so`` with the oracle Makefile (CPU only)."""
+    if force or not os.path.exists(_LIB_PATH):
+        cmd = ["make", "-s", "-C", _HERE] + (["-B"] if force else [])
+        subprocess.run(cmd, check=True, capture_output=True)
+    return _LIB_PATH
+
+
+class _QueryOpts(C.Structure):
+    _fields_ = [
+        ("n_budget", _i32),
+        ("budget", _i64 * 8),
+        ("budget_up", C.c_double),
+        ("budget_down", C.c_double),
+        ("stream_tile", _i32),
+        ("dynamic_block", _i32),
+        ("radius_floor", _f32),
+        ("cap_default", _f32),
+        ("threads", _i32),
+    ]
+
+
+class _Tree(C.Structure):
+    _fields_ = [
+        ("n", _i64),
+        ("cap", _i64),
+        ("parents", C.POINTER(_i64)),
+        ("children", C.POINTER(_i64)),
+        ("next", C.POINTER(_i64)),
+        ("levels", C.POINTER(_i32)),
+        ("si", C.POINTER(_f32)),
+        ("min_level", _i32),
+        ("max_level", _i32),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build_library()
+        L = C.CDLL(_LIB_PATH)
+        L.ox_metric_residual.restype = _p
+        L.ox_metric_residual.argtypes = [_i64, _i32, _i32, _p, _p, _p, _f32, _p, _i32, _i32, _i32]
+        L.ox_metric_euclidean.restype = _p
+        L.ox_metric_euclidean.argtypes = [_i64, _i32, _p]
+        L.ox_metric_free.argtypes = [_p]
+        L.ox_expf.restype = _f32
+        L.ox_expf.argtypes = [_i32, _f32]
+        L.ox_dist_sq_pairs.argtypes = [_p, _p, _p, _i64, _p]
+        L.ox_dist_batch_pairs.argtypes = [_p, _p, _p, _i64, _p]
+        L.ox_morton_order.argtypes = [_i64, _i32, _p, _p, _p]
+        L.ox_hilbert_codes.argtypes = [_i64, _i32, _p, _p]
+        L.ox_tree_new.restype = C.POINTER(_Tree)
+        L.ox_tree_new.argtypes = [_i64]
+        L.ox_tree_free.argtypes = [C.POINTER(_Tree)]
+        L.ox_build.argtypes = [C.POINTER(_Tree), _p, _i64, _p, _i32, _f32, _i32, _p]
+        L.ox_si_cache.argtypes = [C.POINTER(_Tree), _p, _p]
+        L.ox_subtree_bounds.argtypes = [_i64, _p, _p, _p, _p]
+        L.ox_greedy_mis.argtypes = [_i64, _p, _p, _p]
+        L.ox_luby_mis.restype = _i64
+        L.ox_luby_mis.argtypes = [_i64, _p, _p, C.c_uint32, _p]
+        L.ox_grid_edges_dense.argtypes = [_i64, _i32, _p, _f32, _p]
+        L.ox_query_opts_default.argtypes = [C.POINTER(_QueryOpts)]
+        L.ox_knn_residual.argtypes = [C.POINTER(_Tree), _p, _p, _p, _p, _i64, _i32, _i32, _p,
+                                      C.POINTER(_QueryOpts), _p, _p, _p]
+        L.ox_knn_euclidean.argtypes = [C.POINTER(_Tree), _p, _p, _i64, _i32, _i32, _p, _p, _i32]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class _Metric:
+    """Owns an ``ox_metric`` over tree-ordered rows."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ox_metric_free(self.h)
+            self.h = None
+
+
+def residual_metric(coords, v, p, var, ls, kernel_type, exp_mode=EXP_LIBM) -> _Metric:
+    coords = _c(coords, np.float32)
+    v = _c(v, np.float32)
+    p = _c(p, np.float32)
+    ls = _c(np.atleast_1d(ls), np.float32)
+    n, d = coords.shape
+    m = v.shape[1]
+    h = lib().ox_metric_residual(n, m, d, _ptr(coords), _ptr(v), _ptr(p), float(var), _ptr(ls),
+                                 ls.size, int(kernel_type), int(exp_mode))
+    return _Metric(h)
+
+
+def euclidean_metric(points) -> _Metric:
+    pts = _c(points, np.float32)
+    return _Metric(lib().ox_metric_euclidean(pts.shape[0], pts.shape[1], _ptr(pts)))
+
+
+def expf(x: float, mode: int = EXP_LIBM) -> float:
+    return float(lib().ox_expf(mode, float(x)))
+
+
+def dist_sq_pairs(metric: _Metric, a, b) -> np.ndarray:
+    a = _c(a, np.int64)
+    b = _c(b, np.int64)
+    out = np.empty(a.size, np.float32)
+    lib().ox_dist_sq_pairs(metric.h, _ptr(a), _ptr(b), a.size, _ptr(out))
+    return out
+
+
+def dist_batch_pairs(metric: _Metric, a, b) -> np.ndarray:
+    a = _c(a, np.int64)
+    b = _c(b, np.int64)
+    out = np.empty(a.size, np.float32)
+    lib().ox_dist_batch_pairs(metric.h, _ptr(a), _ptr(b), a.size, _ptr(out))
+    return out
+
+
+def morton_order(coords) -> tuple[np.ndarray, np.ndarray]:
+    """hilbert_like_order (src/pcct.rs:27-67): permutation and sorted codes."""
+    coords = _c(coords, np.float32)
+    n = coords.shape[0]
+    d = coords.shape[1] if coords.ndim == 2 else 1
+    order = np.empty(n, np.int64)
+    codes = np.empty(n, np.uint64)
+    lib().ox_morton_order(n, d, _ptr(coords), _ptr(order), _ptr(codes))
+    return order, codes
+
+
+def hilbert_codes(points) -> np.ndarray:
+    pts = _c(points, np.float64)
+    out = np.empty(pts.shape[0], np.uint64)
+    lib().ox_hilbert_codes(pts.shape[0], pts.shape[1], _ptr(pts), _ptr(out))
+    return out
+
+
+def hilbert_order(points) -> np.ndarray:
+    """_hilbert_permutation (covertreex/algo/order/strategy.py:38-60)."""
+    codes = hilbert_codes(points)
+    n = codes.shape[0]
+    return np.lexsort((np.arange(n, dtype=np.int64), codes)).astype(np.int64)
+
+
+def greedy_mis(indptr, indices) -> np.ndarray:
+    indptr = _c(indptr, np.int64)
+    indices = _c(indices, np.int64)
+    n = indptr.size - 1
+    out = np.zeros(max(n, 1), np.uint8)
+    lib().ox_greedy_mis(n, _ptr(indptr), _ptr(indices), _ptr(out))
+    return out[:n].astype(bool)
+
+
+def luby_mis(indptr, indices, seed: int) -> tuple[np.ndarray, int]:
+    indptr = _c(indptr, np.int64)
+    indices = _c(indices, np.int64)
+    n = indptr.size - 1
+    out = np.zeros(max(n, 1), np.uint8)
+    it = lib().ox_luby_mis(n, _ptr(indptr), _ptr(indices), int(seed) & 0xFFFFFFFF, _ptr(out))
+    return out[:n].astype(bool), int(it)
+
+
+def grid_adjacency(coords, scale: float = 1.0) -> np.ndarray:
+    coords = _c(coords, np.float32)
+    n, d = coords.shape
+    adj = np.zeros((n, n), np.uint8)
+    lib().ox_grid_edges_dense(n, d, _ptr(coords), float(scale), _ptr(adj))
+    return adj.astype(bool)
+
+
+def subtree_bounds(parents, n2d) -> tuple[np.ndarray, np.ndarray]:
+    parents = _c(parents, np.int64)
+    n2d = _c(n2d, np.int64)
+    n = parents.size
+    mn = np.empty(n, np.int64)
+    mx = np.empty(n, np.int64)
+    lib().ox_subtree_bounds(n, _ptr(parents), _ptr(n2d), _ptr(mn), _ptr(mx))
+    return mn, mx
+
+
+@dataclass
+class OracleTree:
+    """A tree built by the oracle on tree-ordered (Morton) rows."""
+
+    order: np.ndarray            # node -> dataset (node_to_dataset)
+    inv_order: np.ndarray        # dataset -> node
+    parents: np.ndarray
+    children: np.ndarray
+    next: np.ndarray
+    levels: np.ndarray
+    si: np.ndarray
+    min_level: int
+    max_level: int
+    subtree_min: np.ndarray
+    subtree_max: np.ndarray
+    stats: dict = field(default_factory=dict)
+    _tree: object = None
+    _metric: _Metric | None = None
+
+    def __del__(self):
+        if self._tree is not None and _lib is not None:
+            _lib.ox_tree_free(self._tree)
+            self._tree = None
+
+
+def _tree_arrays(t, n):
+    tt = t.contents
+    cp = lambda ptr, dt: np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True)  # noqa: E731
+    return (cp(tt.parents, np.int64), cp(tt.children, np.int64), cp(tt.next, np.int64),
+            cp(tt.levels, np.int32), cp(tt.si, np.float32), int(tt.min_level), int(tt.max_level))
+
+
+def _build(metric: _Metric, n: int, order, chunk, grid_coords, grid_scale, use_grid):
+    L = lib()
+    t = L.ox_tree_new(max(n, 1))
+    stats = np.zeros(8, np.int64)
+    gc = None if grid_coords is None else _c(grid_coords, np.float32)
+    gd = 0 if gc is None else gc.shape[1]
+    L.ox_build(t, metric.h, int(chunk), _ptr(gc), gd, float(grid_scale), int(use_grid), _ptr(stats))
+    si = np.zeros(max(n, 1), np.float32)
+    L.ox_si_cache(t, metric.h, _ptr(si))
+    parents, children, nxt, levels, _, mn, mx = _tree_arrays(t, n)
+    # the reference stores the si_cache back into the tree (src/lib.rs:1548-1561)
+    C.memmove(t.contents.si, si.ctypes.data, 4 * n)
+    inv = np.empty(n, np.int64)
+    inv[order] = np.arange(n, dtype=np.int64)
+    smin, smax = subtree_bounds(parents, order)
+    keys = ["filter_evals", "conflict_pairs", "nonleader_evals", "leaders", "orphans_dropped",
+            "far_levels"]
+    return OracleTree(order=order, inv_order=inv, parents=parents, children=children, next=nxt,
+                      levels=levels, si=si[:n].copy(), min_level=mn, max_level=mx,
+                      subtree_min=smin, subtree_max=smax,
+                      stats={k: int(v) for k, v in zip(keys, stats)}, _tree=t, _metric=metric)
+
+
+def build_residual_tree(coords, v, p, *, var=1.0, ls=1.0, kernel_type=0, chunk=512,
+                        batch_order="hilbert", grid_scale=1.0, exp_mode=EXP_LIBM) -> OracleTree:
+    """build_pcct2_residual_tree (src/lib.rs:1373-1564) + compute_subtree_bounds_py."""
+    coords = _c(coords, np.float32)
+    n = coords.shape[0]
+    if batch_order == "natural":
+        order = np.arange(n, dtype=np.int64)
+    else:
+        order, _ = morton_order(coords)
+    c_ord = coords[order]
+    metric = residual_metric(c_ord, np.asarray(v, np.float32)[order], np.asarray(p, np.float32)[order],
+                             var, ls, kernel_type, exp_mode)
+    return _build(metric, n, order, chunk, c_ord, grid_scale, 1)
+
+
+def build_euclidean_tree(points, *, chunk=512) -> OracleTree:
+    """Chunked batch_insert_collect with the Euclidean metric in natural order."""
+    pts = _c(points, np.float32)
+    n = pts.shape[0]
+    order = np.arange(n, dtype=np.int64)
+    metric = euclidean_metric(pts)
+    return _build(metric, n, order, chunk, None, 1.0, 0)
+
+
+def si_cache(tree: OracleTree) -> np.ndarray:
+    si = np.zeros(max(tree.parents.size, 1), np.float32)
+    lib().ox_si_cache(tree._tree, tree._metric.h, _ptr(si))
+    return si[: tree.parents.size]
+
+
+def _opts(budget, threads):
+    o = _QueryOpts()
+    lib().ox_query_opts_default(C.byref(o))
+    if budget is not None and budget != "default":
+        sched = list(budget)
+        o.n_budget = len(sched)
+        for i, b in enumerate(sched[:8]):
+            o.budget[i] = int(b)
+    o.threads = int(threads)
+    return o
+
+
+def knn_residual(tree: OracleTree, queries, k: int, *, predecessor=False, budget=(),
+                 threads: int = 0, use_bounds=True):
+    """CoverTreeWrapper.knn_query_residual cached path (src/lib.rs:369-507).
+
+    ``budget=()`` is the exact search (empty schedule); ``budget="default"``
+    the reference default ladder [32,64,96]."""
+    q = _c(np.asarray(queries).reshape(-1), np.int64)
+    n = tree.order.size
+    q_tree = np.where((q >= 0) & (q < n), tree.inv_order[np.clip(q, 0, max(n - 1, 0))], 0)
+    q_tree = _c(q_tree, np.int64)
+    nq = q.size
+    out_i = np.empty((nq, k), np.int64)
+    out_d = np.empty((nq, k), np.float32)
+    evals = np.zeros(nq, np.int64)
+    o = _opts(budget, threads)
+    smin = tree.subtree_min if (predecessor and use_bounds) else None
+    lib().ox_knn_residual(tree._tree, tree._metric.h, _ptr(tree.order), _ptr(q_tree), _ptr(q),
+                          nq, int(k), int(bool(predecessor)), _ptr(smin), C.byref(o),
+                          _ptr(out_i), _ptr(out_d), _ptr(evals))
+    return out_i, out_d, evals
+
+
+def knn_euclidean(tree: OracleTree, queries, k: int, *, predecessor=False, threads=0):
+    """batch_knn_query / single_knn_query (src/algo.rs:179-323)."""
+    qp = _c(queries, np.float32)
+    nq = qp.shape[0]
+    out_i = np.empty((nq, k), np.int64)
+    out_d = np.empty((nq, k), np.float32)
+    lib().ox_knn_euclidean(tree._tree, tree._metric.h, _ptr(qp), nq, int(k),
+                           int(bool(predecessor)), _ptr(out_i), _ptr(out_d), int(threads))
+    return out_i, out_d
