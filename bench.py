@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark: residual kNN queries/s at N=32768, D=3, k=50, predecessor mode.
+
+Workload (one step): the Vecchia-shape residual kNN over every point of a
+32,768-point, 3-D N(0,1) dataset with a Matern-5/2 (lengthscale 1) Nystrom
+factor of rank 256, k=50, neighbours j < i, answered by the mixed-precision
+engine from inputs already resident in HBM.  Beside it, in the same run: the
+plain FP32 GPU form (the reference's search semantics and f32 arithmetic, one
+wavefront per query) and the C oracle's restatement of the reference search on
+the host cores over a bounded query sample.
+
+Prints ONE JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N, D, M, K = 32768, 3, 256, 50
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+HBM_PEAK_GBS = 8000.0
+
+
+def make_inputs():
+    from covertreex_amd.nystrom import nystrom_device
+    from covertreex_amd.residual import inducing_indices
+
+    x = np.random.default_rng(0).standard_normal((N, D)).astype(np.float32)
+    ind = inducing_indices(N, M, 0)
+    v, p = nystrom_device(x.astype(np.float64), ind, variance=1.0, lengthscale=1.0,
+                          kernel_type=1)
+    return x, v, p
+
+
+def cpu_baseline(x, v, p, sample_q: int):
+    """The oracle's restatement of the reference search (exact semantics) on
+    the host cores over a bounded sample of queries; build excluded."""
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    t = oracle.build_residual_tree(x, v, p, kernel_type=1)
+    q = np.random.default_rng(3).choice(N, sample_q, replace=False)
+    t0 = time.perf_counter()
+    _, _, evals = oracle.knn_residual(t, q, K, predecessor=True, budget=(), threads=threads)
+    el = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.knn_residual(t, q, K, predecessor=True, budget="default", threads=threads)
+    el_b = time.perf_counter() - t0
+    return {"value": round(sample_q / el, 1), "unit": "queries/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{sample_q} random queries of the same workload, exact search "
+                      f"(empty budget schedule), {threads} OpenMP threads",
+            "budgeted_default_qps": round(sample_q / el_b, 1),
+            "mean_evals_per_query": float(evals.mean())}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from covertreex_amd.native import HipTree
+
+    dev = torch.device("cuda:0")
+    x, v, p = make_inputs()
+    t0 = time.perf_counter()
+    tree = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
+    build_s = time.perf_counter() - t0
+
+    q = torch.arange(N, dtype=torch.int64, device=dev)
+    o_idx = torch.empty((N, K), dtype=torch.int64, device=dev)
+    o_dst = torch.empty((N, K), dtype=torch.float32, device=dev)
+    o_err = torch.empty((N, K), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    def step(mode="mixed"):
+        tree.knn_residual_device(q.data_ptr(), N, K, o_idx.data_ptr(), o_dst.data_ptr(),
+                                 o_err.data_ptr(), predecessor_mode=True, mode=mode)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        step()
+        lat.append(time.perf_counter() - s0)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    qps = N * args.steps / elapsed
+
+    # per-phase device timing on the tree's own stream (HIP events)
+    tree.set_profiling(True)
+    prof = []
+    for _ in range(3):
+        step()
+        prof.append(tree.last_profile())
+    tree.set_profiling(False)
+    prof = np.array(prof)
+    phases = dict(zip(HipTree.PHASES["mixed"], np.median(prof, axis=0).tolist()))
+
+    # plain FP32 GPU form (the denominator of the speedup)
+    step("reference")
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        step("reference")
+    torch.cuda.synchronize()
+    plain_qps = N * args.plain_steps / (time.perf_counter() - s0)
+
+    cpu = None
+    evals = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(x, v, p, args.cpu_sample)
+        evals = cpu.get("mean_evals_per_query")
+    # algorithmic work per query row: E_q (x) (2m + 2D + 20) flops with E_q the
+    # reference exact search's distance evaluations on this tree
+    e_q = evals if evals else float(N)
+    flops_per_q = e_q * (2 * M + 2 * D + 20)
+    screen_ms = phases["screen_pass1"]
+    achieved = N * flops_per_q / (screen_ms * 1e-3) / 1e12
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": "residual kNN queries/s @N=32k,D=3,k=50 vs plain GPU; p50/p99 serve latency",
+        "value": round(qps, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16-hilo-mfma/f32-compensated",
+        "data": "synthetic",
+        "config": {"workload": "vecchia residual kNN: N=32768 D=3 N(0,1) Matern52 l=1 "
+                               "Nystrom m=256 k=50 predecessor all queries",
+                   "n": N, "d": D, "m": M, "k": K, "predecessor": True},
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "plain_f32_gpu_qps": round(plain_qps, 1),
+        "speedup_vs_plain_f32_gpu": round(qps / plain_qps, 2),
+        "build_s": round(build_s, 3),
+        "phases_ms": {k_: round(v_, 4) for k_, v_ in phases.items()},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2),
+                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "screen_kernel<1>",
+                     "algorithmic_flops_per_launch": N * flops_per_q},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

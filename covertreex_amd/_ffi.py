@@ -52,6 +52,8 @@ SIGNATURES = {
     "cx_morton_order": (_i32, [_p, _i64, _i32, _p]),
     "cx_nystrom_residual": (_i32, [_p, _i64, _i32, _p, _i32, C.c_double, _p, _i32, _i32,
                                    C.c_double, _p, _p]),
+    "cx_tree_set_profiling": (_i32, [_p, _i32]),
+    "cx_tree_last_profile": (_i32, [_p, _p, _i32, _p]),
     "cx_tree_free": (None, [_p]),
 }
 

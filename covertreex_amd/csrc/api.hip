@@ -37,11 +37,25 @@ TreeState::~TreeState()
     vhi.release();
     vlo.release();
     vnorm.release();
-    xnorm.release();
-    si_acc.release();
+    x_ds.release();
+    p_ds.release();
     work.release();
+    for (int i = 0; i < kMaxMarks; i++)
+        if (marks[i])
+            (void)hipEventDestroy(marks[i]);
     if (stream)
         (void)hipStreamDestroy(stream);
+}
+
+void prof_mark(TreeState &t, int i)
+{
+    if (!t.profiling || i < 0 || i >= TreeState::kMaxMarks)
+        return;
+    if (!t.marks[i])
+        CX_HIP(hipEventCreate(&t.marks[i]));
+    CX_HIP(hipEventRecord(t.marks[i], t.stream));
+    if (i + 1 > t.n_marks)
+        t.n_marks = i + 1;
 }
 
 uint8_t *workspace(TreeState &t, size_t bytes)
@@ -134,9 +148,11 @@ cx_status cx_tree_build_residual(const float *V, const float *p, const float *co
             // (ResidualMetric::new, src/metric.rs:93-101)
             const float fallback = (ls && n_ls > 0) ? ls[0] : 1.0f;
             t.inv_ls.resize(d);
+            t.ls.resize(d);
             for (int32_t c = 0; c < d; c++) {
                 float raw = (ls && c < n_ls) ? ls[c] : fallback;
                 float safe = std::fabs(raw) < 1e-6f ? 1e-6f : raw;
+                t.ls[c] = safe;
                 t.inv_ls[c] = 1.0f / safe;
             }
             if (n > 0)
@@ -162,6 +178,8 @@ cx_status cx_tree_knn_residual_device(cx_tree *t, const int64_t *d_queries, int6
         CX_HIP(hipSetDevice(t->s.device));
         if (nq == 0)
             return;
+        t->s.n_marks = 0;
+        t->s.last_mode = mode;
         if (mode == CX_KNN_REFERENCE_F32)
             knn_reference_f32(t->s, d_queries, nq, k, predecessor, d_out_idx, d_out_dist,
                               d_out_err);
@@ -288,6 +306,32 @@ cx_status cx_nystrom_residual(const double *points, int64_t n, int32_t d,
         require_device();
         nystrom_device(points, n, d, inducing_idx, m, var, ls, n_ls,
                        kernel_type == CX_KERNEL_MATERN52 ? 1 : 0, jitter, v_out, p_out);
+    });
+}
+
+cx_status cx_tree_set_profiling(cx_tree *t, int32_t enable)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        std::lock_guard<std::mutex> lock(t->s.mu);
+        t->s.profiling = enable != 0;
+        t->s.n_marks = 0;
+    });
+}
+
+cx_status cx_tree_last_profile(cx_tree *t, float *ms_out, int32_t cap, int32_t *n_out)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr && ms_out != nullptr && n_out != nullptr, "NULL pointer");
+        std::lock_guard<std::mutex> lock(t->s.mu);
+        TreeState &s = t->s;
+        const int phases = s.n_marks > 1 ? s.n_marks - 1 : 0;
+        *n_out = phases;
+        for (int i = 0; i < phases && i < cap; i++) {
+            float ms = 0.0f;
+            CX_HIP(hipEventElapsedTime(&ms, s.marks[i], s.marks[i + 1]));
+            ms_out[i] = ms;
+        }
     });
 }
 

@@ -88,7 +88,7 @@ struct TreeState {
     float var = 1.0f;
     int32_t kernel = 0;
     int32_t min_level = -20, max_level = 20;
-    std::vector<float> inv_ls;
+    std::vector<float> inv_ls, ls;
 
     // metric (tree order)
     DevBuf<float> v, scaled, raw, norms, p;
@@ -111,11 +111,19 @@ struct TreeState {
     bool mixed_ready = false;
     DevBuf<uint16_t> vhi, vlo;     // n x mp bf16, zero padded to mp
     int32_t mp = 0;                // m padded to a multiple of 32
-    DevBuf<float> vnorm, xnorm;    // |V_i|, |x_i| (scaled coords)
-    DevBuf<float> si_acc;          // certified cover radii under the accurate metric
+    DevBuf<float> vnorm;           // |V_i| rounded up, dataset order
+    DevBuf<float> x_ds, p_ds;      // raw coords and p in dataset order
 
     // reusable query workspace
     DevBuf<uint8_t> work;
+
+    // optional per-phase timing of the last query: events recorded on the
+    // tree's stream at phase boundaries (see cx_tree_last_profile)
+    static constexpr int kMaxMarks = 8;
+    bool profiling = false;
+    hipEvent_t marks[kMaxMarks] = {};
+    int n_marks = 0;
+    int last_mode = -1;
 
     ~TreeState();
 };
@@ -141,6 +149,9 @@ void nystrom_device(const double *pts_host, int64_t n, int32_t d, const int64_t 
                     double jitter, float *v_host, float *p_host);
 
 uint8_t *workspace(TreeState &t, size_t bytes);
+
+// Record phase boundary `i` on the tree's stream when profiling is enabled.
+void prof_mark(TreeState &t, int i);
 
 } // namespace cx
 

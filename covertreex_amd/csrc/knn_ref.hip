@@ -359,6 +359,7 @@ void knn_reference_f32(TreeState &t, const int64_t *d_queries, int64_t nq, int32
     int32_t *ws_node = (int32_t *)(ws + 256);
     float *ws_val = (float *)(ws + 256 + (size_t)waves * per_wave * sizeof(int32_t));
     CX_HIP(hipMemsetAsync(counter, 0, sizeof(int32_t), st));
+    prof_mark(t, 0);
     MetricView mv{t.scaled.ptr, t.norms.ptr, t.v.ptr, t.p.ptr, t.d, t.m, t.var, t.kernel};
     RefTree tr{t.order.ptr, t.inv_order.ptr,  t.roots.ptr, t.n_roots, t.child_off.ptr,
                t.child_list.ptr, t.si.ptr, t.smin.ptr, t.n};
@@ -366,6 +367,7 @@ void knn_reference_f32(TreeState &t, const int64_t *d_queries, int64_t nq, int32
                                                    d_dist, d_err, counter, ws_node, ws_val,
                                                    fr_cap, ch_cap);
     CX_HIP(hipGetLastError());
+    prof_mark(t, 1);
 }
 
 } // namespace cx

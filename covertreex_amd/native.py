@@ -104,6 +104,19 @@ class HipTree:
                 MODES[mode], C.c_void_p(d_idx), C.c_void_p(d_dist),
                 C.c_void_p(d_err) if d_err else None))
 
+    # -- telemetry ------------------------------------------------------------------
+    PHASES = {"mixed": ["prep", "query_sort", "screen_pass1", "tau", "screen_pass2", "rescore"],
+              "reference": ["traversal"]}
+
+    def set_profiling(self, enable: bool = True) -> None:
+        check(_ffi.load().cx_tree_set_profiling(self._h, int(bool(enable))))
+
+    def last_profile(self) -> list:
+        out = np.zeros(8, np.float32)
+        n = C.c_int32(0)
+        check(_ffi.load().cx_tree_last_profile(self._h, ptr(out), out.size, C.byref(n)))
+        return [float(x) for x in out[: n.value]]
+
     # -- getters (src/lib.rs:825-904) -------------------------------------------
     def info(self):
         n, m, d = C.c_int64(), C.c_int32(), C.c_int32()

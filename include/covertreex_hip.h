@@ -127,6 +127,15 @@ cx_status cx_nystrom_residual(const double *points, int64_t n, int32_t d,
                               const double *ls, int32_t n_ls, int32_t kernel_type, double jitter,
                               float *v_out, float *p_out);
 
+/* Per-phase device timing of the last kNN call on this tree (telemetry, the
+ * analogue of the reference's query telemetry, src/telemetry.rs:1-60).
+ * When enabled, HIP events are recorded on the tree's stream at phase
+ * boundaries.  cx_tree_last_profile writes up to cap durations (ms):
+ *   CX_KNN_MIXED:         [prep, query sort, screen pass 1, tau, screen pass 2, rescore]
+ *   CX_KNN_REFERENCE_F32: [traversal]                                         */
+cx_status cx_tree_set_profiling(cx_tree *t, int32_t enable);
+cx_status cx_tree_last_profile(cx_tree *t, float *ms_out, int32_t cap, int32_t *n_out);
+
 void cx_tree_free(cx_tree *t);
 
 #ifdef __cplusplus

@@ -1,0 +1,112 @@
+"""Device tree build and reference-form query vs the oracle (bit for bit).
+
+The oracle runs with OX_EXP_CR, the double-evaluated expf the device kernels
+use; the oracle's libm mode (the reference on this platform) is compared with
+it in tests/test_oracle_tree.py."""
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import fp64
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(n, d, m, kt, seed=0, dist="normal"):
+    rng = np.random.default_rng(seed)
+    if dist == "normal":
+        x = rng.standard_normal((n, d)).astype(np.float32)
+    else:
+        x = rng.uniform(-3, 3, (n, d)).astype(np.float32)
+    v, p, *_ = fp64.nystrom(x, inducing_count=m, seed=0, kernel_type=kt)
+    return x, v, p
+
+
+BUILD_CASES = [
+    # C1 shape: N=4096, D=3, Matern, m=64
+    dict(n=4096, d=3, m=64, kt=1, chunk=512, gs=1.0),
+    dict(n=2000, d=2, m=32, kt=0, chunk=64, gs=8.0),
+    dict(n=3000, d=5, m=48, kt=0, chunk=256, gs=1.0, dist="uniform"),
+    dict(n=1500, d=3, m=40, kt=1, chunk=1024, gs=4.0),
+    dict(n=1, d=3, m=1, kt=1, chunk=512, gs=1.0),
+    dict(n=513, d=3, m=16, kt=0, chunk=512, gs=1.0),
+]
+
+
+def _tree_pair(hip, case, seed=0):
+    from covertreex_amd.native import HipTree
+
+    x, v, p = _data(case["n"], case["d"], case["m"], case["kt"], seed=seed,
+                    dist=case.get("dist", "normal"))
+    ot = oracle.build_residual_tree(x, v, p, kernel_type=case["kt"], chunk=case["chunk"],
+                                    grid_scale=case["gs"], exp_mode=oracle.EXP_CR)
+    ht = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32),
+                                chunk_size=case["chunk"], grid_whiten_scale=case["gs"],
+                                kernel_type=case["kt"])
+    return x, v, p, ot, ht
+
+
+@pytest.mark.parametrize("case", BUILD_CASES)
+def test_device_tree_bit_exact(hip, case):
+    print(f"case {case}", flush=True)
+    x, v, p, ot, ht = _tree_pair(hip, case)
+    assert np.array_equal(ht.get_order(), ot.order)
+    assert np.array_equal(ht.get_parents(), ot.parents)
+    assert np.array_equal(ht.get_children(), ot.children)
+    assert np.array_equal(ht.get_next_node(), ot.next)
+    assert np.array_equal(ht.get_levels(), ot.levels)
+    assert np.array_equal(ht.get_si_cache().view(np.uint32), ot.si.view(np.uint32))
+    mn, mx = ht.get_subtree_bounds()
+    assert np.array_equal(mn, ot.subtree_min) and np.array_equal(mx, ot.subtree_max)
+    info = ht.info()
+    assert (info["min_level"], info["max_level"]) == (ot.min_level, ot.max_level)
+
+
+@pytest.mark.parametrize("case,k,pred", [
+    (BUILD_CASES[0], 16, False),   # C1
+    (BUILD_CASES[0], 16, True),
+    (BUILD_CASES[1], 10, True),    # proper tree: pruning active
+    (BUILD_CASES[1], 7, False),
+    (BUILD_CASES[2], 30, True),
+    (BUILD_CASES[3], 64, True),
+])
+def test_reference_form_knn_matches_oracle(hip, case, k, pred):
+    print(f"case {case} k={k} pred={pred}", flush=True)
+    x, v, p, ot, ht = _tree_pair(hip, case)
+    q = np.arange(case["n"])
+    oi, od, _ = oracle.knn_residual(ot, q, k, predecessor=pred, budget=())
+    hi, hd = ht.knn_residual(q, k, predecessor_mode=pred, mode="reference")
+    # distances bit-exact; index sets equal (order may differ only within
+    # exactly equal f32 distances)
+    assert np.array_equal(hd.view(np.uint32), od.view(np.uint32))
+    for r in range(q.size):
+        if not np.array_equal(hi[r], oi[r]):
+            for dv in np.unique(hd[r]):
+                assert set(hi[r][hd[r] == dv]) == set(oi[r][od[r] == dv])
+
+
+def test_reference_form_edge_queries(hip):
+    case = BUILD_CASES[1]
+    x, v, p, ot, ht = _tree_pair(hip, case, seed=3)
+    q = np.array([0, 1, 2, case["n"] - 1, 5, 5])
+    oi, od, _ = oracle.knn_residual(ot, q, 12, predecessor=True, budget=())
+    hi, hd = ht.knn_residual(q, 12, predecessor_mode=True, mode="reference")
+    assert np.array_equal(hd.view(np.uint32), od.view(np.uint32))
+    assert np.all(hi[0] == -1) and np.all(hd[0] == np.finfo(np.float32).max)
+    assert set(hi[1][hi[1] >= 0]) == {0}
+
+
+def test_c2_shape_build_and_reference_queries(hip):
+    # N=32768, D=3, Matern, m=256: full build bit-exact, a query sample bit-exact
+    case = dict(n=32768, d=3, m=256, kt=1, chunk=512, gs=1.0)
+    print("C2 shape", flush=True)
+    x, v, p, ot, ht = _tree_pair(hip, case)
+    assert np.array_equal(ht.get_parents(), ot.parents)
+    assert np.array_equal(ht.get_next_node(), ot.next)
+    assert np.array_equal(ht.get_levels(), ot.levels)
+    assert np.array_equal(ht.get_si_cache().view(np.uint32), ot.si.view(np.uint32))
+    q = np.random.default_rng(7).choice(32768, 256, replace=False)
+    oi, od, _ = oracle.knn_residual(ot, q, 50, predecessor=True, budget=())
+    hi, hd = ht.knn_residual(q, 50, predecessor_mode=True, mode="reference")
+    assert np.array_equal(hd.view(np.uint32), od.view(np.uint32))

@@ -1,0 +1,117 @@
+"""Mixed-precision engine vs the FP64 brute-force oracle on the same stored
+inputs: neighbour sets equal except at distance ties within 1e-6, distances
+within 1e-5 of FP64, and every reported error bound covers the actual error."""
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import fp64
+
+pytestmark = pytest.mark.gpu
+
+TIE = 1e-6
+DIST_TOL = 1e-5
+
+
+def _check(idx, dist, err, x, v, p, q, k, pred, kt, var=1.0, ls=1.0):
+    bi, bd, bn = fp64.brute_knn_fp64(x, v, p, q, k, predecessor=pred, kernel_type=kt,
+                                     variance=var, lengthscale=ls)
+    worst = 0.0
+    for r in range(q.size):
+        got = idx[r][idx[r] >= 0]
+        ref = bi[r][bi[r] >= 0]
+        assert got.size == ref.size, f"row {r}: {got.size} vs {ref.size} neighbours"
+        if got.size == 0:
+            continue
+        d64 = fp64.residual_dist_fp64(x, v, p, [q[r]], got, kernel_type=kt, variance=var,
+                                      lengthscale=ls)[0]
+        e = np.abs(dist[r][: got.size].astype(np.float64) - d64)
+        worst = max(worst, float(e.max()))
+        assert np.all(e <= DIST_TOL), f"row {r}: distance error {e.max()}"
+        assert np.all(e <= err[r][: got.size].astype(np.float64) + 2e-7), f"row {r}: bound"
+        if set(got.tolist()) != set(ref.tolist()):
+            kth = bd[r][ref.size - 1]
+            diff = set(got.tolist()) ^ set(ref.tolist())
+            dd = fp64.residual_dist_fp64(x, v, p, [q[r]], np.array(sorted(diff)),
+                                         kernel_type=kt, variance=var, lengthscale=ls)[0]
+            assert np.all(np.abs(dd - kth) <= TIE), f"row {r}: sets differ beyond ties"
+    return worst
+
+
+@pytest.mark.parametrize("n,d,m,kt,k,pred,dist", [
+    (4096, 3, 64, 1, 16, False, "normal"),   # C1
+    (4096, 3, 64, 1, 16, True, "normal"),
+    (3000, 2, 128, 0, 10, True, "normal"),
+    (2500, 5, 96, 0, 30, True, "uniform"),
+    (700, 3, 32, 1, 50, True, "normal"),     # k near the early rows' counts
+    (300, 3, 300, 1, 8, False, "normal"),    # m = N: every row inducing
+])
+def test_mixed_exact_vs_fp64(hip, n, d, m, kt, k, pred, dist):
+    from covertreex_amd.native import HipTree
+
+    print(f"n={n} d={d} m={m} kt={kt} k={k} pred={pred}", flush=True)
+    rng = np.random.default_rng(11)
+    x = (rng.standard_normal((n, d)) if dist == "normal" else rng.uniform(-3, 3, (n, d)))
+    x = x.astype(np.float32)
+    v, p, *_ = fp64.nystrom(x, inducing_count=m, seed=0, kernel_type=kt)
+    t = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=kt)
+    q = np.arange(n) if n <= 1000 else rng.choice(n, 600, replace=False)
+    idx, dst, err = t.knn_residual(q, k, predecessor_mode=pred, mode="mixed", return_error=True)
+    worst = _check(idx, dst, err, x, v, p, q, k, pred, kt)
+    print(f"max |d - d64| = {worst:.3e}", flush=True)
+    if pred:
+        for r, qi in enumerate(q):
+            assert np.all(idx[r][idx[r] >= 0] < qi)
+            assert (idx[r] >= 0).sum() == min(k, qi)
+
+
+def test_mixed_ard_lengthscale(hip):
+    from covertreex_amd.native import HipTree
+
+    rng = np.random.default_rng(5)
+    n, d = 2000, 5
+    x = rng.uniform(-3, 3, (n, d)).astype(np.float32)
+    ls = np.exp(rng.uniform(np.log(0.5), np.log(2.0), d)).astype(np.float32)
+    v, p, *_ = fp64.nystrom(x, inducing_count=128, seed=0, kernel_type=0, lengthscale=ls)
+    t = HipTree.build_residual(v, p, x, 1.0, ls, kernel_type=0)
+    q = rng.choice(n, 300, replace=False)
+    idx, dst, err = t.knn_residual(q, 30, predecessor_mode=True, mode="mixed", return_error=True)
+    _check(idx, dst, err, x, v, p, q, 30, True, 0, ls=ls.astype(np.float64))
+
+
+def test_mixed_query_order_and_duplicates(hip):
+    from covertreex_amd.native import HipTree
+
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((1500, 3)).astype(np.float32)
+    v, p, *_ = fp64.nystrom(x, inducing_count=64, seed=0, kernel_type=1)
+    t = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
+    q = np.array([1499, 3, 3, 0, 700, 1, 1200])
+    a_i, a_d = t.knn_residual(q, 12, predecessor_mode=True, mode="mixed")
+    for r, qi in enumerate(q):
+        b_i, b_d = t.knn_residual(np.array([qi]), 12, predecessor_mode=True, mode="mixed")
+        assert np.array_equal(a_i[r], b_i[0]) and np.array_equal(a_d[r], b_d[0])
+    assert np.all(a_i[3] == -1)
+    assert set(a_i[5][a_i[5] >= 0]) == {0}
+
+
+def test_c2_shape_sample(hip):
+    # N=32768, D=3, Matern l=1, m=256, k=50, predecessor mode
+    from covertreex_amd.native import HipTree
+
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((32768, 3)).astype(np.float32)
+    v, p, *_ = fp64.nystrom(x, inducing_count=256, seed=0, kernel_type=1)
+    t = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
+    q = np.arange(32768)
+    idx, dst, err = t.knn_residual(q, 50, predecessor_mode=True, mode="mixed", return_error=True)
+    sample = np.random.default_rng(1).choice(32768, 256, replace=False)
+    worst = _check(idx[sample], dst[sample], err[sample], x, v, p, sample, 50, True, 1)
+    print(f"C2 sample max |d - d64| = {worst:.3e}", flush=True)
+    # size-independent properties over all rows
+    valid = idx >= 0
+    assert np.array_equal(valid.sum(1), np.minimum(50, q))
+    assert np.all(np.where(valid, idx, -1) < q[:, None])
+    dd = np.where(valid, dst, 2.0)
+    assert np.all(np.diff(dd, axis=1) >= 0)
