@@ -26,6 +26,7 @@ CX_ORDER_NATURAL = 1
 
 CX_KNN_REFERENCE_F32 = 0
 CX_KNN_MIXED = 1
+CX_KNN_REFERENCE_F32_BUDGETED = 2
 
 _i64 = C.c_int64
 _i32 = C.c_int32

@@ -180,8 +180,9 @@ cx_status cx_tree_knn_residual_device(cx_tree *t, const int64_t *d_queries, int6
             return;
         t->s.n_marks = 0;
         t->s.last_mode = mode;
-        if (mode == CX_KNN_REFERENCE_F32)
-            knn_reference_f32(t->s, d_queries, nq, k, predecessor, d_out_idx, d_out_dist,
+        if (mode == CX_KNN_REFERENCE_F32 || mode == CX_KNN_REFERENCE_F32_BUDGETED)
+            knn_reference_f32(t->s, d_queries, nq, k, predecessor,
+                              mode == CX_KNN_REFERENCE_F32_BUDGETED, d_out_idx, d_out_dist,
                               d_out_err);
         else if (mode == CX_KNN_MIXED)
             knn_mixed(t->s, d_queries, nq, k, predecessor, d_out_idx, d_out_dist, d_out_err);

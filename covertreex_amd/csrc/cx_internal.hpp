@@ -137,7 +137,8 @@ void subtree_bounds_device(const int64_t *parents_host, const int64_t *n2d_host,
 
 // knn_ref.hip
 void knn_reference_f32(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k,
-                       int32_t predecessor, int64_t *d_idx, float *d_dist, float *d_err);
+                       int32_t predecessor, int32_t budgeted, int64_t *d_idx, float *d_dist,
+                       float *d_err);
 
 // knn_mixed.hip
 void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k,

@@ -186,6 +186,7 @@ __global__ void __launch_bounds__(256) screen_kernel(ScreenArgs A)
     const int wq0 = (wid >> 1) * 64, wc0 = (wid & 1) * 64;
 
     for (int64_t c0 = c_begin; c0 < c_end; c0 += kCT) {
+        __syncthreads(); // the previous tile's epilogue still reads sCx/sCp/sCn
         for (int i = tid; i < kCT; i += 256) {
             const int64_t j = c0 + i;
             const int64_t jr = j < A.n ? j : 0;
@@ -395,7 +396,7 @@ __device__ __forceinline__ df accurate_distance(const RescoreArgs &A, int64_t qt
     df rho = df_div(num, den);
     if (rho.hi < 0.0f)
         rho = df_neg(rho);
-    if (rho.hi >= 1.0f)
+    if (rho.hi > 1.0f || (rho.hi == 1.0f && rho.lo > 0.0f)) // |rho| clamped to 1
         rho = {1.0f, 0.0f};
     df s = df_sub(df{1.0f, 0.0f}, rho);
     df dist = df_sqrt(s);

@@ -23,6 +23,13 @@ namespace cx {
 
 constexpr int kRefMaxK = 256;
 
+// Budget ladder (src/algo.rs:648-683, 1191-1219); n == 0 disables it.
+struct Budget {
+    int32_t n;
+    int64_t sched[4];
+    double up, down;
+};
+
 struct RefTree {
     const int64_t *__restrict__ order;     // node -> dataset
     const int64_t *__restrict__ inv_order; // dataset -> node
@@ -112,7 +119,7 @@ __global__ void __launch_bounds__(64)
     knn_ref_kernel(MetricView mv, RefTree tr, const int64_t *__restrict__ queries, int64_t nq,
                    int32_t k, int32_t pred, int64_t *out_idx, float *out_dist, float *out_err,
                    int32_t *counter, int32_t *ws_node, float *ws_val, int64_t fr_cap,
-                   int64_t ch_cap)
+                   int64_t ch_cap, Budget bud)
 {
     __shared__ float s_td[kRefMaxK];
     __shared__ int32_t s_tn[kRefMaxK];
@@ -151,6 +158,10 @@ __global__ void __launch_bounds__(64)
             continue;
 
         // ---- roots (688-734) ------------------------------------------------
+        int32_t budget_idx = 0;
+        int64_t budget_limit = bud.n > 0 ? bud.sched[0] : INT64_MAX;
+        int64_t survivors = 0;
+        int32_t low_yield = 0;
         int64_t heap_len = 0;
         float kth = 3.40282347e38f;
         bool all_prunable = true;
@@ -179,6 +190,7 @@ __global__ void __launch_bounds__(64)
             fr_count += __popcll(__ballot(live));
             uint64_t vmask = __ballot(valid);
             heap_len += __popcll(vmask);
+            survivors += __popcll(vmask);
             float vmin = valid ? rd : 3.40282347e38f;
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
@@ -201,7 +213,8 @@ __global__ void __launch_bounds__(64)
         bool early = pred && all_prunable; // 737-749
 
         // ---- level loop (828-1226) -------------------------------------------
-        while (!early && fr_count > 0) {
+        while (!early && fr_count > 0 &&
+               (survivors < budget_limit || (pred && heap_len < k))) {
             int64_t ch_count = 0, nfr_count = 0, nfr_stored = 0;
             // gather (862-995)
             for (int64_t f = 0; f < fr_stored; f++) {
@@ -241,7 +254,7 @@ __global__ void __launch_bounds__(64)
             }
             // tiles (997-1222)
             int64_t start = 0;
-            while (start < ch_count) {
+            while (start < ch_count && (survivors < budget_limit || (pred && heap_len < k))) {
                 const int64_t active = ch_count - start;
                 const int64_t wave = fr_count > 1 ? fr_count : 1;
                 const int64_t comb = active + wave;
@@ -287,6 +300,7 @@ __global__ void __launch_bounds__(64)
                         }
                     }
                 }
+                int64_t added = 0;
                 for (int e = 0; e < n_ev; e++) { // 1111-1189
                     const float dv = __shfl(dist, e, 64);
                     const int32_t child = __shfl(node, e, 64);
@@ -301,6 +315,10 @@ __global__ void __launch_bounds__(64)
                     if (heap_len < k)
                         heap_len++;
                     kth = tk.maxv; // heap peek after push (+pop)
+                    survivors++;
+                    added++;
+                    if (!(survivors < budget_limit || (pred && heap_len < k)))
+                        break;
                     bool sub_valid = pred ? tr.smin[child] < max_idx : true;
                     if (sub_valid) {
                         if (has_children(tr, child)) {
@@ -313,6 +331,26 @@ __global__ void __launch_bounds__(64)
                         nfr_count++;
                     }
                 }
+                bool stop = false;
+                if (bud.n > 0 && len > 0) { // budget ladder (1191-1219)
+                    const double ratio = (double)added / (double)len;
+                    if (ratio >= bud.up) {
+                        if (budget_idx + 1 < bud.n) {
+                            budget_idx++;
+                            budget_limit = bud.sched[budget_idx];
+                        }
+                        low_yield = 0;
+                    } else if (ratio < bud.down) {
+                        low_yield++;
+                        const bool need_more = pred && heap_len < k;
+                        if (low_yield >= 2 && !need_more)
+                            stop = true;
+                    } else {
+                        low_yield = 0;
+                    }
+                }
+                if (stop)
+                    break;
                 start = end;
             }
             int32_t *tn = fa_n;
@@ -344,7 +382,8 @@ __global__ void __launch_bounds__(64)
 }
 
 void knn_reference_f32(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k,
-                       int32_t predecessor, int64_t *d_idx, float *d_dist, float *d_err)
+                       int32_t predecessor, int32_t budgeted, int64_t *d_idx, float *d_dist,
+                       float *d_err)
 {
     CX_REQUIRE(k >= 1 && k <= kRefMaxK, "reference-form kNN supports 1 <= k <= 256");
     if (nq <= 0)
@@ -363,9 +402,18 @@ void knn_reference_f32(TreeState &t, const int64_t *d_queries, int64_t nq, int32
     MetricView mv{t.scaled.ptr, t.norms.ptr, t.v.ptr, t.p.ptr, t.d, t.m, t.var, t.kernel};
     RefTree tr{t.order.ptr, t.inv_order.ptr,  t.roots.ptr, t.n_roots, t.child_off.ptr,
                t.child_list.ptr, t.si.ptr, t.smin.ptr, t.n};
+    Budget bud{};
+    if (budgeted) { // reference defaults (src/algo.rs:653-676)
+        bud.n = 3;
+        bud.sched[0] = 32;
+        bud.sched[1] = 64;
+        bud.sched[2] = 96;
+        bud.up = 0.015;
+        bud.down = 0.002;
+    }
     knn_ref_kernel<<<(unsigned)waves, 64, 0, st>>>(mv, tr, d_queries, nq, k, predecessor, d_idx,
                                                    d_dist, d_err, counter, ws_node, ws_val,
-                                                   fr_cap, ch_cap);
+                                                   fr_cap, ch_cap, bud);
     CX_HIP(hipGetLastError());
     prof_mark(t, 1);
 }

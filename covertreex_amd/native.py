@@ -17,6 +17,7 @@ from . import _ffi
 from ._ffi import check, ptr
 
 MODES = {"reference": _ffi.CX_KNN_REFERENCE_F32, "reference-f32": _ffi.CX_KNN_REFERENCE_F32,
+         "reference-budgeted": _ffi.CX_KNN_REFERENCE_F32_BUDGETED,
          "mixed": _ffi.CX_KNN_MIXED}
 
 
@@ -106,7 +107,7 @@ class HipTree:
 
     # -- telemetry ------------------------------------------------------------------
     PHASES = {"mixed": ["prep", "query_sort", "screen_pass1", "tau", "screen_pass2", "rescore"],
-              "reference": ["traversal"]}
+              "reference": ["traversal"], "reference-budgeted": ["traversal"]}
 
     def set_profiling(self, enable: bool = True) -> None:
         check(_ffi.load().cx_tree_set_profiling(self._h, int(bool(enable))))

@@ -47,6 +47,10 @@ enum {
      * in compensated FP32; neighbour sets are exact under the residual metric
      * evaluated on the stored inputs. */
     CX_KNN_MIXED = 1,
+    /* The reference's default search: CX_KNN_REFERENCE_F32 with the budget
+     * ladder [32, 64, 96], up 0.015, down 0.002 (src/algo.rs:648-683,
+     * 1191-1219) -- approximate, as the reference's default is. */
+    CX_KNN_REFERENCE_F32_BUDGETED = 2,
 };
 
 typedef struct cx_tree cx_tree;
@@ -58,8 +62,8 @@ const char *cx_last_error(void);
 cx_status cx_device_count(int32_t *out);
 
 /* Replaces build_pcct2_residual_tree (src/lib.rs:1373-1564), including the
- * si_cache pass (src/lib.rs:1548-1561) and, when subtree_bounds != 0, the
- * engine's compute_subtree_bounds_py call (covertreex/engine.py:1038-1043).
+ * si_cache pass (src/lib.rs:1548-1561) and the engine's
+ * compute_subtree_bounds_py call (covertreex/engine.py:1038-1043).
  *   V: n x m float32 row-major (Nystrom factor), p: n residual variances,
  *   coords: n x d float32, ls: n_ls lengthscales (1 or d values),
  *   chunk: batch size (reference default 512), order_mode: CX_ORDER_*,

@@ -86,6 +86,22 @@ def test_reference_form_knn_matches_oracle(hip, case, k, pred):
                 assert set(hi[r][hd[r] == dv]) == set(oi[r][od[r] == dv])
 
 
+@pytest.mark.parametrize("case,k,pred", [
+    (BUILD_CASES[0], 16, False),
+    (BUILD_CASES[1], 10, True),
+    (BUILD_CASES[1], 10, False),
+    (BUILD_CASES[2], 30, True),
+])
+def test_budgeted_reference_form_matches_oracle(hip, case, k, pred):
+    # the reference's default search: budget ladder [32, 64, 96]
+    print(f"case {case} k={k} pred={pred}", flush=True)
+    x, v, p, ot, ht = _tree_pair(hip, case)
+    q = np.arange(case["n"])
+    oi, od, _ = oracle.knn_residual(ot, q, k, predecessor=pred, budget="default")
+    hi, hd = ht.knn_residual(q, k, predecessor_mode=pred, mode="reference-budgeted")
+    assert np.array_equal(hd.view(np.uint32), od.view(np.uint32))
+
+
 def test_reference_form_edge_queries(hip):
     case = BUILD_CASES[1]
     x, v, p, ot, ht = _tree_pair(hip, case, seed=3)
