@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcovertreex_hip.so")
 OBJDIR = os.path.join(os.path.dirname(HERE), "_build", "covertreex_hip")
-SOURCES = ["api.hip", "build.hip", "knn_ref.hip", "knn_mixed.hip", "nystrom.hip"]
+SOURCES = ["api.hip", "build.hip", "knn_ref.hip", "knn_mixed.hip", "knn_euclid.hip",
+           "nystrom.hip", "graph.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLAGS = [

@@ -27,6 +27,7 @@ CX_ORDER_NATURAL = 1
 CX_KNN_REFERENCE_F32 = 0
 CX_KNN_MIXED = 1
 CX_KNN_REFERENCE_F32_BUDGETED = 2
+CX_KNN_EXACT = 1
 
 _i64 = C.c_int64
 _i32 = C.c_int32
@@ -41,6 +42,8 @@ SIGNATURES = {
                                       _i32, _f32, _p]),
     "cx_tree_knn_residual": (_i32, [_p, _p, _i64, _i32, _i32, _i32, _p, _p, _p]),
     "cx_tree_knn_residual_device": (_i32, [_p, _p, _i64, _i32, _i32, _i32, _p, _p, _p]),
+    "cx_tree_build_euclidean": (_i32, [_p, _i64, _i32, _i32, _p]),
+    "cx_tree_knn_euclidean": (_i32, [_p, _p, _i64, _i32, _i32, _i32, _p, _p]),
     "cx_tree_info": (_i32, [_p, _p, _p, _p, _p, _p]),
     "cx_tree_get_parents": (_i32, [_p, _p]),
     "cx_tree_get_children": (_i32, [_p, _p]),
@@ -51,6 +54,8 @@ SIGNATURES = {
     "cx_tree_get_subtree_bounds": (_i32, [_p, _p, _p]),
     "cx_subtree_bounds": (_i32, [_p, _p, _i64, _p, _p]),
     "cx_morton_order": (_i32, [_p, _i64, _i32, _p]),
+    "cx_hilbert_order": (_i32, [_p, _i64, _i32, _p]),
+    "cx_luby_mis": (_i32, [_p, _p, _i64, C.c_uint32, _p, _p]),
     "cx_nystrom_residual": (_i32, [_p, _i64, _i32, _p, _i32, C.c_double, _p, _i32, _i32,
                                    C.c_double, _p, _p]),
     "cx_tree_set_profiling": (_i32, [_p, _i32]),

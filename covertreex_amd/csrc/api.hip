@@ -175,6 +175,7 @@ cx_status cx_tree_knn_residual_device(cx_tree *t, const int64_t *d_queries, int6
         CX_REQUIRE(nq >= 0, "nq must be >= 0");
         CX_REQUIRE(nq == 0 || (d_queries && d_out_idx && d_out_dist), "NULL query/output pointer");
         std::lock_guard<std::mutex> lock(t->s.mu);
+        CX_REQUIRE(t->s.metric == 0, "residual kNN on a Euclidean tree");
         CX_HIP(hipSetDevice(t->s.device));
         if (nq == 0)
             return;
@@ -218,6 +219,65 @@ cx_status cx_tree_knn_residual(cx_tree *t, const int64_t *queries, int64_t nq, i
         if (out_err)
             CX_HIP(hipMemcpyAsync(out_err, oe.ptr, sizeof(float) * nq * k,
                                   hipMemcpyDeviceToHost, st));
+        CX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+cx_status cx_tree_build_euclidean(const float *points, int64_t n, int32_t d, int32_t chunk,
+                                  cx_tree **out)
+{
+    return guarded([&] {
+        CX_REQUIRE(out != nullptr, "out is NULL");
+        *out = nullptr;
+        CX_REQUIRE(n >= 0 && n < (int64_t)1 << 31, "n must be in [0, 2^31)");
+        CX_REQUIRE(d >= 1 && d <= 64, "need 1 <= d <= 64");
+        CX_REQUIRE(n == 0 || points, "points must be non-NULL");
+        require_device();
+        cx_tree *h = new cx_tree();
+        try {
+            TreeState &t = h->s;
+            CX_HIP(hipGetDevice(&t.device));
+            CX_HIP(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
+            t.n = n;
+            t.d = d;
+            t.metric = 1;
+            if (n > 0)
+                build_euclidean_tree(t, points, chunk);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+cx_status cx_tree_knn_euclidean(cx_tree *t, const float *queries, int64_t nq, int32_t k,
+                                int32_t predecessor, int32_t mode, int64_t *out_idx,
+                                float *out_dist)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        CX_REQUIRE(t->s.metric == 1, "Euclidean kNN on a residual tree");
+        CX_REQUIRE(k >= 1 && nq >= 0, "need k >= 1 and nq >= 0");
+        CX_REQUIRE(nq == 0 || (queries && out_idx && out_dist), "NULL query/output pointer");
+        CX_REQUIRE(mode == CX_KNN_REFERENCE_F32 || mode == CX_KNN_EXACT, "unknown kNN mode");
+        if (nq == 0)
+            return;
+        std::lock_guard<std::mutex> lock(t->s.mu);
+        CX_HIP(hipSetDevice(t->s.device));
+        TreeState &s = t->s;
+        s.n_marks = 0;
+        s.last_mode = mode;
+        hipStream_t st = s.stream;
+        DevBuf<float> q((size_t)nq * s.d), od((size_t)nq * k);
+        DevBuf<int64_t> oi((size_t)nq * k);
+        CX_HIP(hipMemcpyAsync(q.ptr, queries, sizeof(float) * nq * s.d, hipMemcpyHostToDevice,
+                              st));
+        knn_euclidean(s, q.ptr, nq, k, predecessor, mode, oi.ptr, od.ptr);
+        CX_HIP(hipMemcpyAsync(out_idx, oi.ptr, sizeof(int64_t) * nq * k, hipMemcpyDeviceToHost,
+                              st));
+        CX_HIP(hipMemcpyAsync(out_dist, od.ptr, sizeof(float) * nq * k, hipMemcpyDeviceToHost,
+                              st));
         CX_HIP(hipStreamSynchronize(st));
     });
 }
@@ -333,6 +393,40 @@ cx_status cx_tree_last_profile(cx_tree *t, float *ms_out, int32_t cap, int32_t *
             CX_HIP(hipEventElapsedTime(&ms, s.marks[i], s.marks[i + 1]));
             ms_out[i] = ms;
         }
+    });
+}
+
+cx_status cx_hilbert_order(const double *points, int64_t n, int32_t d, int64_t *order_out)
+{
+    return guarded([&] {
+        CX_REQUIRE(n >= 0 && n < (int64_t)1 << 31, "n must be in [0, 2^31)");
+        CX_REQUIRE(d >= 1 && d <= 64, "need 1 <= d <= 64 (codes must fit 64 bits)");
+        if (n == 0)
+            return;
+        CX_REQUIRE(points && order_out, "NULL pointer");
+        require_device();
+        hilbert_order_device(points, n, d, order_out);
+    });
+}
+
+cx_status cx_luby_mis(const int64_t *indptr, const int64_t *indices, int64_t n, uint32_t seed,
+                      uint8_t *selected_out, int64_t *iterations_out)
+{
+    return guarded([&] {
+        CX_REQUIRE(n >= 0 && n < (int64_t)1 << 31, "n must be in [0, 2^31)");
+        CX_REQUIRE(iterations_out != nullptr, "NULL pointer");
+        *iterations_out = 0;
+        if (n == 0)
+            return;
+        CX_REQUIRE(indptr && selected_out, "NULL pointer");
+        CX_REQUIRE(indptr[0] == 0, "indptr[0] must be 0");
+        for (int64_t v = 0; v < n; v++)
+            CX_REQUIRE(indptr[v + 1] >= indptr[v], "indptr must be non-decreasing");
+        CX_REQUIRE(indptr[n] == 0 || indices, "NULL indices");
+        for (int64_t e = 0; e < indptr[n]; e++)
+            CX_REQUIRE(indices[e] >= 0 && indices[e] < n, "neighbour index out of range");
+        require_device();
+        *iterations_out = luby_mis_device(indptr, indices, n, seed, selected_out);
     });
 }
 

@@ -713,6 +713,62 @@ void build_residual_tree(TreeState &t, const float *V, const float *p, const flo
     t.n_roots = nr;
 }
 
+// Euclidean tree: chunked batch_insert_collect with the Euclidean metric in
+// natural order and no grid edges (src/lib.rs:207-223, src/algo/batch/mod.rs:
+// 402-409), plus the same cover-radius recurrence with Euclidean distances.
+void build_euclidean_tree(TreeState &t, const float *points, int32_t chunk)
+{
+    const int64_t n = t.n;
+    const int32_t d = t.d;
+    hipStream_t st = t.stream;
+    CX_REQUIRE(chunk > 0 && chunk <= kBiMaxChunk, "chunk must be in [1, 1024]");
+    t.raw.alloc((size_t)n * d);
+    CX_HIP(hipMemcpyAsync(t.raw.ptr, points, sizeof(float) * n * d, hipMemcpyHostToDevice, st));
+    t.order.alloc(n);
+    t.inv_order.alloc(n);
+    iota_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.order.ptr, n);
+    iota_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.inv_order.ptr, n);
+    t.parents.alloc(n);
+    t.children.alloc(n);
+    t.next.alloc(n);
+    t.levels.alloc(n);
+    MetricView mv{t.raw.ptr, nullptr, nullptr, nullptr, d, 0, 1.0f, 0};
+    TreeLinks links{t.parents.ptr, t.children.ptr, t.next.ptr, t.levels.ptr};
+    batch_insert_kernel<kMetricEuclidean><<<1, kBiThreads, 0, st>>>(mv, links, t.raw.ptr, d, 1.0f,
+                                                                    0, n, chunk, nullptr);
+    CX_HIP(hipGetLastError());
+    t.min_level = n > 0 ? -10 : -20;
+    t.max_level = n > 0 ? 10 : 20;
+    build_child_csr(t, st);
+    t.depth.alloc(n);
+    DevBuf<int32_t> top(n);
+    t.max_depth = compute_depths(t.parents.ptr, n, t.depth.ptr, top.ptr, st);
+    t.si.alloc(n);
+    CX_HIP(hipMemsetAsync(t.si.ptr, 0, sizeof(float) * n, st));
+    for (int32_t lv = t.max_depth; lv >= 1; lv--)
+        si_level_kernel<kMetricEuclidean><<<nblocks(n, 256), 256, 0, st>>>(
+            mv, t.parents.ptr, t.depth.ptr, top.ptr, n, lv, t.si.ptr);
+    CX_HIP(hipGetLastError());
+    t.smin.alloc(n);
+    t.smax.alloc(n);
+    subtree_bounds_on_stream(t.parents.ptr, t.order.ptr, t.depth.ptr, t.max_depth, n, t.smin.ptr,
+                             t.smax.ptr, st);
+    DevBuf<int32_t> flags(n), nsel(1);
+    t.roots.alloc(n);
+    root_flags_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.parents.ptr, n, flags.ptr);
+    hipcub::CountingInputIterator<int32_t> it(0);
+    size_t tmp_bytes = 0;
+    CX_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp_bytes, it, flags.ptr, t.roots.ptr, nsel.ptr,
+                                         (int)n, st));
+    DevBuf<uint8_t> tmp(tmp_bytes);
+    CX_HIP(hipcub::DeviceSelect::Flagged(tmp.ptr, tmp_bytes, it, flags.ptr, t.roots.ptr, nsel.ptr,
+                                         (int)n, st));
+    int32_t nr = 0;
+    CX_HIP(hipMemcpyAsync(&nr, nsel.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CX_HIP(hipStreamSynchronize(st));
+    t.n_roots = nr;
+}
+
 void morton_order_device(const float *coords_host, int64_t n, int32_t d, int64_t *order_host)
 {
     hipStream_t st = nullptr;

@@ -85,6 +85,7 @@ struct TreeState {
 
     int64_t n = 0;
     int32_t m = 0, d = 0;
+    int32_t metric = 0; // 0 residual correlation, 1 Euclidean
     float var = 1.0f;
     int32_t kernel = 0;
     int32_t min_level = -20, max_level = 20;
@@ -131,6 +132,7 @@ struct TreeState {
 // build.hip
 void build_residual_tree(TreeState &t, const float *V, const float *p, const float *coords,
                          int32_t chunk, int32_t order_mode, float grid_scale);
+void build_euclidean_tree(TreeState &t, const float *points, int32_t chunk);
 void morton_order_device(const float *coords_host, int64_t n, int32_t d, int64_t *order_host);
 void subtree_bounds_device(const int64_t *parents_host, const int64_t *n2d_host, int64_t n,
                            int64_t *min_host, int64_t *max_host);
@@ -143,6 +145,15 @@ void knn_reference_f32(TreeState &t, const int64_t *d_queries, int64_t nq, int32
 // knn_mixed.hip
 void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k,
                int32_t predecessor, int64_t *d_idx, float *d_dist, float *d_err);
+
+// knn_euclid.hip
+void knn_euclidean(TreeState &t, const float *d_queries, int64_t nq, int32_t k, int32_t pred,
+                   int32_t mode, int64_t *d_idx, float *d_dist);
+
+// graph.hip
+void hilbert_order_device(const double *points_host, int64_t n, int32_t d, int64_t *order_host);
+int64_t luby_mis_device(const int64_t *indptr_host, const int64_t *indices_host, int64_t n,
+                        uint32_t seed, uint8_t *sel_host);
 
 // nystrom.hip
 void nystrom_device(const double *pts_host, int64_t n, int32_t d, const int64_t *ind_host,

@@ -1,0 +1,258 @@
+// Python-path build primitives on the device:
+//   Hilbert order  -> _hilbert_permutation, covertreex/algo/order/strategy.py:38-99
+//   Luby MIS       -> _run_mis_numba_impl, covertreex/algo/_mis_numba.py:64-131
+// Both are integer work: bit-exact with the reference's semantics.
+#include <hipcub/hipcub.hpp>
+
+#include "cx_internal.hpp"
+
+namespace cx {
+
+// ---- Hilbert ---------------------------------------------------------------
+
+__global__ void minmax_f64_kernel(const double *__restrict__ x, int64_t n, int32_t d, double *lo,
+                                  double *hi)
+{
+    // one block per dimension
+    const int32_t c = blockIdx.x;
+    __shared__ double smin[256], smax[256];
+    double mn = __builtin_huge_val(), mx = -__builtin_huge_val();
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        double v = x[i * d + c];
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            smin[threadIdx.x] = smin[threadIdx.x + o] < smin[threadIdx.x] ? smin[threadIdx.x + o]
+                                                                          : smin[threadIdx.x];
+            smax[threadIdx.x] = smax[threadIdx.x + o] > smax[threadIdx.x] ? smax[threadIdx.x + o]
+                                                                          : smax[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        lo[c] = smin[0];
+        hi[c] = smax[0];
+    }
+}
+
+__global__ void hilbert_codes_kernel(const double *__restrict__ x, int64_t n, int32_t d,
+                                     int32_t bits, const double *__restrict__ lo,
+                                     const double *__restrict__ hi, uint64_t *codes, int64_t *idx)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint64_t v[64];
+    const uint64_t max_val = (1ull << bits) - 1;
+    for (int32_t c = 0; c < d; c++) {
+        double span = hi[c] - lo[c];
+        span = span > 1e-9 ? span : 1e-9; // np.maximum(hi - lo, 1e-9)
+        double s = (x[i * d + c] - lo[c]) / span;
+        double r = __builtin_rint(s * (double)max_val); // np.round: half to even
+        r = r < 0.0 ? 0.0 : r;
+        r = r > (double)max_val ? (double)max_val : r;
+        v[c] = (uint64_t)r;
+    }
+    // Skilling AxesToTranspose (strategy.py:63-88)
+    const uint64_t M = 1ull << (bits - 1);
+    for (uint64_t Q = M; Q > 1; Q >>= 1) {
+        const uint64_t P = Q - 1;
+        for (int32_t c = 0; c < d; c++) {
+            if (v[c] & Q) {
+                v[0] ^= P;
+            } else {
+                uint64_t t = (v[0] ^ v[c]) & P;
+                v[0] ^= t;
+                v[c] ^= t;
+            }
+        }
+    }
+    for (int32_t c = 1; c < d; c++)
+        v[c] ^= v[c - 1];
+    uint64_t t = 0;
+    for (uint64_t Q = M; Q > 1; Q >>= 1)
+        if (v[d - 1] & Q)
+            t ^= Q - 1;
+    for (int32_t c = 0; c < d; c++)
+        v[c] ^= t;
+    // interleave, most significant bit first (strategy.py:91-99)
+    uint64_t code = 0;
+    for (int32_t b = bits - 1; b >= 0; b--) {
+        uint64_t word = 0;
+        for (int32_t c = 0; c < d; c++)
+            word = (word << 1) | ((v[c] >> b) & 1ull);
+        code = (code << d) | word;
+    }
+    codes[i] = code;
+    idx[i] = i;
+}
+
+void hilbert_order_device(const double *points_host, int64_t n, int32_t d, int64_t *order_host)
+{
+    hipStream_t st = nullptr;
+    int32_t bits = 64 / d;
+    bits = bits > 16 ? 16 : (bits < 1 ? 1 : bits);
+    DevBuf<double> x((size_t)n * d), lo(d), hi(d);
+    DevBuf<uint64_t> codes(n), sorted(n);
+    DevBuf<int64_t> idx(n), order(n);
+    CX_HIP(hipMemcpy(x.ptr, points_host, sizeof(double) * n * d, hipMemcpyHostToDevice));
+    minmax_f64_kernel<<<(unsigned)d, 256, 0, st>>>(x.ptr, n, d, lo.ptr, hi.ptr);
+    hilbert_codes_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(x.ptr, n, d, bits, lo.ptr,
+                                                                      hi.ptr, codes.ptr, idx.ptr);
+    CX_HIP(hipGetLastError());
+    // lexsort((arange, codes)): stable radix sort on the code
+    size_t tmp_bytes = 0;
+    CX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, codes.ptr, sorted.ptr, idx.ptr,
+                                              order.ptr, (int)n, 0, 64, st));
+    DevBuf<uint8_t> tmp(tmp_bytes);
+    CX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, codes.ptr, sorted.ptr, idx.ptr,
+                                              order.ptr, (int)n, 0, 64, st));
+    CX_HIP(hipMemcpy(order_host, order.ptr, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+}
+
+// ---- Luby MIS ----------------------------------------------------------------
+
+// xorshift priority (_mis_numba.py:19-28) with 32-bit wrapping as in the JAX
+// path (covertreex/algo/mis.py:146-168)
+__device__ __forceinline__ double luby_priority(int64_t node, uint32_t seed, int64_t iteration)
+{
+    uint32_t x = (uint32_t)node ^ seed;
+    x ^= (uint32_t)((uint64_t)iteration * 0x9E3779B9ull);
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return ((double)x + 1.0) / 4294967296.0;
+}
+
+__global__ void luby_round_kernel(const int64_t *__restrict__ indptr,
+                                  const int64_t *__restrict__ indices, int64_t n, uint32_t seed,
+                                  int64_t iteration, const uint8_t *__restrict__ active,
+                                  uint8_t *win, int32_t *any_win)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n)
+        return;
+    win[v] = 0;
+    if (!active[v])
+        return;
+    const double pv = luby_priority(v, seed, iteration);
+    for (int64_t e = indptr[v]; e < indptr[v + 1]; e++) {
+        const int64_t u = indices[e];
+        if (!active[u])
+            continue;
+        const double pu = luby_priority(u, seed, iteration);
+        if (pu > pv || (pu == pv && u < v))
+            return;
+    }
+    win[v] = 1;
+    *any_win = 1;
+}
+
+__global__ void luby_fallback_kernel(const uint8_t *__restrict__ active, int64_t n,
+                                     const int32_t *__restrict__ any_win, uint8_t *win)
+{
+    // no winner: the first active node wins (_mis_numba.py:110-114)
+    if (*any_win || threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    for (int64_t v = 0; v < n; v++)
+        if (active[v]) {
+            win[v] = 1;
+            return;
+        }
+}
+
+__global__ void luby_apply_kernel(const int64_t *__restrict__ indptr,
+                                  const int64_t *__restrict__ indices, int64_t n,
+                                  const uint8_t *__restrict__ win, uint8_t *active, uint8_t *dom,
+                                  uint8_t *sel)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n || !win[v])
+        return;
+    sel[v] = 1;
+    for (int64_t e = indptr[v]; e < indptr[v + 1]; e++)
+        dom[indices[e]] = 1;
+}
+
+__global__ void luby_retire_kernel(int64_t n, const uint8_t *__restrict__ win,
+                                   const uint8_t *__restrict__ dom, uint8_t *active,
+                                   int32_t *any_active)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n)
+        return;
+    if (win[v] || dom[v])
+        active[v] = 0;
+    if (active[v])
+        *any_active = 1;
+}
+
+// greedy over index order for graphs of max degree <= 8 (_mis_numba.py:36-62)
+__global__ void greedy_mis_kernel(const int64_t *__restrict__ indptr,
+                                  const int64_t *__restrict__ indices, int64_t n, uint8_t *active,
+                                  uint8_t *sel)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    for (int64_t v = 0; v < n; v++) {
+        if (!active[v])
+            continue;
+        sel[v] = 1;
+        for (int64_t e = indptr[v]; e < indptr[v + 1]; e++)
+            active[indices[e]] = 0;
+        active[v] = 0;
+    }
+}
+
+int64_t luby_mis_device(const int64_t *indptr_host, const int64_t *indices_host, int64_t n,
+                        uint32_t seed, uint8_t *sel_host)
+{
+    hipStream_t st = nullptr;
+    const int64_t ne = indptr_host[n];
+    int64_t maxdeg = 0;
+    for (int64_t v = 0; v < n; v++)
+        maxdeg = indptr_host[v + 1] - indptr_host[v] > maxdeg ? indptr_host[v + 1] - indptr_host[v]
+                                                              : maxdeg;
+    DevBuf<int64_t> indptr(n + 1), indices(ne > 0 ? ne : 1);
+    DevBuf<uint8_t> active(n), win(n), dom(n), sel(n);
+    DevBuf<int32_t> flags(2);
+    CX_HIP(hipMemcpy(indptr.ptr, indptr_host, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    if (ne > 0)
+        CX_HIP(hipMemcpy(indices.ptr, indices_host, sizeof(int64_t) * ne, hipMemcpyHostToDevice));
+    CX_HIP(hipMemset(active.ptr, 1, n));
+    CX_HIP(hipMemset(sel.ptr, 0, n));
+    int64_t iterations = 0;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (maxdeg <= 8) {
+        greedy_mis_kernel<<<1, 64, 0, st>>>(indptr.ptr, indices.ptr, n, active.ptr, sel.ptr);
+        CX_HIP(hipGetLastError());
+        iterations = 1;
+    } else {
+        for (;;) {
+            iterations++;
+            CX_HIP(hipMemset(flags.ptr, 0, 2 * sizeof(int32_t)));
+            CX_HIP(hipMemset(dom.ptr, 0, n));
+            luby_round_kernel<<<blocks, 256, 0, st>>>(indptr.ptr, indices.ptr, n, seed,
+                                                      iterations, active.ptr, win.ptr, flags.ptr);
+            luby_fallback_kernel<<<1, 64, 0, st>>>(active.ptr, n, flags.ptr, win.ptr);
+            luby_apply_kernel<<<blocks, 256, 0, st>>>(indptr.ptr, indices.ptr, n, win.ptr,
+                                                      active.ptr, dom.ptr, sel.ptr);
+            luby_retire_kernel<<<blocks, 256, 0, st>>>(n, win.ptr, dom.ptr, active.ptr,
+                                                       flags.ptr + 1);
+            CX_HIP(hipGetLastError());
+            int32_t any_active = 0;
+            CX_HIP(hipMemcpy(&any_active, flags.ptr + 1, sizeof(int32_t), hipMemcpyDeviceToHost));
+            if (!any_active || iterations > n + 1)
+                break;
+        }
+    }
+    CX_HIP(hipMemcpy(sel_host, sel.ptr, n, hipMemcpyDeviceToHost));
+    return iterations;
+}
+
+} // namespace cx

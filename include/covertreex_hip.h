@@ -51,6 +51,8 @@ enum {
      * ladder [32, 64, 96], up 0.015, down 0.002 (src/algo.rs:648-683,
      * 1191-1219) -- approximate, as the reference's default is. */
     CX_KNN_REFERENCE_F32_BUDGETED = 2,
+    /* Euclidean trees: exact k nearest (tiled scan, FP32 differences). */
+    CX_KNN_EXACT = 1,
 };
 
 typedef struct cx_tree cx_tree;
@@ -94,6 +96,23 @@ cx_status cx_tree_knn_residual_device(cx_tree *t, const int64_t *d_queries, int6
                                       int64_t *d_out_idx, float *d_out_dist,
                                       float *d_out_err);
 
+/* Euclidean tree over n points of dimension d: chunked batch insert in
+ * natural order with no grid edges.  Replaces CoverTreeWrapper(...).insert
+ * (src/lib.rs:207-223 -> batch_insert, src/algo/batch/mod.rs:402-409), which
+ * the reference's API path runs on one unchunked batch; chunk = n <= 1024
+ * gives that semantics. */
+cx_status cx_tree_build_euclidean(const float *points, int64_t n, int32_t d, int32_t chunk,
+                                  cx_tree **out);
+
+/* Replaces CoverTreeWrapper.knn_query (src/lib.rs:331-357 -> batch_knn_query,
+ * src/algo.rs:179-323).  queries: nq x d float32 points (host); neighbours are
+ * tree node indices (= input row indices, natural order).  mode:
+ * CX_KNN_REFERENCE_F32 (the reference's best-first search) or CX_KNN_EXACT.
+ * Predecessor mode admits node j for query position i when j < i. */
+cx_status cx_tree_knn_euclidean(cx_tree *t, const float *queries, int64_t nq, int32_t k,
+                                int32_t predecessor, int32_t mode, int64_t *out_idx,
+                                float *out_dist);
+
 /* Tree shape.  Replaces len(node_to_dataset) / get_min_level / get_max_level
  * (src/lib.rs:889-904). */
 cx_status cx_tree_info(const cx_tree *t, int64_t *n, int32_t *m, int32_t *d,
@@ -118,6 +137,20 @@ cx_status cx_subtree_bounds(const int64_t *parents, const int64_t *node_to_datas
 /* Replaces hilbert_order (src/lib.rs:956-962 -> src/pcct.rs:27-67): the 21-bit
  * Morton permutation for d == 3, identity otherwise, computed on the device. */
 cx_status cx_morton_order(const float *coords, int64_t n, int32_t d, int64_t *order_out);
+
+/* Replaces _hilbert_permutation (covertreex/algo/order/strategy.py:38-60),
+ * the python-path batch order: bits = max(1, min(16, 64 / d)) per axis,
+ * round-half-even quantisation in FP64, Skilling transpose, codes sorted
+ * stably (ties by index).  points: n x d float64; d <= 64. */
+cx_status cx_hilbert_order(const double *points, int64_t n, int32_t d, int64_t *order_out);
+
+/* Replaces run_mis_numba / _run_mis_numba_impl (covertreex/algo/mis.py:55-139,
+ * covertreex/algo/_mis_numba.py:64-131): greedy in index order when the
+ * max degree is <= 8, else Luby rounds with xorshift priorities (32-bit
+ * arithmetic, ties to the lower index, first active node when a round has no
+ * winner).  CSR graph (indptr n+1, indices), selected_out: n bytes (0/1). */
+cx_status cx_luby_mis(const int64_t *indptr, const int64_t *indices, int64_t n, uint32_t seed,
+                      uint8_t *selected_out, int64_t *iterations_out);
 
 /* Replaces build_residual_backend's numeric core
  * (covertreex/metrics/residual/host_backend.py:201-240): FP64 kernel
