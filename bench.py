@@ -71,6 +71,8 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
@@ -129,6 +131,24 @@ def main() -> int:
     torch.cuda.synchronize()
     plain_qps = N * args.plain_steps / (time.perf_counter() - s0)
 
+    serve = None
+    if args.serve_requests > 0:
+        from covertreex_amd.serve import Server, latency_summary, make_stream, run_poisson
+
+        reqs = make_stream(args.serve_requests, seed=0, n_min=2048, n_max=32768)
+        server = Server(max_inflight=4, hbm_budget_bytes=64 << 30)
+        try:
+            res = run_poisson(server, reqs, rate_per_s=args.serve_rate, seed=0)
+        finally:
+            server.close()
+        serve = {k_: (round(v_, 3) if isinstance(v_, float) else v_)
+                 for k_, v_ in latency_summary(res).items()}
+        serve.update({"rate_per_s": args.serve_rate, "max_inflight": 4,
+                      "peak_inflight": server.peak_inflight,
+                      "workload": "Poisson arrivals; N log-uniform [2048, 32768], D in {2,3,5}, "
+                                  "Matern/RBF l~U[0.5,2], m in {64,128,256}, k in {10,30,50}, "
+                                  "predecessor; build + all-points kNN per request"})
+
     cpu = None
     evals = None
     if not args.no_cpu:
@@ -169,6 +189,7 @@ def main() -> int:
                      "kernel": "screen_kernel<1>",
                      "algorithmic_flops_per_launch": N * flops_per_q},
         "cpu_baseline": cpu,
+        "serve": serve,
     }
     print(json.dumps(out))
     return 0

@@ -38,7 +38,7 @@ TreeState::~TreeState()
     vlo.release();
     vnorm.release();
     x_ds.release();
-    p_ds.release();
+    isp_ds.release();
     work.release();
     for (int i = 0; i < kMaxMarks; i++)
         if (marks[i])

@@ -113,7 +113,7 @@ struct TreeState {
     DevBuf<uint16_t> vhi, vlo;     // n x mp bf16, zero padded to mp
     int32_t mp = 0;                // m padded to a multiple of 32
     DevBuf<float> vnorm;           // |V_i| rounded up, dataset order
-    DevBuf<float> x_ds, p_ds;      // raw coords and p in dataset order
+    DevBuf<float> x_ds, isp_ds;    // raw coords and 1/sqrt(p) in dataset order
 
     // reusable query workspace
     DevBuf<uint8_t> work;

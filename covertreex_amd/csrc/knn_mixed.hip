@@ -7,13 +7,14 @@
 //
 //   prep      V -> dataset-order BF16 hi/lo split (V = hi + lo + r, |r| <=
 //             2^-18 |V|), row norms |V_i| rounded up.
-//   screen    one workgroup per 128-query tile, four wavefronts of 64 x 64;
-//             V.V on MFMA (v_mfma_f32_32x32x16_bf16) as hi.hi + (hi.lo +
-//             lo.hi) in two FP32 accumulators, kernel value in FP32, and a
-//             certified interval [lb, ub] containing the exact distance.
-//             pass 1: per-query histogram of ub -> tau_q, an upper bound on
-//                     the k-th exact distance;
-//             pass 2: every candidate with lb <= tau_q is appended to the
+//   screen    one workgroup per 128-query tile x 2048-candidate segment, four
+//             wavefronts of 64 x 64; V.V on MFMA (v_mfma_f32_32x32x16_bf16)
+//             as hi.hi + (hi.lo + lo.hi) in two FP32 accumulators, kernel
+//             value in FP32, and a certified interval [lb2, ub2] containing
+//             the exact squared distance 1 - |rho|.
+//             pass 1: per-query histogram of ub2 -> tau_q, an upper bound on
+//                     the k-th exact squared distance;
+//             pass 2: every candidate with lb2 <= tau_q is appended to the
 //                     query's list (a superset of the exact k nearest).
 //   rescore   one wavefront per query: compensated FP32 (double-float)
 //             distance for each listed candidate, exact top-k, ascending
@@ -33,7 +34,8 @@ constexpr int kQT = 128;     // queries per workgroup tile
 constexpr int kCT = 128;     // candidates per candidate tile
 constexpr int kKC = 32;      // K chunk staged in LDS
 constexpr int kKP = kKC + 8; // padded LDS row (bf16)
-constexpr int kBins = 64;    // histogram bins of the upper bound over [0, 1]
+constexpr int kBins = 128;   // histogram bins of the squared upper bound over [0, 1)
+constexpr float kHistCap = 0.99f; // squared bounds at or above this are not counted
 constexpr int kCap = 2048;   // listed candidates per query before overflow
 constexpr int kMaxD = 16;
 constexpr int kMixMaxK = 256;
@@ -85,7 +87,7 @@ __global__ void split_kernel(const float *__restrict__ v, const int64_t *__restr
 
 __global__ void gather_meta_kernel(const float *__restrict__ raw, const float *__restrict__ p,
                                    const int64_t *__restrict__ inv_order, int64_t n, int32_t d,
-                                   float *__restrict__ x_ds, float *__restrict__ p_ds)
+                                   float *__restrict__ x_ds, float *__restrict__ isp_ds)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -93,7 +95,8 @@ __global__ void gather_meta_kernel(const float *__restrict__ raw, const float *_
     const int64_t t = inv_order[i];
     for (int32_t c = 0; c < d; c++)
         x_ds[i * d + c] = raw[t * d + c];
-    p_ds[i] = p[t];
+    // correctly rounded sqrt and division: relative error <= 2^-23
+    isp_ds[i] = 1.0f / __builtin_sqrtf(p[t]);
 }
 
 // ---- screen ------------------------------------------------------------------
@@ -101,7 +104,7 @@ __global__ void gather_meta_kernel(const float *__restrict__ raw, const float *_
 struct ScreenArgs {
     const uint16_t *vhi, *vlo;
     const float *x;     // n x d raw coords, dataset order
-    const float *p;     // n
+    const float *isp;   // n: 1/sqrt(p), relative error <= 2^-23
     const float *vnorm; // n
     const int64_t *qsorted;  // sorted query dataset indices (nq)
     const int32_t *qrow;     // dataset row used for each sorted query
@@ -137,7 +140,8 @@ __global__ void __launch_bounds__(256) screen_kernel(ScreenArgs A)
     __shared__ int32_t sQlive[kQT];
     __shared__ int32_t sQrow[kQT];
     __shared__ float sCx[kCT * kMaxD], sCp[kCT], sCn[kCT];
-    __shared__ int32_t sHist[PASS == 1 ? kQT * kBins : 1];
+    // pass-1 histogram: two 16-bit bins per word (at most kSeg counts per bin)
+    __shared__ uint32_t sHist[PASS == 1 ? kQT * kBins / 2 : 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     // blockIdx.x = query tile, blockIdx.y = candidate segment; heavy tiles first
@@ -165,14 +169,14 @@ __global__ void __launch_bounds__(256) screen_kernel(ScreenArgs A)
         sQlive[i] = live ? 1 : 0;
         for (int c = 0; c < d; c++)
             sQx[i * kMaxD + c] = A.x[(int64_t)row * d + c];
-        sQp[i] = A.p[row];
+        sQp[i] = A.isp[row];
         sQn[i] = A.vnorm[row];
         if (PASS == 2)
             sQtau[i] = live ? A.tau[s] : -1.0f;
     }
     if (PASS == 1)
-        for (int i = tid; i < kQT * kBins; i += 256)
-            sHist[i] = 0;
+        for (int i = tid; i < kQT * kBins / 2; i += 256)
+            sHist[i] = 0u;
     __syncthreads();
 
     int64_t c_end = A.n;
@@ -192,7 +196,7 @@ __global__ void __launch_bounds__(256) screen_kernel(ScreenArgs A)
             const int64_t jr = j < A.n ? j : 0;
             for (int c = 0; c < d; c++)
                 sCx[i * kMaxD + c] = A.x[jr * d + c];
-            sCp[i] = A.p[jr];
+            sCp[i] = A.isp[jr];
             sCn[i] = A.vnorm[jr];
         }
         floatx16 acc_hh[2][2], acc_x[2][2];
@@ -262,10 +266,11 @@ __global__ void __launch_bounds__(256) screen_kernel(ScreenArgs A)
                         continue;
                     if (A.pred && qid >= 0 && j >= qid) // negative index: no bound
                         continue;
-                    float lb, ub;
+                    // squared bounds on d^2 = 1 - |rho|
+                    float lb2, ub2;
                     if (j == (int64_t)sQrow[ql]) {
-                        lb = 0.0f;
-                        ub = 0.0f;
+                        lb2 = 0.0f;
+                        ub2 = 0.0f;
                     } else {
                         float d2 = 0.0f;
                         for (int c = 0; c < d; c++) {
@@ -274,30 +279,27 @@ __global__ void __launch_bounds__(256) screen_kernel(ScreenArgs A)
                         }
                         const float kv = fast_kernel(d2, A.var, A.kernel);
                         const float dot = acc_hh[a][b][r] + acc_x[a][b][r];
-                        const float nrm = sQn[ql] * sCn[cl];
-                        float den2 = sQp[ql] * sCp[cl];
-                        den2 = den2 > 1e-18f ? den2 : 1e-18f;
-                        const float den = __builtin_sqrtf(den2);
+                        // 1/sqrt(max(p_q p_j, 1e-18)) from per-row 1/sqrt(p)
+                        float w = sQp[ql] * sCp[cl];
+                        w = w < 1e9f ? w : 1e9f;
                         const float num = kv - dot;
-                        const float rho = num / den;
+                        const float ar = __builtin_fabsf(num * w);
                         const float erho =
-                            ((A.eK + A.cD * nrm + __builtin_fabsf(num) * 2.4e-7f) / den) *
-                                1.000001f +
-                            __builtin_fabsf(rho) * 4.8e-7f;
-                        const float ar = __builtin_fabsf(rho);
+                            (A.eK + A.cD * (sQn[ql] * sCn[cl]) + __builtin_fabsf(num) * 1.2e-7f) *
+                                w * 1.000002f +
+                            ar * 6e-7f;
                         const float rlo = ar - erho > 0.0f ? ar - erho : 0.0f;
                         const float rhi = ar + erho < 1.0f ? ar + erho : 1.0f;
-                        const float slo = (1.0f - rhi) - 6e-8f;
-                        const float shi = (1.0f - rlo) + 6e-8f;
-                        lb = slo > 0.0f ? __builtin_sqrtf(slo) * 0.9999995f : 0.0f;
-                        ub = __builtin_sqrtf(shi < 1.0f ? shi : 1.0f) * 1.0000005f + 6e-8f;
+                        lb2 = (1.0f - rhi) - 6e-8f;
+                        ub2 = (1.0f - rlo) + 6e-8f;
                     }
                     if (PASS == 1) {
-                        int bin = (int)(ub * (float)kBins);
-                        bin = bin < kBins - 1 ? bin : kBins - 1;
-                        atomicAdd(&sHist[ql * kBins + bin], 1);
+                        if (ub2 < kHistCap) {
+                            const int bin = (int)(ub2 * (float)kBins);
+                            atomicAdd(&sHist[(ql * kBins + bin) >> 1], (bin & 1) ? 65536u : 1u);
+                        }
                     } else {
-                        if (lb <= sQtau[ql]) {
+                        if (lb2 <= sQtau[ql]) {
                             const int64_t s = s0 + ql;
                             const int32_t pos = atomicAdd(&A.cnt[s], 1);
                             if (pos < kCap)
@@ -308,17 +310,23 @@ __global__ void __launch_bounds__(256) screen_kernel(ScreenArgs A)
     }
     if (PASS == 1) {
         __syncthreads();
-        for (int i = tid; i < kQT * kBins; i += 256) {
-            const int64_t s = s0 + i / kBins;
-            const int h = sHist[i];
-            if (s < A.nq && h)
-                atomicAdd(&A.hist[s * kBins + (i % kBins)], h);
+        for (int i = tid; i < kQT * kBins / 2; i += 256) {
+            const int64_t s = s0 + (2 * i) / kBins;
+            const uint32_t h = sHist[i];
+            if (s < A.nq && h) {
+                const int bin = (2 * i) % kBins;
+                if (h & 0xffffu)
+                    atomicAdd(&A.hist[s * kBins + bin], (int32_t)(h & 0xffffu));
+                if (h >> 16)
+                    atomicAdd(&A.hist[s * kBins + bin + 1], (int32_t)(h >> 16));
+            }
         }
     }
 }
 
-// tau_q: upper edge of the first bin where the cumulative count reaches k
-// (2.0, i.e. keep everything, when the query has fewer than k candidates).
+// tau_q (squared scale): upper edge of the first bin where the cumulative count
+// of squared upper bounds reaches k; 2.0 (keep everything) when fewer than k
+// candidates have a squared upper bound below the histogram cap.
 __global__ void tau_kernel(const int32_t *__restrict__ hist, int64_t nq, int32_t k, float *tau)
 {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -577,11 +585,11 @@ static void prepare_mixed(TreeState &t)
     split_kernel<<<(unsigned)n, 256, 0, st>>>(t.v.ptr, t.inv_order.ptr, n, t.m, t.mp, t.vhi.ptr,
                                               t.vlo.ptr, t.vnorm.ptr);
     t.x_ds.alloc((size_t)npad * t.d);
-    t.p_ds.alloc(npad);
+    t.isp_ds.alloc(npad);
     CX_HIP(hipMemsetAsync(t.x_ds.ptr, 0, t.x_ds.bytes(), st));
-    CX_HIP(hipMemsetAsync(t.p_ds.ptr, 0, t.p_ds.bytes(), st));
+    CX_HIP(hipMemsetAsync(t.isp_ds.ptr, 0, t.isp_ds.bytes(), st));
     gather_meta_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
-        t.raw.ptr, t.p.ptr, t.inv_order.ptr, n, t.d, t.x_ds.ptr, t.p_ds.ptr);
+        t.raw.ptr, t.p.ptr, t.inv_order.ptr, n, t.d, t.x_ds.ptr, t.isp_ds.ptr);
     CX_HIP(hipGetLastError());
     CX_HIP(hipStreamSynchronize(st));
     t.mixed_ready = true;
@@ -649,7 +657,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     sa.vhi = t.vhi.ptr;
     sa.vlo = t.vlo.ptr;
     sa.x = t.x_ds.ptr;
-    sa.p = t.p_ds.ptr;
+    sa.isp = t.isp_ds.ptr;
     sa.vnorm = t.vnorm.ptr;
     sa.qsorted = qs;
     sa.qrow = qrow;
