@@ -34,6 +34,21 @@ BUILD_CASES = [
 ]
 
 
+def _same_sets_modulo_cut_ties(hi, hd, oi, od):
+    ties = 0
+    for r in range(hi.shape[0]):
+        if np.array_equal(hi[r], oi[r]):
+            continue
+        valid = hi[r] >= 0
+        cut = hd[r][valid].max()
+        below = hd[r] < cut
+        assert set(hi[r][below & valid]) == set(oi[r][(od[r] < cut) & (oi[r] >= 0)])
+        for dv in np.unique(hd[r][below & valid]):
+            assert set(hi[r][hd[r] == dv]) == set(oi[r][od[r] == dv])
+        ties += 1
+    return ties
+
+
 def _tree_pair(hip, case, seed=0):
     from covertreex_amd.native import HipTree
 
@@ -77,13 +92,10 @@ def test_reference_form_knn_matches_oracle(hip, case, k, pred):
     q = np.arange(case["n"])
     oi, od, _ = oracle.knn_residual(ot, q, k, predecessor=pred, budget=())
     hi, hd = ht.knn_residual(q, k, predecessor_mode=pred, mode="reference")
-    # distances bit-exact; index sets equal (order may differ only within
-    # exactly equal f32 distances)
+    # distances bit-exact; index sets equal except which members of an exact
+    # f32 tie at the k-th distance survive the cut
     assert np.array_equal(hd.view(np.uint32), od.view(np.uint32))
-    for r in range(q.size):
-        if not np.array_equal(hi[r], oi[r]):
-            for dv in np.unique(hd[r]):
-                assert set(hi[r][hd[r] == dv]) == set(oi[r][od[r] == dv])
+    _same_sets_modulo_cut_ties(hi, hd, oi, od)
 
 
 @pytest.mark.parametrize("case,k,pred", [
@@ -100,6 +112,7 @@ def test_budgeted_reference_form_matches_oracle(hip, case, k, pred):
     oi, od, _ = oracle.knn_residual(ot, q, k, predecessor=pred, budget="default")
     hi, hd = ht.knn_residual(q, k, predecessor_mode=pred, mode="reference-budgeted")
     assert np.array_equal(hd.view(np.uint32), od.view(np.uint32))
+    _same_sets_modulo_cut_ties(hi, hd, oi, od)
 
 
 def test_reference_form_edge_queries(hip):
@@ -126,3 +139,4 @@ def test_c2_shape_build_and_reference_queries(hip):
     oi, od, _ = oracle.knn_residual(ot, q, 50, predecessor=True, budget=())
     hi, hd = ht.knn_residual(q, 50, predecessor_mode=True, mode="reference")
     assert np.array_equal(hd.view(np.uint32), od.view(np.uint32))
+    _same_sets_modulo_cut_ties(hi, hd, oi, od)

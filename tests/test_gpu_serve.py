@@ -45,3 +45,30 @@ def test_admission_by_footprint(hip):
     assert np.array_equal(r1.indices, r2.indices)
     with pytest.raises(ValueError):
         Server(max_inflight=1, hbm_budget_bytes=10).submit(req)
+
+
+def test_api_and_factory_surface(hip):
+    # cover_tree(kernel=...).knn and CoverTree(Runtime).fit/knn: the reference's
+    # predecessor invariants (tests/test_predecessor_mode.py:638-686)
+    import covertreex_amd as cx
+    from covertreex_amd.api import CoverTree, Residual, Runtime
+
+    pts = np.random.default_rng(42).standard_normal((500, 3)).astype(np.float32)
+    tree = cx.cover_tree(pts, kernel=cx.Matern52(lengthscale=1.0), engine="hip")
+    nb, dist = tree.knn(k=30, predecessor_mode=True, return_distances=True)
+    for i in range(500):
+        valid = nb[i][nb[i] >= 0]
+        assert np.all(valid < i) and valid.size == min(30, i)
+    assert np.all(nb[0] == -1)
+    assert set(nb[1][nb[1] >= 0]) == {0}
+    rt = Runtime(metric="residual_correlation",
+                 residual=Residual(kernel_type=1, inducing_count=128, seed=42))
+    api = CoverTree(rt).fit(pts)
+    nb2 = api.knn(np.arange(500).reshape(-1, 1), k=30, predecessor_mode=True)
+    v128 = cx.cover_tree(pts, kernel=cx.Matern52(1.0), inducing_count=128).knn(
+        k=30, predecessor_mode=True)
+    assert np.array_equal(nb2, v128)
+    ref = cx.cover_tree(pts, kernel=cx.Matern52(1.0), precision="reference-f32")
+    nr = ref.knn(k=30, predecessor_mode=True)
+    for i in range(500):
+        assert np.all(nr[i][nr[i] >= 0] < i)
