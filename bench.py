@@ -73,6 +73,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
@@ -80,6 +82,19 @@ def main() -> int:
         return 2
 
     import torch
+
+    if args.copy_calibration:
+        # a plain device copy of known size: what FETCH_SIZE / WRITE_SIZE read
+        # for it scales those counters to bytes
+        nbytes = 1 << 30
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}))
+        return 0
 
     from covertreex_amd.native import HipTree
 
