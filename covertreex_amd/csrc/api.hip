@@ -40,6 +40,8 @@ TreeState::~TreeState()
     x_ds.release();
     isp_ds.release();
     work.release();
+    if (host_word)
+        (void)hipHostFree(host_word);
     for (int i = 0; i < kMaxMarks; i++)
         if (marks[i])
             (void)hipEventDestroy(marks[i]);

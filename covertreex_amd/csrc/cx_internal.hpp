@@ -117,6 +117,9 @@ struct TreeState {
 
     // reusable query workspace
     DevBuf<uint8_t> work;
+    // pinned host word for small device-to-host reads (hipHostMalloc)
+    int32_t *host_word = nullptr;
+    int32_t last_hard = 0; // queries scored exhaustively by the last mixed kNN
 
     // optional per-phase timing of the last query: events recorded on the
     // tree's stream at phase boundaries (see cx_tree_last_profile)

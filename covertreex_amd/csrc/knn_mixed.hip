@@ -40,6 +40,7 @@ constexpr int kCap = 2048;   // listed candidates per query before overflow
 constexpr int kMaxD = 16;
 constexpr int kMixMaxK = 256;
 constexpr int64_t kSeg = 16 * kCT; // candidates per workgroup (grid y)
+constexpr int kScreenThreads = 512; // eight wavefronts
 
 // ---- prep ------------------------------------------------------------------
 
@@ -102,13 +103,14 @@ __global__ void gather_meta_kernel(const float *__restrict__ raw, const float *_
 // ---- screen ------------------------------------------------------------------
 
 struct ScreenArgs {
-    const uint16_t *vhi, *vlo;
+    const uint16_t *vhi, *vlo; // npad x mp, dataset order, zero rows past n
     const float *x;     // n x d raw coords, dataset order
     const float *isp;   // n: 1/sqrt(p), relative error <= 2^-23
     const float *vnorm; // n
     const int64_t *qsorted;  // sorted query dataset indices (nq)
     const int32_t *qrow;     // dataset row used for each sorted query
-    int64_t n, nq;
+    int32_t n;
+    int64_t nq;
     int32_t d, mp, kernel, pred, k;
     float var;
     float inv_ls[kMaxD];
@@ -120,197 +122,220 @@ struct ScreenArgs {
     int32_t n_tiles;
 };
 
+// Kernel value in FP32 from the hardware exp2/sqrt (about 1 ulp each); its
+// error is far below eK = var * 2^-18.
 __device__ __forceinline__ float fast_kernel(float d2, float var, int32_t kernel)
 {
+    constexpr float kLog2e = 1.44269504f;
     if (kernel == 0)
-        return var * __expf(-0.5f * d2);
-    float a = __builtin_sqrtf(5.0f * d2);
-    return var * (1.0f + a + a * a * 0.333333343f) * __expf(-a);
+        return var * __builtin_amdgcn_exp2f(d2 * (-0.5f * kLog2e));
+    const float a = __builtin_amdgcn_sqrtf(5.0f * d2);
+    const float poly = __builtin_fmaf(a, __builtin_fmaf(a, 0.333333343f, 1.0f), 1.0f);
+    return var * poly * __builtin_amdgcn_exp2f(-a * kLog2e);
 }
 
+// One workgroup = 128 candidates (MFMA rows) x 128 queries (MFMA columns) per
+// candidate tile, looping over a 2048-candidate segment.  Eight wavefronts:
+// wavefront w owns candidate rows 32*(w&3).. and query columns 64*(w>>2)..,
+// i.e. two 32x32 output blocks, each with a hi.hi and a cross accumulator.
 template <int PASS>
-__global__ void __launch_bounds__(256) screen_kernel(ScreenArgs A)
+__global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t sAh[kQT * kKP];
-    __shared__ __attribute__((aligned(16))) uint16_t sAl[kQT * kKP];
-    __shared__ __attribute__((aligned(16))) uint16_t sBh[kCT * kKP];
-    __shared__ __attribute__((aligned(16))) uint16_t sBl[kCT * kKP];
-    __shared__ float sQx[kQT * kMaxD], sQp[kQT], sQn[kQT], sQtau[kQT];
-    __shared__ int64_t sQid[kQT];
-    __shared__ int32_t sQlive[kQT];
+    __shared__ __attribute__((aligned(16))) uint16_t sAh[kCT * kKP];
+    __shared__ __attribute__((aligned(16))) uint16_t sAl[kCT * kKP];
+    __shared__ __attribute__((aligned(16))) uint16_t sBh[kQT * kKP];
+    __shared__ __attribute__((aligned(16))) uint16_t sBl[kQT * kKP];
+    __shared__ __attribute__((aligned(16))) float sCw[kCT];
+    __shared__ __attribute__((aligned(16))) float sCn[kCT];
     __shared__ int32_t sQrow[kQT];
-    __shared__ float sCx[kCT * kMaxD], sCp[kCT], sCn[kCT];
     // pass-1 histogram: two 16-bit bins per word (at most kSeg counts per bin)
     __shared__ uint32_t sHist[PASS == 1 ? kQT * kBins / 2 : 1];
+    // coordinates, structure of arrays: candidates [d][kCT], then queries [d][kQT]
+    extern __shared__ __attribute__((aligned(16))) float sDyn[];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int cb = wid & 3, qh = wid >> 2;
     // blockIdx.x = query tile, blockIdx.y = candidate segment; heavy tiles first
     const int32_t tile = A.n_tiles - 1 - (int32_t)blockIdx.x;
     const int64_t s0 = (int64_t)tile * kQT;
-    {
+    const int32_t n = A.n, d = A.d, mp = A.mp;
+    int32_t c_end = n;
+    if (A.pred) {
         // candidate range of this tile: [0, largest query index) under the mask
-        int64_t c_lim = A.n;
-        if (A.pred) {
-            const int64_t last = s0 + kQT - 1 < A.nq - 1 ? s0 + kQT - 1 : A.nq - 1;
-            const int64_t qmax = A.qsorted[last];
-            c_lim = qmax < 0 ? A.n : (qmax < A.n ? qmax : A.n);
-        }
-        if ((int64_t)blockIdx.y * kSeg >= c_lim)
-            return;
+        const int64_t last = s0 + kQT - 1 < A.nq - 1 ? s0 + kQT - 1 : A.nq - 1;
+        const int64_t qmax = A.qsorted[last];
+        c_end = qmax < 0 ? n : (qmax < n ? (int32_t)qmax : n);
     }
-    const int32_t d = A.d, mp = A.mp;
+    const int32_t c_begin = (int32_t)blockIdx.y * (int32_t)kSeg;
+    if (c_begin >= c_end)
+        return;
+    c_end = c_end < c_begin + (int32_t)kSeg ? c_end : c_begin + (int32_t)kSeg;
+    float *sCx = sDyn, *sQx = sDyn + d * kCT;
 
-    for (int i = tid; i < kQT; i += 256) {
+    for (int i = tid; i < kQT; i += kScreenThreads) {
         const int64_t s = s0 + i;
-        const bool live = s < A.nq;
-        const int32_t row = live ? A.qrow[s] : 0;
+        const int32_t row = s < A.nq ? A.qrow[s] : 0;
         sQrow[i] = row;
-        sQid[i] = live ? A.qsorted[s] : 0;
-        sQlive[i] = live ? 1 : 0;
         for (int c = 0; c < d; c++)
-            sQx[i * kMaxD + c] = A.x[(int64_t)row * d + c];
-        sQp[i] = A.isp[row];
-        sQn[i] = A.vnorm[row];
-        if (PASS == 2)
-            sQtau[i] = live ? A.tau[s] : -1.0f;
+            sQx[c * kQT + i] = A.x[(int64_t)row * d + c];
     }
     if (PASS == 1)
-        for (int i = tid; i < kQT * kBins / 2; i += 256)
+        for (int i = tid; i < kQT * kBins / 2; i += kScreenThreads)
             sHist[i] = 0u;
-    __syncthreads();
 
-    int64_t c_end = A.n;
-    if (A.pred) {
-        const int64_t last = s0 + kQT - 1 < A.nq - 1 ? s0 + kQT - 1 : A.nq - 1;
-        int64_t qmax = A.qsorted[last];
-        c_end = qmax < 0 ? A.n : (qmax < A.n ? qmax : A.n);
+    // this lane's two query columns: mask limit, own row, weight, norm term,
+    // and (pass 2) the emission threshold on the |rho| upper bound
+    int32_t qcol[2], qlim[2], qself[2];
+    float qw[2], qcn[2], qthr[2];
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+        qcol[b] = 64 * qh + 32 * b + (lane & 31);
+        const int64_t s = s0 + qcol[b];
+        const bool live = s < A.nq;
+        const int64_t qid = live ? A.qsorted[s] : 0;
+        const int32_t row = live ? A.qrow[s] : 0;
+        qlim[b] = !live ? 0 : (A.pred ? (qid < 0 ? n : (qid < n ? (int32_t)qid : n)) : n);
+        qself[b] = live ? row : -1;
+        qw[b] = A.isp[row];
+        qcn[b] = A.cD * A.vnorm[row];
+        qthr[b] = 2.0f;
+        if (PASS == 2 && live) {
+            const float t = A.tau[s];
+            // |rho| >= 1 - tau is implied by d^2 <= tau; 2^-22 covers rounding
+            qthr[b] = t >= 2.0f ? -1.0f : (1.0f - t) - 2.3841858e-7f;
+        }
     }
-    const int64_t c_begin = (int64_t)blockIdx.y * kSeg;
-    c_end = c_end < c_begin + kSeg ? c_end : c_begin + kSeg;
-    const int wq0 = (wid >> 1) * 64, wc0 = (wid & 1) * 64;
 
-    for (int64_t c0 = c_begin; c0 < c_end; c0 += kCT) {
-        __syncthreads(); // the previous tile's epilogue still reads sCx/sCp/sCn
-        for (int i = tid; i < kCT; i += 256) {
-            const int64_t j = c0 + i;
-            const int64_t jr = j < A.n ? j : 0;
+    const int rbase = 32 * cb + 4 * (lane >> 5);
+    for (int32_t c0 = c_begin; c0 < c_end; c0 += kCT) {
+        __syncthreads(); // the previous tile's epilogue still reads sCx/sCw/sCn
+        for (int i = tid; i < kCT; i += kScreenThreads) {
+            const int32_t j = c0 + i;
+            const int32_t jr = j < n ? j : 0;
             for (int c = 0; c < d; c++)
-                sCx[i * kMaxD + c] = A.x[jr * d + c];
-            sCp[i] = A.isp[jr];
+                sCx[c * kCT + i] = A.x[(int64_t)jr * d + c];
+            sCw[i] = A.isp[jr];
             sCn[i] = A.vnorm[jr];
         }
-        floatx16 acc_hh[2][2], acc_x[2][2];
+        floatx16 acc_h[2], acc_x[2];
 #pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-            for (int b = 0; b < 2; b++) {
-                acc_hh[a][b] = floatx16{};
-                acc_x[a][b] = floatx16{};
-            }
+        for (int b = 0; b < 2; b++) {
+            acc_h[b] = floatx16{};
+            acc_x[b] = floatx16{};
+        }
         for (int kc = 0; kc < mp; kc += kKC) {
             __syncthreads();
-            // stage: 128 rows x 32 bf16 (4 x 16 B segments) for hi and lo
+            // stage 128 candidate + 128 query rows x 32 bf16, hi and lo:
+            // 2048 segments of 16 B, four per thread
+            uint4 val[4];
 #pragma unroll
             for (int it = 0; it < 4; it++) {
-                const int idx = tid + 256 * it; // [0, 1024)
-                const int part = idx >> 9, rem = idx & 511;
-                const int row = rem >> 2, seg = rem & 3;
-                const uint16_t *srcA = (part ? A.vlo : A.vhi) +
-                                       (int64_t)sQrow[row] * mp + kc + seg * 8;
-                uint16_t *dstA = (part ? sAl : sAh) + row * kKP + seg * 8;
-                *(uint4 *)dstA = *(const uint4 *)srcA;
-                const int64_t crow = c0 + row < A.n ? c0 + row : 0;
-                const uint16_t *srcB = (part ? A.vlo : A.vhi) + crow * mp + kc + seg * 8;
-                uint16_t *dstB = (part ? sBl : sBh) + row * kKP + seg * 8;
-                *(uint4 *)dstB = *(const uint4 *)srcB;
+                const int idx = tid + kScreenThreads * it;
+                const int side = idx >> 10, part = (idx >> 9) & 1;
+                const int row = (idx >> 2) & 127, seg = idx & 3;
+                const int64_t vrow = side ? (int64_t)sQrow[row] : (int64_t)(c0 + row);
+                const uint16_t *src = (part ? A.vlo : A.vhi) + vrow * mp + kc + seg * 8;
+                val[it] = *(const uint4 *)src;
+            }
+#pragma unroll
+            for (int it = 0; it < 4; it++) {
+                const int idx = tid + kScreenThreads * it;
+                const int side = idx >> 10, part = (idx >> 9) & 1;
+                const int row = (idx >> 2) & 127, seg = idx & 3;
+                uint16_t *dst = (side ? (part ? sBl : sBh) : (part ? sAl : sAh)) + row * kKP +
+                                seg * 8;
+                *(uint4 *)dst = val[it];
             }
             __syncthreads();
 #pragma unroll
             for (int ks = 0; ks < kKC; ks += 16) {
                 const int koff = ks + 8 * (lane >> 5);
-                bf16x8 ah[2], al[2], bh[2], bl[2];
+                const int ar = 32 * cb + (lane & 31);
+                const bf16x8 ah = __builtin_bit_cast(bf16x8, *(const uint4 *)(sAh + ar * kKP + koff));
+                const bf16x8 al = __builtin_bit_cast(bf16x8, *(const uint4 *)(sAl + ar * kKP + koff));
 #pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    const int ar = wq0 + 32 * t + (lane & 31);
-                    const int br = wc0 + 32 * t + (lane & 31);
-                    ah[t] = __builtin_bit_cast(bf16x8, *(const uint4 *)(sAh + ar * kKP + koff));
-                    al[t] = __builtin_bit_cast(bf16x8, *(const uint4 *)(sAl + ar * kKP + koff));
-                    bh[t] = __builtin_bit_cast(bf16x8, *(const uint4 *)(sBh + br * kKP + koff));
-                    bl[t] = __builtin_bit_cast(bf16x8, *(const uint4 *)(sBl + br * kKP + koff));
+                for (int b = 0; b < 2; b++) {
+                    const int br = qcol[b];
+                    const bf16x8 bh =
+                        __builtin_bit_cast(bf16x8, *(const uint4 *)(sBh + br * kKP + koff));
+                    const bf16x8 bl =
+                        __builtin_bit_cast(bf16x8, *(const uint4 *)(sBl + br * kKP + koff));
+                    acc_h[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc_h[b], 0, 0, 0);
+                    acc_x[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc_x[b], 0, 0, 0);
+                    acc_x[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc_x[b], 0, 0, 0);
                 }
-#pragma unroll
-                for (int a = 0; a < 2; a++)
-#pragma unroll
-                    for (int b = 0; b < 2; b++) {
-                        acc_hh[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            ah[a], bh[b], acc_hh[a][b], 0, 0, 0);
-                        acc_x[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            ah[a], bl[b], acc_x[a][b], 0, 0, 0);
-                        acc_x[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            al[a], bh[b], acc_x[a][b], 0, 0, 0);
-                    }
             }
         }
-        // epilogue: certified interval per (query, candidate)
+        // epilogue: output register r of block b is candidate row
+        // rbase + 8*(r/4) + r%4, query column qcol[b]
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int g = 0; g < 4; g++) {
+            const int rg = rbase + 8 * g;
+            const float4 cw = *(const float4 *)(sCw + rg);
+            const float4 cn = *(const float4 *)(sCn + rg);
+            float d2[2][4];
 #pragma unroll
             for (int b = 0; b < 2; b++)
 #pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    const int ql = wq0 + 32 * a + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                    const int cl = wc0 + 32 * b + (lane & 31);
-                    const int64_t qid = sQid[ql];
-                    const int64_t j = c0 + cl;
-                    if (!sQlive[ql] || j >= A.n)
-                        continue;
-                    if (A.pred && qid >= 0 && j >= qid) // negative index: no bound
-                        continue;
-                    // squared bounds on d^2 = 1 - |rho|
-                    float lb2, ub2;
-                    if (j == (int64_t)sQrow[ql]) {
-                        lb2 = 0.0f;
-                        ub2 = 0.0f;
-                    } else {
-                        float d2 = 0.0f;
-                        for (int c = 0; c < d; c++) {
-                            float df_ = (sQx[ql * kMaxD + c] - sCx[cl * kMaxD + c]) * A.inv_ls[c];
-                            d2 += df_ * df_;
-                        }
-                        const float kv = fast_kernel(d2, A.var, A.kernel);
-                        const float dot = acc_hh[a][b][r] + acc_x[a][b][r];
-                        // 1/sqrt(max(p_q p_j, 1e-18)) from per-row 1/sqrt(p)
-                        float w = sQp[ql] * sCp[cl];
-                        w = w < 1e9f ? w : 1e9f;
-                        const float num = kv - dot;
-                        const float ar = __builtin_fabsf(num * w);
-                        const float erho =
-                            (A.eK + A.cD * (sQn[ql] * sCn[cl]) + __builtin_fabsf(num) * 1.2e-7f) *
-                                w * 1.000002f +
-                            ar * 6e-7f;
-                        const float rlo = ar - erho > 0.0f ? ar - erho : 0.0f;
-                        const float rhi = ar + erho < 1.0f ? ar + erho : 1.0f;
-                        lb2 = (1.0f - rhi) - 6e-8f;
-                        ub2 = (1.0f - rlo) + 6e-8f;
-                    }
+                for (int rr = 0; rr < 4; rr++)
+                    d2[b][rr] = 0.0f;
+            for (int c = 0; c < d; c++) {
+                const float4 cx = *(const float4 *)(sCx + c * kCT + rg);
+                const float il = A.inv_ls[c];
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    const float qx = sQx[c * kQT + qcol[b]];
+                    const float t0 = (qx - cx.x) * il, t1 = (qx - cx.y) * il;
+                    const float t2 = (qx - cx.z) * il, t3 = (qx - cx.w) * il;
+                    d2[b][0] = __builtin_fmaf(t0, t0, d2[b][0]);
+                    d2[b][1] = __builtin_fmaf(t1, t1, d2[b][1]);
+                    d2[b][2] = __builtin_fmaf(t2, t2, d2[b][2]);
+                    d2[b][3] = __builtin_fmaf(t3, t3, d2[b][3]);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) {
+                    const int r = 4 * g + rr;
+                    const int32_t j = c0 + rg + rr;
+                    const float cwv = rr == 0 ? cw.x : rr == 1 ? cw.y : rr == 2 ? cw.z : cw.w;
+                    const float cnv = rr == 0 ? cn.x : rr == 1 ? cn.y : rr == 2 ? cn.z : cn.w;
+                    const float kv = fast_kernel(d2[b][rr], A.var, A.kernel);
+                    const float num = kv - (acc_h[b][r] + acc_x[b][r]);
+                    const float an = __builtin_fabsf(num);
+                    // 1/sqrt(max(p_q p_j, 1e-18)) from per-row 1/sqrt(p)
+                    const float w = __builtin_fminf(qw[b] * cwv, 1e9f);
+                    const float ar = an * w;
+                    // certified |rho| error: kernel, dot (cD |V_q||V_j|), the
+                    // numerator's rounding, the weight's and the products'
+                    const float e1 = __builtin_fmaf(an, 1.2e-7f, __builtin_fmaf(qcn[b], cnv, A.eK));
+                    const float erho = __builtin_fmaf(ar, 6e-7f, e1 * (w * 1.000002f));
+                    const bool valid = j < qlim[b];
+                    const bool self = j == qself[b];
                     if (PASS == 1) {
-                        if (ub2 < kHistCap) {
+                        // squared upper bound 1 - (|rho| lower bound), rounded up
+                        float ub2 = (1.0f - (ar - erho)) + 1.5e-7f;
+                        ub2 = self ? 0.0f : ub2;
+                        if (valid && ub2 < kHistCap) {
                             const int bin = (int)(ub2 * (float)kBins);
-                            atomicAdd(&sHist[(ql * kBins + bin) >> 1], (bin & 1) ? 65536u : 1u);
+                            atomicAdd(&sHist[(qcol[b] * kBins + bin) >> 1],
+                                      (bin & 1) ? 65536u : 1u);
                         }
                     } else {
-                        if (lb2 <= sQtau[ql]) {
-                            const int64_t s = s0 + ql;
+                        if (valid && (ar + erho >= qthr[b] || self)) {
+                            const int64_t s = s0 + qcol[b];
                             const int32_t pos = atomicAdd(&A.cnt[s], 1);
                             if (pos < kCap)
-                                A.list[s * kCap + pos] = (int32_t)j;
+                                A.list[s * kCap + pos] = j;
                         }
                     }
                 }
+        }
     }
     if (PASS == 1) {
         __syncthreads();
-        for (int i = tid; i < kQT * kBins / 2; i += 256) {
+        for (int i = tid; i < kQT * kBins / 2; i += kScreenThreads) {
             const int64_t s = s0 + (2 * i) / kBins;
             const uint32_t h = sHist[i];
             if (s < A.nq && h) {
@@ -437,21 +462,19 @@ __global__ void __launch_bounds__(64) rescore_kernel(RescoreArgs A)
         __syncthreads();
         if (s >= A.nq)
             break;
-        const int64_t qid = A.qsorted[s];
         const int64_t pos_out = A.qperm[s];
         const int32_t qrow = A.qrow[s];
         const int64_t qt = A.inv_order[qrow];
+        const int32_t c = A.cnt[s];
+        if (c > kCap)
+            continue; // overflowed list: scored by the hard-query kernels
         for (int j = lane; j < k; j += 64) {
             A.out_idx[pos_out * k + j] = -1;
             A.out_dist[pos_out * k + j] = 3.40282347e38f;
             if (A.out_err)
                 A.out_err[pos_out * k + j] = 0.0f;
         }
-        // Vecchia bound on the original index (negative -> unbounded)
-        const int64_t limit = A.pred ? (qid < 0 ? A.n : (qid < A.n ? qid : A.n)) : A.n;
-        const int32_t c = A.cnt[s];
-        const bool overflow = c > kCap;
-        const int64_t total = overflow ? limit : (int64_t)c;
+        const int64_t total = c;
         int count = 0;
         float maxh = -1.0f, maxl = 0.0f;
         int32_t maxj = -1;
@@ -461,7 +484,7 @@ __global__ void __launch_bounds__(64) rescore_kernel(RescoreArgs A)
             const bool live = e < total;
             int64_t j = 0;
             if (live)
-                j = overflow ? e : (int64_t)A.list[s * kCap + e];
+                j = A.list[s * kCap + e];
             df dist = {0.0f, 0.0f};
             float err = 0.0f;
             if (live) {
@@ -548,6 +571,177 @@ __global__ void __launch_bounds__(64) rescore_kernel(RescoreArgs A)
     }
 }
 
+// ---- hard queries ------------------------------------------------------------
+// A query whose list overflowed (its screen interval was too wide, e.g. tiny
+// p) is scored over all its valid candidates: one workgroup per (query,
+// 4096-candidate segment) keeps a running top-k of compensated distances in
+// LDS; a second kernel merges the per-segment lists.
+
+constexpr int kHardChunk = 1024;             // candidates per sort round
+constexpr int kHardSeg = 4 * kHardChunk;     // candidates per workgroup
+constexpr int kHardSort = 2048;              // running top-k + one chunk, padded
+constexpr int kHardBatch = 2048;             // hard queries per launch pair
+
+struct HardEntry {
+    float dh, dl, err;
+    int32_t j;
+};
+
+struct HardArgs {
+    RescoreArgs r;
+    const int32_t *hard; // sorted positions of the hard queries
+    int32_t h0, hn;      // this batch: hard[h0 .. h0 + hn)
+    int32_t nseg;        // segments per query
+    HardEntry *part;     // hn x nseg x k
+    int32_t *counter;
+};
+
+__device__ __forceinline__ bool entry_lt(float ah, float al, int32_t aj, float bh, float bl,
+                                         int32_t bj)
+{
+    return ah < bh || (ah == bh && (al < bl || (al == bl && aj < bj)));
+}
+
+// ascending bitonic sort of kHardSort (distance hi, lo, index) entries in LDS
+__device__ void sort_entries(float *h, float *l, float *e, int32_t *jv)
+{
+    for (int size = 2; size <= kHardSort; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (int t = threadIdx.x; t < kHardSort / 2; t += blockDim.x) {
+                const int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const bool sw = up ? entry_lt(h[hi], l[hi], jv[hi], h[lo], l[lo], jv[lo])
+                                   : entry_lt(h[lo], l[lo], jv[lo], h[hi], l[hi], jv[hi]);
+                if (sw) {
+                    float a = h[lo]; h[lo] = h[hi]; h[hi] = a;
+                    a = l[lo]; l[lo] = l[hi]; l[hi] = a;
+                    a = e[lo]; e[lo] = e[hi]; e[hi] = a;
+                    const int32_t b = jv[lo]; jv[lo] = jv[hi]; jv[hi] = b;
+                }
+            }
+        }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void clear_entries(float *h, float *l, float *e, int32_t *jv, int from)
+{
+    for (int t = from + (int)threadIdx.x; t < kHardSort; t += blockDim.x) {
+        h[t] = __builtin_huge_valf();
+        l[t] = 0.0f;
+        e[t] = 0.0f;
+        jv[t] = 0x7fffffff;
+    }
+}
+
+__device__ __forceinline__ int64_t vecchia_limit(const RescoreArgs &A, int64_t qid)
+{
+    // Vecchia bound on the original index (negative -> unbounded)
+    return A.pred ? (qid < 0 ? A.n : (qid < A.n ? qid : A.n)) : A.n;
+}
+
+__global__ void hard_collect_kernel(const int32_t *__restrict__ cnt, int64_t nq, int32_t *hard,
+                                    int32_t *n_hard)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nq && cnt[s] > kCap)
+        hard[atomicAdd(n_hard, 1)] = (int32_t)s;
+}
+
+__global__ void __launch_bounds__(256) hard_score_kernel(HardArgs H)
+{
+    __shared__ float sh[kHardSort], sl[kHardSort], se[kHardSort];
+    __shared__ int32_t sj[kHardSort];
+    __shared__ int32_t s_item;
+    const RescoreArgs &A = H.r;
+    const int32_t k = A.k;
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_item = atomicAdd(H.counter, 1);
+        __syncthreads();
+        const int32_t item = s_item;
+        __syncthreads();
+        if (item >= H.hn * H.nseg)
+            break;
+        const int32_t hl = item / H.nseg, seg = item % H.nseg;
+        const int64_t s = H.hard[H.h0 + hl];
+        const int32_t qrow = A.qrow[s];
+        const int64_t qt = A.inv_order[qrow];
+        const int64_t limit = vecchia_limit(A, A.qsorted[s]);
+        const int64_t beg = (int64_t)seg * kHardSeg;
+        if (beg >= limit)
+            continue; // the merge reads only segments below the limit
+        const int64_t end = beg + kHardSeg < limit ? beg + kHardSeg : limit;
+        clear_entries(sh, sl, se, sj, 0);
+        for (int64_t c0 = beg; c0 < end; c0 += kHardChunk) {
+            __syncthreads();
+            // the running top-k sits in [0, k); this chunk fills [k, k + chunk)
+            for (int t = threadIdx.x; t < kHardChunk; t += blockDim.x) {
+                const int64_t j = c0 + t;
+                float dh = __builtin_huge_valf(), dl = 0.0f, er = 0.0f;
+                int32_t jj = 0x7fffffff;
+                if (j < end) {
+                    jj = (int32_t)j;
+                    if (j == qrow) {
+                        dh = 0.0f;
+                    } else {
+                        const df dd = accurate_distance(A, qt, A.inv_order[j], &er);
+                        dh = dd.hi;
+                        dl = dd.lo;
+                    }
+                }
+                sh[k + t] = dh;
+                sl[k + t] = dl;
+                se[k + t] = er;
+                sj[k + t] = jj;
+            }
+            sort_entries(sh, sl, se, sj);
+        }
+        HardEntry *out = H.part + ((int64_t)hl * H.nseg + seg) * k;
+        for (int t = threadIdx.x; t < k; t += blockDim.x)
+            out[t] = HardEntry{sh[t], sl[t], se[t], sj[t]};
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) hard_merge_kernel(HardArgs H)
+{
+    __shared__ float sh[kHardSort], sl[kHardSort], se[kHardSort];
+    __shared__ int32_t sj[kHardSort];
+    const RescoreArgs &A = H.r;
+    const int32_t k = A.k;
+    for (int32_t hl = blockIdx.x; hl < H.hn; hl += gridDim.x) {
+        const int64_t s = H.hard[H.h0 + hl];
+        const int64_t limit = vecchia_limit(A, A.qsorted[s]);
+        const int32_t nseg = (int32_t)((limit + kHardSeg - 1) / kHardSeg);
+        const int32_t per = (kHardSort - k) / k; // lists merged per round
+        __syncthreads();
+        clear_entries(sh, sl, se, sj, 0);
+        for (int32_t g = 0; g < nseg; g += per) {
+            const int32_t nl = nseg - g < per ? nseg - g : per;
+            __syncthreads();
+            // entries left past k + nl*k by an earlier round rank behind the
+            // current top-k, so they never re-enter it
+            for (int t = threadIdx.x; t < nl * k; t += blockDim.x) {
+                const HardEntry e = H.part[((int64_t)hl * H.nseg + g + t / k) * k + t % k];
+                sh[k + t] = e.dh;
+                sl[k + t] = e.dl;
+                se[k + t] = e.err;
+                sj[k + t] = e.j;
+            }
+            sort_entries(sh, sl, se, sj);
+        }
+        const int64_t pos_out = A.qperm[s];
+        for (int t = threadIdx.x; t < k; t += blockDim.x) {
+            const bool ok = sh[t] < __builtin_huge_valf();
+            A.out_idx[pos_out * k + t] = ok ? (int64_t)sj[t] : -1;
+            A.out_dist[pos_out * k + t] = ok ? sh[t] + sl[t] : 3.40282347e38f;
+            if (A.out_err)
+                A.out_err[pos_out * k + t] = ok ? se[t] : 0.0f;
+        }
+    }
+}
+
 // ---- driver -----------------------------------------------------------------
 
 __global__ void query_rows_kernel(const int64_t *__restrict__ qsorted, int64_t nq, int64_t n,
@@ -600,6 +794,8 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
 {
     CX_REQUIRE(k >= 1 && k <= kMixMaxK, "mixed-precision kNN supports 1 <= k <= 256");
     CX_REQUIRE(t.d <= kMaxD, "mixed-precision kNN supports d <= 16");
+    CX_REQUIRE(t.n < ((int64_t)1 << 31) - 4 * kSeg,
+               "mixed-precision kNN supports fewer than 2^31 - 8192 points");
     if (nq <= 0)
         return;
     prof_mark(t, 0);
@@ -629,6 +825,11 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     off += 256;
     const size_t o_list = off;
     off += align(sizeof(int32_t) * (size_t)nq * kCap);
+    const size_t o_hard = off;
+    off += align(sizeof(int32_t) * nq);
+    const size_t o_part = off;
+    const int64_t hard_cap = nq < kHardBatch ? nq : kHardBatch;
+    off += align(sizeof(HardEntry) * (size_t)hard_cap * ((n + kHardSeg - 1) / kHardSeg) * k);
     size_t sort_bytes = 0;
     CX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int64_t *)nullptr,
                                               (int64_t *)nullptr, (const int64_t *)nullptr,
@@ -640,6 +841,8 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     int64_t *iota = (int64_t *)(ws + o_iota);
     int32_t *qrow = (int32_t *)(ws + o_row), *cnt = (int32_t *)(ws + o_cnt);
     int32_t *ctr = (int32_t *)(ws + o_ctr), *list = (int32_t *)(ws + o_list);
+    int32_t *hard = (int32_t *)(ws + o_hard);
+    HardEntry *part = (HardEntry *)(ws + o_part);
     float *tau = (float *)(ws + o_tau);
     int32_t *hist = (int32_t *)(ws + o_hist);
 
@@ -651,7 +854,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     prof_mark(t, 2);
     CX_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * nq, st));
     CX_HIP(hipMemsetAsync(hist, 0, sizeof(int32_t) * nq * kBins, st));
-    CX_HIP(hipMemsetAsync(ctr, 0, sizeof(int32_t), st));
+    CX_HIP(hipMemsetAsync(ctr, 0, 256, st));
 
     ScreenArgs sa{};
     sa.vhi = t.vhi.ptr;
@@ -661,7 +864,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     sa.vnorm = t.vnorm.ptr;
     sa.qsorted = qs;
     sa.qrow = qrow;
-    sa.n = n;
+    sa.n = (int32_t)n;
     sa.nq = nq;
     sa.d = t.d;
     sa.mp = t.mp;
@@ -685,12 +888,13 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     sa.list = list;
     sa.n_tiles = (int32_t)((nq + kQT - 1) / kQT);
     const dim3 grid((unsigned)sa.n_tiles, (unsigned)((n + kSeg - 1) / kSeg));
-    screen_kernel<1><<<grid, 256, 0, st>>>(sa);
+    const size_t dyn_lds = sizeof(float) * 2 * (size_t)t.d * kCT;
+    screen_kernel<1><<<grid, kScreenThreads, dyn_lds, st>>>(sa);
     CX_HIP(hipGetLastError());
     prof_mark(t, 3);
     tau_kernel<<<qb, 256, 0, st>>>(hist, nq, k, tau);
     prof_mark(t, 4);
-    screen_kernel<2><<<grid, 256, 0, st>>>(sa);
+    screen_kernel<2><<<grid, kScreenThreads, dyn_lds, st>>>(sa);
     CX_HIP(hipGetLastError());
     prof_mark(t, 5);
 
@@ -721,10 +925,36 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     ra.out_dist = d_dist;
     ra.out_err = d_err;
     ra.counter = ctr;
+    // queries whose list overflowed go to the hard-query kernels
+    hard_collect_kernel<<<qb, 256, 0, st>>>(cnt, nq, hard, ctr + 1);
+    if (!t.host_word)
+        CX_HIP(hipHostMalloc((void **)&t.host_word, 64, hipHostMallocDefault));
+    CX_HIP(hipMemcpyAsync(t.host_word, ctr + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     const int64_t waves = nq < 4096 ? nq : 4096;
     rescore_kernel<<<(unsigned)waves, 64, 0, st>>>(ra);
     CX_HIP(hipGetLastError());
     prof_mark(t, 6);
+    CX_HIP(hipStreamSynchronize(st));
+    const int32_t n_hard = *t.host_word;
+    HardArgs ha{};
+    ha.r = ra;
+    ha.hard = hard;
+    ha.nseg = (int32_t)((n + kHardSeg - 1) / kHardSeg);
+    ha.part = part;
+    for (int32_t h0 = 0; h0 < n_hard; h0 += kHardBatch) {
+        ha.h0 = h0;
+        ha.hn = n_hard - h0 < kHardBatch ? n_hard - h0 : kHardBatch;
+        ha.counter = ctr + 2;
+        if (h0 > 0)
+            CX_HIP(hipMemsetAsync(ha.counter, 0, sizeof(int32_t), st));
+        hard_score_kernel<<<1024, 256, 0, st>>>(ha);
+        CX_HIP(hipGetLastError());
+        const int64_t blocks = ha.hn < 2048 ? ha.hn : 2048;
+        hard_merge_kernel<<<(unsigned)blocks, 256, 0, st>>>(ha);
+        CX_HIP(hipGetLastError());
+    }
+    prof_mark(t, 7);
+    t.last_hard = n_hard;
 }
 
 } // namespace cx
