@@ -39,6 +39,8 @@ TreeState::~TreeState()
     vnorm.release();
     x_ds.release();
     isp_ds.release();
+    pool.release();
+    part.release();
     work.release();
     if (host_word)
         (void)hipHostFree(host_word);

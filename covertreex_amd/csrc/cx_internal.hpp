@@ -114,6 +114,8 @@ struct TreeState {
     int32_t mp = 0;                // m padded to a multiple of 32
     DevBuf<float> vnorm;           // |V_i| rounded up, dataset order
     DevBuf<float> x_ds, isp_ds;    // raw coords and 1/sqrt(p) in dataset order
+    DevBuf<int32_t> pool;          // full candidate lists of overflowed queries
+    DevBuf<uint8_t> part;          // per-slice top-k lists of overflowed queries
 
     // reusable query workspace
     DevBuf<uint8_t> work;

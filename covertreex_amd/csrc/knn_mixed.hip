@@ -7,20 +7,26 @@
 //
 //   prep      V -> dataset-order BF16 hi/lo split (V = hi + lo + r, |r| <=
 //             2^-18 |V|), row norms |V_i| rounded up.
-//   screen    one workgroup per 128-query tile x 2048-candidate segment, four
-//             wavefronts of 64 x 64; V.V on MFMA (v_mfma_f32_32x32x16_bf16)
-//             as hi.hi + (hi.lo + lo.hi) in two FP32 accumulators, kernel
-//             value in FP32, and a certified interval [lb2, ub2] containing
-//             the exact squared distance 1 - |rho|.
+//   screen    one workgroup per 128-query tile x 2048-candidate segment, eight
+//             wavefronts of 32 candidates x 64 queries; V.V on MFMA
+//             (v_mfma_f32_32x32x16_bf16) as hi.hi + (hi.lo + lo.hi) in two
+//             FP32 accumulators, kernel value in FP32, and a certified
+//             interval [lb2, ub2] containing the exact squared distance
+//             1 - |rho|.
 //             pass 1: per-query histogram of ub2 -> tau_q, an upper bound on
 //                     the k-th exact squared distance;
 //             pass 2: every candidate with lb2 <= tau_q is appended to the
 //                     query's list (a superset of the exact k nearest).
-//   rescore   one wavefront per query: compensated FP32 (double-float)
+//   rescore   one workgroup per query: compensated FP32 (double-float)
 //             distance for each listed candidate, exact top-k, ascending
 //             output with a per-result error bound.  A query whose list
-//             overflowed is rescored over all its valid candidates.
+//             overflowed its kCap slots is listed again, in full, into a pool
+//             sized by its pass-2 count, and rescored from there.
 #include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
 
 #include "cx_internal.hpp"
 #include "df.hpp"
@@ -120,7 +126,18 @@ struct ScreenArgs {
     int32_t *cnt;    // nq
     int32_t *list;   // nq x kCap
     int32_t n_tiles;
+    // optional remap (hard queries): tile position -> sorted query position,
+    // and each position's list capacity and offset (minus off_base)
+    const int32_t *qmap;
+    const int32_t *cap;
+    const int64_t *off;
+    int64_t off_base;
 };
+
+__device__ __forceinline__ int64_t screen_qpos(const ScreenArgs &A, int64_t t)
+{
+    return A.qmap ? (int64_t)A.qmap[t] : t;
+}
 
 // Kernel value in FP32 from the hardware exp2/sqrt (about 1 ulp each); its
 // error is far below eK = var * 2^-18.
@@ -163,7 +180,7 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
     if (A.pred) {
         // candidate range of this tile: [0, largest query index) under the mask
         const int64_t last = s0 + kQT - 1 < A.nq - 1 ? s0 + kQT - 1 : A.nq - 1;
-        const int64_t qmax = A.qsorted[last];
+        const int64_t qmax = A.qsorted[screen_qpos(A, last)];
         c_end = qmax < 0 ? n : (qmax < n ? (int32_t)qmax : n);
     }
     const int32_t c_begin = (int32_t)blockIdx.y * (int32_t)kSeg;
@@ -174,7 +191,7 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
 
     for (int i = tid; i < kQT; i += kScreenThreads) {
         const int64_t s = s0 + i;
-        const int32_t row = s < A.nq ? A.qrow[s] : 0;
+        const int32_t row = s < A.nq ? A.qrow[screen_qpos(A, s)] : 0;
         sQrow[i] = row;
         for (int c = 0; c < d; c++)
             sQx[c * kQT + i] = A.x[(int64_t)row * d + c];
@@ -192,15 +209,16 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
         qcol[b] = 64 * qh + 32 * b + (lane & 31);
         const int64_t s = s0 + qcol[b];
         const bool live = s < A.nq;
-        const int64_t qid = live ? A.qsorted[s] : 0;
-        const int32_t row = live ? A.qrow[s] : 0;
+        const int64_t sp = live ? screen_qpos(A, s) : 0;
+        const int64_t qid = live ? A.qsorted[sp] : 0;
+        const int32_t row = live ? A.qrow[sp] : 0;
         qlim[b] = !live ? 0 : (A.pred ? (qid < 0 ? n : (qid < n ? (int32_t)qid : n)) : n);
         qself[b] = live ? row : -1;
         qw[b] = A.isp[row];
         qcn[b] = A.cD * A.vnorm[row];
         qthr[b] = 2.0f;
         if (PASS == 2 && live) {
-            const float t = A.tau[s];
+            const float t = A.tau[sp];
             // |rho| >= 1 - tau is implied by d^2 <= tau; 2^-22 covers rounding
             qthr[b] = t >= 2.0f ? -1.0f : (1.0f - t) - 2.3841858e-7f;
         }
@@ -326,8 +344,12 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
                         if (valid && (ar + erho >= qthr[b] || self)) {
                             const int64_t s = s0 + qcol[b];
                             const int32_t pos = atomicAdd(&A.cnt[s], 1);
-                            if (pos < kCap)
+                            if (A.qmap) {
+                                if (pos < A.cap[s])
+                                    A.list[A.off[s] - A.off_base + pos] = j;
+                            } else if (pos < kCap) {
                                 A.list[s * kCap + pos] = j;
+                            }
                         }
                     }
                 }
@@ -371,6 +393,15 @@ __global__ void tau_kernel(const int32_t *__restrict__ hist, int64_t nq, int32_t
 
 // ---- rescore -----------------------------------------------------------------
 
+constexpr int kRsThreads = 256;                 // one workgroup per query
+constexpr int kRsSort = 2048;                   // running top-k + one round, padded
+constexpr int kRsChunk = kRsSort - kMixMaxK;    // listed candidates per sort round
+
+struct HardEntry {
+    float dh, dl, err;
+    int32_t j;
+};
+
 struct RescoreArgs {
     const float *v;       // n x m tree order
     const float *raw;     // n x d tree order
@@ -379,22 +410,34 @@ struct RescoreArgs {
     const int64_t *qsorted;
     const int64_t *qperm; // sorted position -> original query position
     const int32_t *qrow;
-    const int32_t *cnt;
-    const int32_t *list;
+    const int32_t *cnt;   // listed candidates per sorted query (or per hard query)
+    const int32_t *list;  // kCap slots per sorted query (or the hard pool)
+    // hard queries (pool != 0): hard[h0 + i] is the sorted position of item i,
+    // its list is list[off[h0 + i] - off_base ..] with cap[h0 + i] entries
+    int32_t h0, hn;
+    const int32_t *hard, *cap;
+    const int64_t *off;
+    int64_t off_base;
+    // slices of kRsChunk: hard query h owns global slices [chunk_off[h],
+    // chunk_off[h + 1]); this batch holds n_chunks of them from chunk_base
+    const int64_t *chunk_off;
+    int64_t chunk_base, n_chunks;
+    HardEntry *part;      // n_chunks x k slice top-k lists
+    int32_t *mismatch;    // counts pool lists whose size differs from the pass-2 count
     int64_t n, nq;
     int32_t d, m, kernel, pred, k;
     float var;
     float inv_ls_hi[kMaxD], inv_ls_lo[kMaxD];
     int64_t *out_idx;
     float *out_dist, *out_err;
-    int32_t *counter;
+    int64_t *counter;
 };
 
-// Exact-arithmetic surrogate of the residual distance between dataset rows q
-// and j: every step in double-float.  Returns the distance and a bound on its
-// error.
-__device__ __forceinline__ df accurate_distance(const RescoreArgs &A, int64_t qt, int64_t jt,
-                                                float *err)
+// The residual distance from a compensated dot V_q.V_j (double-float) and
+// sum|V_q[c] V_j[c]|: kernel value, numerator, division and square root in
+// double-float.  Returns the distance and a bound on its error.
+__device__ __forceinline__ df finish_distance(const RescoreArgs &A, int64_t qt, int64_t jt, df dot,
+                                              float sabs, float *err)
 {
     const int32_t d = A.d, m = A.m;
     const float *xq = A.raw + qt * d, *xj = A.raw + jt * d;
@@ -414,13 +457,6 @@ __device__ __forceinline__ df accurate_distance(const RescoreArgs &A, int64_t qt
         kv = df_mul(poly, df_exp(df_neg(a)));
     }
     kv = df_mul_f(kv, A.var);
-    const float *vq = A.v + qt * m, *vj = A.v + jt * m;
-    df dot = {0.0f, 0.0f};
-    float sabs = 0.0f;
-    for (int c = 0; c < m; c++) {
-        dot = df_acc(dot, two_prod(vq[c], vj[c]));
-        sabs += __builtin_fabsf(vq[c] * vj[c]);
-    }
     df num = df_sub(kv, dot);
     df pp = two_prod(A.p[qt], A.p[jt]);
     if (pp.hi < 1e-18f)
@@ -447,168 +483,20 @@ __device__ __forceinline__ df accurate_distance(const RescoreArgs &A, int64_t qt
     return dist;
 }
 
-__global__ void __launch_bounds__(64) rescore_kernel(RescoreArgs A)
-{
-    __shared__ float s_dh[kMixMaxK], s_dl[kMixMaxK], s_er[kMixMaxK];
-    __shared__ int32_t s_j[kMixMaxK];
-    __shared__ int64_t s_qi;
-    const int lane = threadIdx.x;
-    const int32_t k = A.k;
-    for (;;) {
-        if (lane == 0)
-            s_qi = atomicAdd(A.counter, 1);
-        __syncthreads();
-        const int64_t s = s_qi;
-        __syncthreads();
-        if (s >= A.nq)
-            break;
-        const int64_t pos_out = A.qperm[s];
-        const int32_t qrow = A.qrow[s];
-        const int64_t qt = A.inv_order[qrow];
-        const int32_t c = A.cnt[s];
-        if (c > kCap)
-            continue; // overflowed list: scored by the hard-query kernels
-        for (int j = lane; j < k; j += 64) {
-            A.out_idx[pos_out * k + j] = -1;
-            A.out_dist[pos_out * k + j] = 3.40282347e38f;
-            if (A.out_err)
-                A.out_err[pos_out * k + j] = 0.0f;
-        }
-        const int64_t total = c;
-        int count = 0;
-        float maxh = -1.0f, maxl = 0.0f;
-        int32_t maxj = -1;
-        int maxpos = 0;
-        for (int64_t base = 0; base < total; base += 64) {
-            const int64_t e = base + lane;
-            const bool live = e < total;
-            int64_t j = 0;
-            if (live)
-                j = A.list[s * kCap + e];
-            df dist = {0.0f, 0.0f};
-            float err = 0.0f;
-            if (live) {
-                if (j == qrow)
-                    dist = {0.0f, 0.0f};
-                else
-                    dist = accurate_distance(A, qt, A.inv_order[j], &err);
-            }
-            bool better = live && (count < k || df_lt(dist, df{maxh, maxl}) ||
-                                   (dist.hi == maxh && dist.lo == maxl && j < maxj));
-            uint64_t mask = __ballot(better);
-            while (mask) {
-                const int l = __ffsll((long long)mask) - 1;
-                mask &= mask - 1;
-                const float dh = __shfl(dist.hi, l, 64);
-                const float dl = __shfl(dist.lo, l, 64);
-                const float er = __shfl(err, l, 64);
-                const int32_t jj = (int32_t)__shfl((int)j, l, 64);
-                int slot;
-                if (count < k) {
-                    slot = count++;
-                } else {
-                    const bool lt = dh < maxh || (dh == maxh && (dl < maxl ||
-                                                  (dl == maxl && jj < maxj)));
-                    if (!lt)
-                        continue;
-                    slot = maxpos;
-                }
-                if (lane == 0) {
-                    s_dh[slot] = dh;
-                    s_dl[slot] = dl;
-                    s_er[slot] = er;
-                    s_j[slot] = jj;
-                }
-                __syncthreads();
-                // refresh the maximum (largest distance, then largest index)
-                float bh = -1.0f, bl = 0.0f;
-                int32_t bj = -1;
-                int bp = 0;
-                for (int t = lane; t < count; t += 64) {
-                    const float h = s_dh[t], lo = s_dl[t];
-                    const int32_t jv = s_j[t];
-                    if (h > bh || (h == bh && (lo > bl || (lo == bl && jv > bj)))) {
-                        bh = h;
-                        bl = lo;
-                        bj = jv;
-                        bp = t;
-                    }
-                }
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) {
-                    const float oh = __shfl_xor(bh, o, 64), ol = __shfl_xor(bl, o, 64);
-                    const int32_t oj = __shfl_xor(bj, o, 64);
-                    const int op = __shfl_xor(bp, o, 64);
-                    if (oh > bh || (oh == bh && (ol > bl || (ol == bl && oj > bj)))) {
-                        bh = oh;
-                        bl = ol;
-                        bj = oj;
-                        bp = op;
-                    }
-                }
-                maxh = bh;
-                maxl = bl;
-                maxj = bj;
-                maxpos = bp;
-            }
-        }
-        __syncthreads();
-        // ascending output: (distance, dataset index)
-        for (int t = lane; t < count; t += 64) {
-            const float h = s_dh[t], lo = s_dl[t];
-            const int32_t jv = s_j[t];
-            int rank = 0;
-            for (int u = 0; u < count; u++) {
-                const float uh = s_dh[u], ul = s_dl[u];
-                rank += uh < h || (uh == h && (ul < lo || (ul == lo && s_j[u] < jv)));
-            }
-            A.out_idx[pos_out * k + rank] = jv;
-            A.out_dist[pos_out * k + rank] = h + lo;
-            if (A.out_err)
-                A.out_err[pos_out * k + rank] = s_er[t];
-        }
-        __syncthreads();
-    }
-}
-
-// ---- hard queries ------------------------------------------------------------
-// A query whose list overflowed (its screen interval was too wide, e.g. tiny
-// p) is scored over all its valid candidates: one workgroup per (query,
-// 4096-candidate segment) keeps a running top-k of compensated distances in
-// LDS; a second kernel merges the per-segment lists.
-
-constexpr int kHardChunk = 1024;             // candidates per sort round
-constexpr int kHardSeg = 4 * kHardChunk;     // candidates per workgroup
-constexpr int kHardSort = 2048;              // running top-k + one chunk, padded
-constexpr int kHardBatch = 2048;             // hard queries per launch pair
-
-struct HardEntry {
-    float dh, dl, err;
-    int32_t j;
-};
-
-struct HardArgs {
-    RescoreArgs r;
-    const int32_t *hard; // sorted positions of the hard queries
-    int32_t h0, hn;      // this batch: hard[h0 .. h0 + hn)
-    int32_t nseg;        // segments per query
-    HardEntry *part;     // hn x nseg x k
-    int32_t *counter;
-};
-
 __device__ __forceinline__ bool entry_lt(float ah, float al, int32_t aj, float bh, float bl,
                                          int32_t bj)
 {
     return ah < bh || (ah == bh && (al < bl || (al == bl && aj < bj)));
 }
 
-// ascending bitonic sort of kHardSort (distance hi, lo, index) entries in LDS
-__device__ void sort_entries(float *h, float *l, float *e, int32_t *jv)
+// ascending bitonic sort of the first `size` (a power of two) entries
+// (distance hi, lo, index; error rides along) in LDS
+__device__ void sort_entries(float *h, float *l, float *e, int32_t *jv, int size_total)
 {
-    for (int size = 2; size <= kHardSort; size <<= 1)
+    for (int size = 2; size <= size_total; size <<= 1)
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
             __syncthreads();
-            for (int t = threadIdx.x; t < kHardSort / 2; t += blockDim.x) {
+            for (int t = threadIdx.x; t < size_total / 2; t += blockDim.x) {
                 const int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
                 const bool up = (lo & size) == 0;
                 const bool sw = up ? entry_lt(h[hi], l[hi], jv[hi], h[lo], l[lo], jv[lo])
@@ -624,122 +512,260 @@ __device__ void sort_entries(float *h, float *l, float *e, int32_t *jv)
     __syncthreads();
 }
 
-__device__ __forceinline__ void clear_entries(float *h, float *l, float *e, int32_t *jv, int from)
+// Compensated dot of V rows q and j over this lane's columns (Dot2 within a
+// lane: TwoProduct + TwoSum per term), eight lanes per row pair.  Rows are
+// float4-aligned when m % 4 == 0; the lanes of a group then read 128
+// contiguous bytes per load.
+__device__ __forceinline__ void dot2_step(float a, float b, float &s, float &c, float &sabs)
 {
-    for (int t = from + (int)threadIdx.x; t < kHardSort; t += blockDim.x) {
-        h[t] = __builtin_huge_valf();
-        l[t] = 0.0f;
-        e[t] = 0.0f;
-        jv[t] = 0x7fffffff;
+    const float ph = a * b;
+    const float pl = __builtin_fmaf(a, b, -ph);
+    const float t = s + ph;
+    const float bb = t - s;
+    c += ((s - (t - bb)) + (ph - bb)) + pl;
+    s = t;
+    sabs += __builtin_fabsf(ph);
+}
+
+__device__ __forceinline__ void group_dot(const float *__restrict__ vq,
+                                          const float *__restrict__ vj, int32_t m, int sub,
+                                          df &dot, float &sabs)
+{
+    float s = 0.0f, c = 0.0f;
+    sabs = 0.0f;
+    if ((m & 3) == 0) {
+#pragma unroll 2
+        for (int32_t i = 4 * sub; i < m; i += 32) {
+            const float4 a = *(const float4 *)(vq + i);
+            const float4 b = *(const float4 *)(vj + i);
+            dot2_step(a.x, b.x, s, c, sabs);
+            dot2_step(a.y, b.y, s, c, sabs);
+            dot2_step(a.z, b.z, s, c, sabs);
+            dot2_step(a.w, b.w, s, c, sabs);
+        }
+    } else {
+#pragma unroll 4
+        for (int32_t i = sub; i < m; i += 8)
+            dot2_step(vq[i], vj[i], s, c, sabs);
+    }
+    dot = two_sum(s, c);
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        const df other = {__shfl_xor(dot.hi, o, 64), __shfl_xor(dot.lo, o, 64)};
+        dot = df_add(dot, other);
+        sabs += __shfl_xor(sabs, o, 64);
     }
 }
 
-__device__ __forceinline__ int64_t vecchia_limit(const RescoreArgs &A, int64_t qid)
+// Score one list slice into LDS slots [k, k + n_in) and merge it into the
+// running top-k at [0, k): eight lanes per candidate for the dot (results
+// parked in the slots), then one lane per candidate for the distance.
+__device__ void score_slice(const RescoreArgs &A, const int32_t *__restrict__ lst, int32_t n_in,
+                            int32_t qrow, int64_t qt, float *sh, float *sl, float *se,
+                            int32_t *sj)
 {
-    // Vecchia bound on the original index (negative -> unbounded)
-    return A.pred ? (qid < 0 ? A.n : (qid < A.n ? qid : A.n)) : A.n;
+    const int tid = threadIdx.x, grp = tid >> 3, sub = tid & 7;
+    const int32_t k = A.k, m = A.m;
+    const float *vq = A.v + qt * m;
+    __syncthreads();
+    for (int32_t e = grp; e < n_in; e += kRsThreads / 8) {
+        const int32_t j = lst[e];
+        df dot;
+        float sabs;
+        group_dot(vq, A.v + A.inv_order[j] * m, m, sub, dot, sabs);
+        if (sub == 0) {
+            sh[k + e] = dot.hi;
+            sl[k + e] = dot.lo;
+            se[k + e] = sabs;
+            sj[k + e] = j;
+        }
+    }
+    __syncthreads();
+    for (int32_t e = tid; e < n_in; e += kRsThreads) {
+        const int32_t j = sj[k + e];
+        float dh = 0.0f, dl = 0.0f, er = 0.0f;
+        if (j != qrow) {
+            const df dd = finish_distance(A, qt, A.inv_order[j], df{sh[k + e], sl[k + e]},
+                                          se[k + e], &er);
+            dh = dd.hi;
+            dl = dd.lo;
+        }
+        sh[k + e] = dh;
+        sl[k + e] = dl;
+        se[k + e] = er;
+    }
+    int p2 = 2;
+    while (p2 < k + n_in)
+        p2 <<= 1;
+    // entries past k left by an earlier round rank behind the current top-k,
+    // so they never re-enter it
+    sort_entries(sh, sl, se, sj, p2);
 }
 
-__global__ void hard_collect_kernel(const int32_t *__restrict__ cnt, int64_t nq, int32_t *hard,
-                                    int32_t *n_hard)
+__device__ __forceinline__ void clear_entries(float *sh, float *sl, float *se, int32_t *sj)
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nq && cnt[s] > kCap)
-        hard[atomicAdd(n_hard, 1)] = (int32_t)s;
+    for (int t = threadIdx.x; t < kRsSort; t += kRsThreads) {
+        sh[t] = __builtin_huge_valf();
+        sl[t] = 0.0f;
+        se[t] = 0.0f;
+        sj[t] = 0x7fffffff;
+    }
 }
 
-__global__ void __launch_bounds__(256) hard_score_kernel(HardArgs H)
+__device__ __forceinline__ void write_row(const RescoreArgs &A, int64_t pos_out, const float *sh,
+                                          const float *sl, const float *se, const int32_t *sj)
 {
-    __shared__ float sh[kHardSort], sl[kHardSort], se[kHardSort];
-    __shared__ int32_t sj[kHardSort];
-    __shared__ int32_t s_item;
-    const RescoreArgs &A = H.r;
     const int32_t k = A.k;
+    for (int t = threadIdx.x; t < k; t += kRsThreads) {
+        const bool ok = sh[t] < __builtin_huge_valf();
+        A.out_idx[pos_out * k + t] = ok ? (int64_t)sj[t] : -1;
+        A.out_dist[pos_out * k + t] = ok ? sh[t] + sl[t] : 3.40282347e38f;
+        if (A.out_err)
+            A.out_err[pos_out * k + t] = ok ? se[t] : 0.0f;
+    }
+}
+
+// MODE 0: one workgroup per query, its kCap-slot list scored in rounds of
+// kRsChunk, output row written.  MODE 1: one workgroup per kRsChunk slice of a
+// hard query's pool list, the slice's top-k written to A.part.
+template <int MODE> __global__ void __launch_bounds__(kRsThreads) rescore_kernel(RescoreArgs A)
+{
+    __shared__ float sh[kRsSort], sl[kRsSort], se[kRsSort];
+    __shared__ int32_t sj[kRsSort];
+    __shared__ int64_t s_item;
+    const int32_t k = A.k;
+    const int64_t n_items = MODE ? A.n_chunks : A.nq;
     for (;;) {
         if (threadIdx.x == 0)
-            s_item = atomicAdd(H.counter, 1);
+            s_item = (int64_t)atomicAdd((unsigned long long *)A.counter, 1ull);
         __syncthreads();
-        const int32_t item = s_item;
+        const int64_t item = s_item;
         __syncthreads();
-        if (item >= H.hn * H.nseg)
+        if (item >= n_items)
             break;
-        const int32_t hl = item / H.nseg, seg = item % H.nseg;
-        const int64_t s = H.hard[H.h0 + hl];
-        const int32_t qrow = A.qrow[s];
-        const int64_t qt = A.inv_order[qrow];
-        const int64_t limit = vecchia_limit(A, A.qsorted[s]);
-        const int64_t beg = (int64_t)seg * kHardSeg;
-        if (beg >= limit)
-            continue; // the merge reads only segments below the limit
-        const int64_t end = beg + kHardSeg < limit ? beg + kHardSeg : limit;
-        clear_entries(sh, sl, se, sj, 0);
-        for (int64_t c0 = beg; c0 < end; c0 += kHardChunk) {
-            __syncthreads();
-            // the running top-k sits in [0, k); this chunk fills [k, k + chunk)
-            for (int t = threadIdx.x; t < kHardChunk; t += blockDim.x) {
-                const int64_t j = c0 + t;
-                float dh = __builtin_huge_valf(), dl = 0.0f, er = 0.0f;
-                int32_t jj = 0x7fffffff;
-                if (j < end) {
-                    jj = (int32_t)j;
-                    if (j == qrow) {
-                        dh = 0.0f;
-                    } else {
-                        const df dd = accurate_distance(A, qt, A.inv_order[j], &er);
-                        dh = dd.hi;
-                        dl = dd.lo;
-                    }
-                }
-                sh[k + t] = dh;
-                sl[k + t] = dl;
-                se[k + t] = er;
-                sj[k + t] = jj;
+        if (MODE == 0) {
+            const int64_t s = item;
+            const int32_t count = A.cnt[s];
+            if (count > kCap)
+                continue; // listed again, in full, in the hard pool
+            const int32_t qrow = A.qrow[s];
+            const int64_t qt = A.inv_order[qrow];
+            clear_entries(sh, sl, se, sj);
+            for (int32_t c0 = 0; c0 < count; c0 += kRsChunk) {
+                const int32_t n_in = count - c0 < kRsChunk ? count - c0 : kRsChunk;
+                score_slice(A, A.list + s * kCap + c0, n_in, qrow, qt, sh, sl, se, sj);
             }
-            sort_entries(sh, sl, se, sj);
+            write_row(A, A.qperm[s], sh, sl, se, sj);
+        } else {
+            // chunk -> hard query: last h in [h0, h0 + hn) with chunk_off[h] <= item
+            const int64_t g = A.chunk_base + item;
+            int32_t lo = A.h0, hi = A.h0 + A.hn - 1;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi + 1) >> 1;
+                if (A.chunk_off[mid] <= g)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int32_t h = lo;
+            const int64_t s = A.hard[h];
+            const int32_t got = A.cnt[h - A.h0], want = A.cap[h];
+            const int32_t total = got < want ? got : want;
+            const int64_t c0 = (g - A.chunk_off[h]) * kRsChunk;
+            if (c0 == 0 && got != want && threadIdx.x == 0)
+                atomicAdd(A.mismatch, 1);
+            const int32_t n_in =
+                c0 >= total ? 0 : (total - c0 < kRsChunk ? (int32_t)(total - c0) : kRsChunk);
+            const int32_t qrow = A.qrow[s];
+            const int64_t qt = A.inv_order[qrow];
+            clear_entries(sh, sl, se, sj);
+            if (n_in > 0)
+                score_slice(A, A.list + (A.off[h] - A.off_base) + c0, n_in, qrow, qt, sh, sl,
+                            se, sj);
+            HardEntry *out = A.part + item * k;
+            for (int t = threadIdx.x; t < k; t += kRsThreads)
+                out[t] = HardEntry{sh[t], sl[t], se[t], sj[t]};
         }
-        HardEntry *out = H.part + ((int64_t)hl * H.nseg + seg) * k;
-        for (int t = threadIdx.x; t < k; t += blockDim.x)
-            out[t] = HardEntry{sh[t], sl[t], se[t], sj[t]};
         __syncthreads();
     }
 }
 
-__global__ void __launch_bounds__(256) hard_merge_kernel(HardArgs H)
+// Merge the per-slice top-k lists of each hard query and write its row.
+__global__ void __launch_bounds__(kRsThreads) hard_merge_kernel(RescoreArgs A)
 {
-    __shared__ float sh[kHardSort], sl[kHardSort], se[kHardSort];
-    __shared__ int32_t sj[kHardSort];
-    const RescoreArgs &A = H.r;
+    __shared__ float sh[kRsSort], sl[kRsSort], se[kRsSort];
+    __shared__ int32_t sj[kRsSort];
     const int32_t k = A.k;
-    for (int32_t hl = blockIdx.x; hl < H.hn; hl += gridDim.x) {
-        const int64_t s = H.hard[H.h0 + hl];
-        const int64_t limit = vecchia_limit(A, A.qsorted[s]);
-        const int32_t nseg = (int32_t)((limit + kHardSeg - 1) / kHardSeg);
-        const int32_t per = (kHardSort - k) / k; // lists merged per round
+    const int32_t per = (kRsSort - k) / k; // lists merged per round
+    for (int32_t i = blockIdx.x; i < A.hn; i += gridDim.x) {
+        const int32_t h = A.h0 + i;
+        const int64_t first = A.chunk_off[h] - A.chunk_base;
+        const int32_t nl_all =
+            (int32_t)((h + 1 < A.h0 + A.hn ? A.chunk_off[h + 1] : A.chunk_base + A.n_chunks) -
+                      A.chunk_off[h]);
         __syncthreads();
-        clear_entries(sh, sl, se, sj, 0);
-        for (int32_t g = 0; g < nseg; g += per) {
-            const int32_t nl = nseg - g < per ? nseg - g : per;
+        clear_entries(sh, sl, se, sj);
+        for (int32_t g = 0; g < nl_all; g += per) {
+            const int32_t nl = nl_all - g < per ? nl_all - g : per;
             __syncthreads();
-            // entries left past k + nl*k by an earlier round rank behind the
-            // current top-k, so they never re-enter it
-            for (int t = threadIdx.x; t < nl * k; t += blockDim.x) {
-                const HardEntry e = H.part[((int64_t)hl * H.nseg + g + t / k) * k + t % k];
+            for (int t = threadIdx.x; t < nl * k; t += kRsThreads) {
+                const HardEntry e = A.part[(first + g + t / k) * k + t % k];
                 sh[k + t] = e.dh;
                 sl[k + t] = e.dl;
                 se[k + t] = e.err;
                 sj[k + t] = e.j;
             }
-            sort_entries(sh, sl, se, sj);
+            int p2 = 2;
+            while (p2 < k + nl * k)
+                p2 <<= 1;
+            sort_entries(sh, sl, se, sj, p2);
         }
-        const int64_t pos_out = A.qperm[s];
-        for (int t = threadIdx.x; t < k; t += blockDim.x) {
-            const bool ok = sh[t] < __builtin_huge_valf();
-            A.out_idx[pos_out * k + t] = ok ? (int64_t)sj[t] : -1;
-            A.out_dist[pos_out * k + t] = ok ? sh[t] + sl[t] : 3.40282347e38f;
-            if (A.out_err)
-                A.out_err[pos_out * k + t] = ok ? se[t] : 0.0f;
-        }
+        write_row(A, A.qperm[A.hard[h]], sh, sl, se, sj);
     }
+}
+
+// ---- hard queries ------------------------------------------------------------
+// A query whose list overflowed kCap (a wide screen interval, e.g. tiny p) is
+// listed again in full: its pass-2 count sizes a slice of a pool, the same
+// screen_kernel<2> re-runs over tiles of the hard queries only and writes
+// there, and rescore_kernel scores the slices.
+
+__global__ void hard_flag_kernel(const int32_t *__restrict__ cnt, int64_t nq, int32_t *flag)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nq)
+        flag[s] = cnt[s] > kCap ? 1 : 0;
+}
+
+// ordered compaction: hard[pos[s]] = s, its pass-2 count and kRsChunk
+// slices; the number of hard queries goes to n_hard
+__global__ void hard_scatter_kernel(const int32_t *__restrict__ cnt,
+                                    const int32_t *__restrict__ flag,
+                                    const int32_t *__restrict__ pos, int64_t nq, int32_t *hard,
+                                    int32_t *hard_cnt, int64_t *hard_cnt64, int64_t *hard_chunks,
+                                    int32_t *n_hard)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nq)
+        return;
+    if (flag[s]) {
+        hard[pos[s]] = (int32_t)s;
+        hard_cnt[pos[s]] = cnt[s];
+        hard_cnt64[pos[s]] = cnt[s];
+        hard_chunks[pos[s]] = (cnt[s] + kRsChunk - 1) / kRsChunk;
+    }
+    if (s == nq - 1)
+        *n_hard = pos[s] + flag[s];
+}
+
+// totals[0] = pool entries, totals[1] = slices over all hard queries
+__global__ void hard_totals_kernel(const int64_t *__restrict__ off,
+                                   const int64_t *__restrict__ cnt64,
+                                   const int64_t *__restrict__ choff,
+                                   const int64_t *__restrict__ chunks, int32_t n, int64_t *totals)
+{
+    totals[0] = off[n - 1] + cnt64[n - 1];
+    totals[1] = choff[n - 1] + chunks[n - 1];
 }
 
 // ---- driver -----------------------------------------------------------------
@@ -789,6 +815,19 @@ static void prepare_mixed(TreeState &t)
     t.mixed_ready = true;
 }
 
+// read a few device words through the tree's pinned host buffer (synchronises)
+template <typename T> static T read_word(TreeState &t, const T *dptr)
+{
+    static_assert(sizeof(T) <= 8, "one word");
+    if (!t.host_word)
+        CX_HIP(hipHostMalloc((void **)&t.host_word, 64, hipHostMallocDefault));
+    CX_HIP(hipMemcpyAsync(t.host_word, dptr, sizeof(T), hipMemcpyDeviceToHost, t.stream));
+    CX_HIP(hipStreamSynchronize(t.stream));
+    T v;
+    std::memcpy(&v, t.host_word, sizeof(T));
+    return v;
+}
+
 void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, int32_t predecessor,
                int64_t *d_idx, float *d_dist, float *d_err)
 {
@@ -804,51 +843,65 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     hipStream_t st = t.stream;
     const int64_t n = t.n;
 
-    // workspace: sorted queries, permutation, rows, tau, counts, lists
+    // workspace: sorted queries, permutation, rows, tau, counts, lists, and
+    // the hard-query bookkeeping
     auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t off = 0;
-    const size_t o_qs = off;
-    off += align(sizeof(int64_t) * nq);
-    const size_t o_perm = off;
-    off += align(sizeof(int64_t) * nq);
-    const size_t o_iota = off;
-    off += align(sizeof(int64_t) * nq);
-    const size_t o_row = off;
-    off += align(sizeof(int32_t) * nq);
-    const size_t o_tau = off;
-    off += align(sizeof(float) * nq);
-    const size_t o_cnt = off;
-    off += align(sizeof(int32_t) * nq);
-    const size_t o_hist = off;
-    off += align(sizeof(int32_t) * nq * kBins);
-    const size_t o_ctr = off;
-    off += 256;
-    const size_t o_list = off;
-    off += align(sizeof(int32_t) * (size_t)nq * kCap);
-    const size_t o_hard = off;
-    off += align(sizeof(int32_t) * nq);
-    const size_t o_part = off;
-    const int64_t hard_cap = nq < kHardBatch ? nq : kHardBatch;
-    off += align(sizeof(HardEntry) * (size_t)hard_cap * ((n + kHardSeg - 1) / kHardSeg) * k);
-    size_t sort_bytes = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += align(bytes);
+        return o;
+    };
+    const size_t o_qs = take(sizeof(int64_t) * nq);
+    const size_t o_perm = take(sizeof(int64_t) * nq);
+    const size_t o_iota = take(sizeof(int64_t) * nq);
+    const size_t o_row = take(sizeof(int32_t) * nq);
+    const size_t o_tau = take(sizeof(float) * nq);
+    const size_t o_cnt = take(sizeof(int32_t) * nq);
+    const size_t o_hist = take(sizeof(int32_t) * nq * kBins);
+    const size_t o_ctr = take(256);
+    const size_t o_list = take(sizeof(int32_t) * (size_t)nq * kCap);
+    const size_t o_flag = take(sizeof(int32_t) * nq);
+    const size_t o_pos = take(sizeof(int32_t) * nq);
+    const size_t o_hard = take(sizeof(int32_t) * nq);
+    const size_t o_hcnt = take(sizeof(int32_t) * nq);
+    const size_t o_hcnt64 = take(sizeof(int64_t) * nq);
+    const size_t o_hoff = take(sizeof(int64_t) * nq);
+    const size_t o_cnt3 = take(sizeof(int32_t) * nq);
+    const size_t o_hch = take(sizeof(int64_t) * nq);
+    const size_t o_choff = take(sizeof(int64_t) * nq);
+    size_t sort_bytes = 0, scan32_bytes = 0, scan64_bytes = 0;
     CX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int64_t *)nullptr,
                                               (int64_t *)nullptr, (const int64_t *)nullptr,
                                               (int64_t *)nullptr, (int)nq, 0, 64, st));
-    const size_t o_sort = off;
-    off += align(sort_bytes);
+    CX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan32_bytes, (const int32_t *)nullptr,
+                                            (int32_t *)nullptr, (int)nq, st));
+    CX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan64_bytes, (const int64_t *)nullptr,
+                                            (int64_t *)nullptr, (int)nq, st));
+    const size_t tmp_bytes = std::max(sort_bytes, std::max(scan32_bytes, scan64_bytes));
+    const size_t o_tmp = take(tmp_bytes);
     uint8_t *ws = workspace(t, off);
     int64_t *qs = (int64_t *)(ws + o_qs), *perm = (int64_t *)(ws + o_perm);
     int64_t *iota = (int64_t *)(ws + o_iota);
     int32_t *qrow = (int32_t *)(ws + o_row), *cnt = (int32_t *)(ws + o_cnt);
     int32_t *ctr = (int32_t *)(ws + o_ctr), *list = (int32_t *)(ws + o_list);
-    int32_t *hard = (int32_t *)(ws + o_hard);
-    HardEntry *part = (HardEntry *)(ws + o_part);
     float *tau = (float *)(ws + o_tau);
     int32_t *hist = (int32_t *)(ws + o_hist);
+    int32_t *flag = (int32_t *)(ws + o_flag), *pos = (int32_t *)(ws + o_pos);
+    int32_t *hard = (int32_t *)(ws + o_hard), *hcnt = (int32_t *)(ws + o_hcnt);
+    int64_t *hcnt64 = (int64_t *)(ws + o_hcnt64), *hoff = (int64_t *)(ws + o_hoff);
+    int32_t *cnt3 = (int32_t *)(ws + o_cnt3);
+    int64_t *hch = (int64_t *)(ws + o_hch), *choff = (int64_t *)(ws + o_choff);
+    // counters: ctr[0..1] rescore items (int64), ctr[2..3] hard slice items
+    // (int64), ctr[4] hard count, ctr[5] pool-size mismatches, ctr[8..11]
+    // pool entries and slices over all hard queries (int64)
+    int64_t *rs_items = (int64_t *)ctr, *hs_items = (int64_t *)(ctr + 2);
+    int32_t *n_hard_d = ctr + 4;
+    int64_t *totals = (int64_t *)(ctr + 8);
 
     const unsigned qb = (unsigned)((nq + 255) / 256);
     iota64_kernel<<<qb, 256, 0, st>>>(iota, nq);
-    CX_HIP(hipcub::DeviceRadixSort::SortPairs(ws + o_sort, sort_bytes, d_queries, qs, iota, perm,
+    CX_HIP(hipcub::DeviceRadixSort::SortPairs(ws + o_tmp, sort_bytes, d_queries, qs, iota, perm,
                                               (int)nq, 0, 64, st));
     query_rows_kernel<<<qb, 256, 0, st>>>(qs, nq, n, t.order.ptr, qrow);
     prof_mark(t, 2);
@@ -887,14 +940,14 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     sa.cnt = cnt;
     sa.list = list;
     sa.n_tiles = (int32_t)((nq + kQT - 1) / kQT);
-    const dim3 grid((unsigned)sa.n_tiles, (unsigned)((n + kSeg - 1) / kSeg));
+    const unsigned n_seg = (unsigned)((n + kSeg - 1) / kSeg);
     const size_t dyn_lds = sizeof(float) * 2 * (size_t)t.d * kCT;
-    screen_kernel<1><<<grid, kScreenThreads, dyn_lds, st>>>(sa);
+    screen_kernel<1><<<dim3((unsigned)sa.n_tiles, n_seg), kScreenThreads, dyn_lds, st>>>(sa);
     CX_HIP(hipGetLastError());
     prof_mark(t, 3);
     tau_kernel<<<qb, 256, 0, st>>>(hist, nq, k, tau);
     prof_mark(t, 4);
-    screen_kernel<2><<<grid, kScreenThreads, dyn_lds, st>>>(sa);
+    screen_kernel<2><<<dim3((unsigned)sa.n_tiles, n_seg), kScreenThreads, dyn_lds, st>>>(sa);
     CX_HIP(hipGetLastError());
     prof_mark(t, 5);
 
@@ -924,37 +977,108 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     ra.out_idx = d_idx;
     ra.out_dist = d_dist;
     ra.out_err = d_err;
-    ra.counter = ctr;
-    // queries whose list overflowed go to the hard-query kernels
-    hard_collect_kernel<<<qb, 256, 0, st>>>(cnt, nq, hard, ctr + 1);
-    if (!t.host_word)
-        CX_HIP(hipHostMalloc((void **)&t.host_word, 64, hipHostMallocDefault));
-    CX_HIP(hipMemcpyAsync(t.host_word, ctr + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    const int64_t waves = nq < 4096 ? nq : 4096;
-    rescore_kernel<<<(unsigned)waves, 64, 0, st>>>(ra);
+    ra.counter = rs_items;
+    ra.mismatch = ctr + 5;
+    const int64_t blocks = nq < 2048 ? nq : 2048;
+    rescore_kernel<0><<<(unsigned)blocks, kRsThreads, 0, st>>>(ra);
     CX_HIP(hipGetLastError());
+
+    // hard queries: ordered list, pass-2 counts, pool offsets, slices
+    hard_flag_kernel<<<qb, 256, 0, st>>>(cnt, nq, flag);
+    CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan32_bytes, flag, pos, (int)nq, st));
+    hard_scatter_kernel<<<qb, 256, 0, st>>>(cnt, flag, pos, nq, hard, hcnt, hcnt64, hch,
+                                            n_hard_d);
     prof_mark(t, 6);
-    CX_HIP(hipStreamSynchronize(st));
-    const int32_t n_hard = *t.host_word;
-    HardArgs ha{};
-    ha.r = ra;
-    ha.hard = hard;
-    ha.nseg = (int32_t)((n + kHardSeg - 1) / kHardSeg);
-    ha.part = part;
-    for (int32_t h0 = 0; h0 < n_hard; h0 += kHardBatch) {
-        ha.h0 = h0;
-        ha.hn = n_hard - h0 < kHardBatch ? n_hard - h0 : kHardBatch;
-        ha.counter = ctr + 2;
-        if (h0 > 0)
-            CX_HIP(hipMemsetAsync(ha.counter, 0, sizeof(int32_t), st));
-        hard_score_kernel<<<1024, 256, 0, st>>>(ha);
-        CX_HIP(hipGetLastError());
-        const int64_t blocks = ha.hn < 2048 ? ha.hn : 2048;
-        hard_merge_kernel<<<(unsigned)blocks, 256, 0, st>>>(ha);
-        CX_HIP(hipGetLastError());
+    const int32_t n_hard = read_word(t, n_hard_d);
+    t.last_hard = n_hard;
+    if (n_hard > 0) {
+        CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan64_bytes, hcnt64, hoff, n_hard,
+                                                st));
+        CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan64_bytes, hch, choff, n_hard,
+                                                st));
+        hard_totals_kernel<<<1, 1, 0, st>>>(hoff, hcnt64, choff, hch, n_hard, totals);
+        const int64_t pool_total = read_word(t, totals);
+        const int64_t chunk_total = read_word(t, totals + 1);
+        // batches of hard queries whose lists fit the pool and whose slice
+        // lists fit the partial buffer
+        const int64_t pool_max = (int64_t)1 << 28;
+        const int64_t chunk_max = (int64_t)1 << 16;
+        const bool split = pool_total > pool_max || chunk_total > chunk_max;
+        std::vector<int64_t> h_off, h_ch;
+        if (split) {
+            h_off.resize((size_t)n_hard + 1);
+            h_ch.resize((size_t)n_hard + 1);
+            CX_HIP(hipMemcpy(h_off.data(), hoff, sizeof(int64_t) * n_hard,
+                             hipMemcpyDeviceToHost));
+            CX_HIP(hipMemcpy(h_ch.data(), choff, sizeof(int64_t) * n_hard,
+                             hipMemcpyDeviceToHost));
+            h_off[(size_t)n_hard] = pool_total;
+            h_ch[(size_t)n_hard] = chunk_total;
+        }
+        const size_t pool_need = (size_t)std::min(pool_total, pool_max);
+        const size_t part_need = sizeof(HardEntry) * (size_t)std::min(chunk_total, chunk_max) * k;
+        if (t.pool.count < pool_need || t.part.count < part_need) {
+            CX_HIP(hipStreamSynchronize(st));
+            if (t.pool.count < pool_need)
+                t.pool.alloc(pool_need);
+            if (t.part.count < part_need)
+                t.part.alloc(part_need);
+        }
+        ScreenArgs sh = sa;
+        sh.list = t.pool.ptr;
+        sh.cnt = cnt3;
+        for (int32_t h0 = 0; h0 < n_hard;) {
+            int32_t h1 = n_hard;
+            if (split) {
+                h1 = h0 + 1;
+                while (h1 < n_hard && h_off[(size_t)h1 + 1] - h_off[(size_t)h0] <= pool_max &&
+                       h_ch[(size_t)h1 + 1] - h_ch[(size_t)h0] <= chunk_max)
+                    h1++;
+                CX_REQUIRE(h_off[(size_t)h1] - h_off[(size_t)h0] <= pool_max &&
+                               h_ch[(size_t)h1] - h_ch[(size_t)h0] <= chunk_max,
+                           "a single query's candidate list exceeds the hard pool");
+            }
+            const int32_t hn = h1 - h0;
+            const int64_t base = split ? h_off[(size_t)h0] : 0;
+            const int64_t cbase = split ? h_ch[(size_t)h0] : 0;
+            const int64_t nch = split ? h_ch[(size_t)h1] - h_ch[(size_t)h0] : chunk_total;
+            CX_HIP(hipMemsetAsync(cnt3, 0, sizeof(int32_t) * hn, st));
+            sh.qmap = hard + h0;
+            sh.cap = hcnt + h0;
+            sh.off = hoff + h0;
+            sh.off_base = base;
+            sh.nq = hn;
+            sh.n_tiles = (hn + kQT - 1) / kQT;
+            screen_kernel<2><<<dim3((unsigned)sh.n_tiles, n_seg), kScreenThreads, dyn_lds, st>>>(
+                sh);
+            CX_HIP(hipGetLastError());
+            RescoreArgs rh = ra;
+            rh.h0 = h0;
+            rh.hn = hn;
+            rh.hard = hard;
+            rh.cap = hcnt;
+            rh.off = hoff;
+            rh.off_base = base;
+            rh.chunk_off = choff;
+            rh.chunk_base = cbase;
+            rh.n_chunks = nch;
+            rh.part = (HardEntry *)t.part.ptr;
+            rh.cnt = cnt3;
+            rh.list = t.pool.ptr;
+            rh.counter = hs_items;
+            CX_HIP(hipMemsetAsync(hs_items, 0, sizeof(int64_t), st));
+            const int64_t hb = nch < 2048 ? nch : 2048;
+            rescore_kernel<1><<<(unsigned)hb, kRsThreads, 0, st>>>(rh);
+            CX_HIP(hipGetLastError());
+            const int64_t mb = hn < 2048 ? hn : 2048;
+            hard_merge_kernel<<<(unsigned)mb, kRsThreads, 0, st>>>(rh);
+            CX_HIP(hipGetLastError());
+            h0 = h1;
+        }
+        const int32_t bad = read_word(t, ctr + 5);
+        CX_REQUIRE(bad == 0, "internal error: hard-query list size differs from its count");
     }
     prof_mark(t, 7);
-    t.last_hard = n_hard;
 }
 
 } // namespace cx
