@@ -148,6 +148,10 @@ cx_status cx_tree_build_residual(const float *V, const float *p, const float *co
             t.d = d;
             t.var = var;
             t.kernel = kernel_type == CX_KERNEL_MATERN52 ? 1 : 0; // src/metric.rs:50-56
+            // smallest p (not NaN-aware: NaN compares false and stays out)
+            t.p_min = 1.0f;
+            for (int64_t i = 0; i < n; i++)
+                t.p_min = p[i] < t.p_min ? p[i] : t.p_min;
             // inv_ls[i] = 1 / (|ls| < 1e-6 ? 1e-6 : ls), fallback ls[0] or 1
             // (ResidualMetric::new, src/metric.rs:93-101)
             const float fallback = (ls && n_ls > 0) ? ls[0] : 1.0f;
@@ -159,6 +163,15 @@ cx_status cx_tree_build_residual(const float *V, const float *p, const float *co
                 t.ls[c] = safe;
                 t.inv_ls[c] = 1.0f / safe;
             }
+            // largest |coordinate / lengthscale| (bounds the rounding of the
+            // scaled coordinates the mixed-precision screen reads)
+            double xs = 0.0;
+            for (int64_t i = 0; i < n; i++)
+                for (int32_t c = 0; c < d; c++) {
+                    const double v = std::fabs((double)coords[i * d + c] * (double)t.inv_ls[c]);
+                    xs = v > xs ? v : xs;
+                }
+            t.xs_max = xs;
             if (n > 0)
                 build_residual_tree(t, V, p, coords, chunk, order_mode, grid_scale);
         } catch (...) {

@@ -121,7 +121,9 @@ struct TreeState {
     DevBuf<uint8_t> work;
     // pinned host word for small device-to-host reads (hipHostMalloc)
     int32_t *host_word = nullptr;
-    int32_t last_hard = 0; // queries scored exhaustively by the last mixed kNN
+    int32_t last_hard = 0; // overflowed queries of the last mixed kNN
+    float p_min = 1.0f;    // smallest p_diag, host-side at build
+    double xs_max = 0.0;   // largest |coordinate / lengthscale|, host-side at build
 
     // optional per-phase timing of the last query: events recorded on the
     // tree's stream at phase boundaries (see cx_tree_last_profile)

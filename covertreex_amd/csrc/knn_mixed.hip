@@ -92,16 +92,22 @@ __global__ void split_kernel(const float *__restrict__ v, const int64_t *__restr
     }
 }
 
+struct MetaScale {
+    float il[kMaxD];
+};
+
+// dataset-order coordinates scaled by 1/lengthscale, and 1/sqrt(p)
 __global__ void gather_meta_kernel(const float *__restrict__ raw, const float *__restrict__ p,
                                    const int64_t *__restrict__ inv_order, int64_t n, int32_t d,
-                                   float *__restrict__ x_ds, float *__restrict__ isp_ds)
+                                   MetaScale sc, float *__restrict__ x_ds,
+                                   float *__restrict__ isp_ds)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const int64_t t = inv_order[i];
     for (int32_t c = 0; c < d; c++)
-        x_ds[i * d + c] = raw[t * d + c];
+        x_ds[i * d + c] = raw[t * d + c] * sc.il[c];
     // correctly rounded sqrt and division: relative error <= 2^-23
     isp_ds[i] = 1.0f / __builtin_sqrtf(p[t]);
 }
@@ -110,7 +116,7 @@ __global__ void gather_meta_kernel(const float *__restrict__ raw, const float *_
 
 struct ScreenArgs {
     const uint16_t *vhi, *vlo; // npad x mp, dataset order, zero rows past n
-    const float *x;     // n x d raw coords, dataset order
+    const float *x;     // n x d coords / lengthscale, dataset order
     const float *isp;   // n: 1/sqrt(p), relative error <= 2^-23
     const float *vnorm; // n
     const int64_t *qsorted;  // sorted query dataset indices (nq)
@@ -119,8 +125,8 @@ struct ScreenArgs {
     int64_t nq;
     int32_t d, mp, kernel, pred, k;
     float var;
-    float inv_ls[kMaxD];
     float cD, eK;
+    float kA, kB, kC;  // folded kernel constants (fast_kernel)
     float *tau;      // nq
     int32_t *hist;   // nq x kBins
     int32_t *cnt;    // nq
@@ -139,23 +145,27 @@ __device__ __forceinline__ int64_t screen_qpos(const ScreenArgs &A, int64_t t)
     return A.qmap ? (int64_t)A.qmap[t] : t;
 }
 
-// Kernel value in FP32 from the hardware exp2/sqrt (about 1 ulp each); its
-// error is far below eK = var * 2^-18.
-__device__ __forceinline__ float fast_kernel(float d2, float var, int32_t kernel)
+// Kernel value in FP32 from the hardware exp2/sqrt (about 1 ulp each) with
+// the constants folded: RBF var * 2^(kA d2), kA = -log2(e)/2; Matern-5/2 with
+// a' = sqrt(kA d2) = log2(e) sqrt(5 d2): (var + kB a' + kC a'^2) 2^-a'.  The
+// error stays far below eK = var * 2^-18.
+template <int KT> __device__ __forceinline__ float fast_kernel(float d2, const ScreenArgs &A)
 {
-    constexpr float kLog2e = 1.44269504f;
-    if (kernel == 0)
-        return var * __builtin_amdgcn_exp2f(d2 * (-0.5f * kLog2e));
-    const float a = __builtin_amdgcn_sqrtf(5.0f * d2);
-    const float poly = __builtin_fmaf(a, __builtin_fmaf(a, 0.333333343f, 1.0f), 1.0f);
-    return var * poly * __builtin_amdgcn_exp2f(-a * kLog2e);
+    if (KT == 0)
+        return A.var * __builtin_amdgcn_exp2f(d2 * A.kA);
+    const float a = __builtin_amdgcn_sqrtf(d2 * A.kA);
+    const float poly = __builtin_fmaf(a, __builtin_fmaf(a, A.kC, A.kB), A.var);
+    return poly * __builtin_amdgcn_exp2f(-a);
 }
 
 // One workgroup = 128 candidates (MFMA rows) x 128 queries (MFMA columns) per
 // candidate tile, looping over a 2048-candidate segment.  Eight wavefronts:
 // wavefront w owns candidate rows 32*(w&3).. and query columns 64*(w>>2)..,
 // i.e. two 32x32 output blocks, each with a hi.hi and a cross accumulator.
-template <int PASS>
+// DD > 0 fixes the coordinate dimension (query coordinates then live in
+// registers); DD == 0 reads them from LDS for any d <= kMaxD.  KT is the
+// kernel (0 RBF, 1 Matern-5/2).  Coordinates arrive scaled by 1/lengthscale.
+template <int PASS, int DD, int KT>
 __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
 {
     __shared__ __attribute__((aligned(16))) uint16_t sAh[kCT * kKP];
@@ -175,7 +185,8 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
     // blockIdx.x = query tile, blockIdx.y = candidate segment; heavy tiles first
     const int32_t tile = A.n_tiles - 1 - (int32_t)blockIdx.x;
     const int64_t s0 = (int64_t)tile * kQT;
-    const int32_t n = A.n, d = A.d, mp = A.mp;
+    const int32_t n = A.n, mp = A.mp;
+    const int32_t d = DD > 0 ? DD : A.d;
     int32_t c_end = n;
     if (A.pred) {
         // candidate range of this tile: [0, largest query index) under the mask
@@ -200,10 +211,11 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
         for (int i = tid; i < kQT * kBins / 2; i += kScreenThreads)
             sHist[i] = 0u;
 
-    // this lane's two query columns: mask limit, own row, weight, norm term,
-    // and (pass 2) the emission threshold on the |rho| upper bound
+    // this lane's two query columns: mask limit, own row, weight 1/sqrt(p),
+    // norm term, and (pass 2) the emission threshold on the |rho| upper bound
     int32_t qcol[2], qlim[2], qself[2];
     float qw[2], qcn[2], qthr[2];
+    float qx[2][DD > 0 ? DD : 1];
 #pragma unroll
     for (int b = 0; b < 2; b++) {
         qcol[b] = 64 * qh + 32 * b + (lane & 31);
@@ -215,14 +227,22 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
         qlim[b] = !live ? 0 : (A.pred ? (qid < 0 ? n : (qid < n ? (int32_t)qid : n)) : n);
         qself[b] = live ? row : -1;
         qw[b] = A.isp[row];
-        qcn[b] = A.cD * A.vnorm[row];
+        qcn[b] = A.cD * 1.000002f * A.vnorm[row];
         qthr[b] = 2.0f;
         if (PASS == 2 && live) {
             const float t = A.tau[sp];
             // |rho| >= 1 - tau is implied by d^2 <= tau; 2^-22 covers rounding
             qthr[b] = t >= 2.0f ? -1.0f : (1.0f - t) - 2.3841858e-7f;
         }
+        if (DD > 0)
+#pragma unroll
+            for (int c = 0; c < (DD > 0 ? DD : 1); c++)
+                qx[b][c] = A.x[(int64_t)row * d + c];
     }
+    const float eKp = A.eK * 1.000002f;
+    // pass 1 counts a pair when 1 - rlo + 1.5e-7 < kHistCap; screen first on
+    // rlo itself with a slightly lower threshold
+    const float thr1 = 1.0f + 1.5e-7f - kHistCap - 1e-6f;
 
     const int rbase = 32 * cb + 4 * (lane >> 5);
     for (int32_t c0 = c_begin; c0 < c_end; c0 += kCT) {
@@ -286,6 +306,7 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
         }
         // epilogue: output register r of block b is candidate row
         // rbase + 8*(r/4) + r%4, query column qcol[b]
+        uint32_t emit[2] = {0u, 0u};
 #pragma unroll
         for (int g = 0; g < 4; g++) {
             const int rg = rbase + 8 * g;
@@ -297,18 +318,33 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
 #pragma unroll
                 for (int rr = 0; rr < 4; rr++)
                     d2[b][rr] = 0.0f;
-            for (int c = 0; c < d; c++) {
-                const float4 cx = *(const float4 *)(sCx + c * kCT + rg);
-                const float il = A.inv_ls[c];
+            if (DD > 0) {
 #pragma unroll
-                for (int b = 0; b < 2; b++) {
-                    const float qx = sQx[c * kQT + qcol[b]];
-                    const float t0 = (qx - cx.x) * il, t1 = (qx - cx.y) * il;
-                    const float t2 = (qx - cx.z) * il, t3 = (qx - cx.w) * il;
-                    d2[b][0] = __builtin_fmaf(t0, t0, d2[b][0]);
-                    d2[b][1] = __builtin_fmaf(t1, t1, d2[b][1]);
-                    d2[b][2] = __builtin_fmaf(t2, t2, d2[b][2]);
-                    d2[b][3] = __builtin_fmaf(t3, t3, d2[b][3]);
+                for (int c = 0; c < (DD > 0 ? DD : 1); c++) {
+                    const float4 cx = *(const float4 *)(sCx + c * kCT + rg);
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        const float t0 = qx[b][c] - cx.x, t1 = qx[b][c] - cx.y;
+                        const float t2 = qx[b][c] - cx.z, t3 = qx[b][c] - cx.w;
+                        d2[b][0] = __builtin_fmaf(t0, t0, d2[b][0]);
+                        d2[b][1] = __builtin_fmaf(t1, t1, d2[b][1]);
+                        d2[b][2] = __builtin_fmaf(t2, t2, d2[b][2]);
+                        d2[b][3] = __builtin_fmaf(t3, t3, d2[b][3]);
+                    }
+                }
+            } else {
+                for (int c = 0; c < d; c++) {
+                    const float4 cx = *(const float4 *)(sCx + c * kCT + rg);
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        const float q = sQx[c * kQT + qcol[b]];
+                        const float t0 = q - cx.x, t1 = q - cx.y;
+                        const float t2 = q - cx.z, t3 = q - cx.w;
+                        d2[b][0] = __builtin_fmaf(t0, t0, d2[b][0]);
+                        d2[b][1] = __builtin_fmaf(t1, t1, d2[b][1]);
+                        d2[b][2] = __builtin_fmaf(t2, t2, d2[b][2]);
+                        d2[b][3] = __builtin_fmaf(t3, t3, d2[b][3]);
+                    }
                 }
             }
 #pragma unroll
@@ -319,40 +355,64 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
                     const int32_t j = c0 + rg + rr;
                     const float cwv = rr == 0 ? cw.x : rr == 1 ? cw.y : rr == 2 ? cw.z : cw.w;
                     const float cnv = rr == 0 ? cn.x : rr == 1 ? cn.y : rr == 2 ? cn.z : cn.w;
-                    const float kv = fast_kernel(d2[b][rr], A.var, A.kernel);
+                    const float kv = fast_kernel<KT>(d2[b][rr], A);
                     const float num = kv - (acc_h[b][r] + acc_x[b][r]);
-                    const float an = __builtin_fabsf(num);
                     // 1/sqrt(max(p_q p_j, 1e-18)) from per-row 1/sqrt(p)
                     const float w = __builtin_fminf(qw[b] * cwv, 1e9f);
-                    const float ar = an * w;
-                    // certified |rho| error: kernel, dot (cD |V_q||V_j|), the
-                    // numerator's rounding, the weight's and the products'
-                    const float e1 = __builtin_fmaf(an, 1.2e-7f, __builtin_fmaf(qcn[b], cnv, A.eK));
-                    const float erho = __builtin_fmaf(ar, 6e-7f, e1 * (w * 1.000002f));
+                    const float ar = __builtin_fabsf(num) * w;
+                    // certified |rho| error: w (eK + cD |V_q||V_j|) (1 + 2e-6)
+                    // for the kernel, the dot and the final sum, plus 7.2e-7 ar
+                    // for the roundings of num, w and ar
+                    const float wE = w * __builtin_fmaf(qcn[b], cnv, eKp);
                     const bool valid = j < qlim[b];
-                    const bool self = j == qself[b];
                     if (PASS == 1) {
-                        // squared upper bound 1 - (|rho| lower bound), rounded up
-                        float ub2 = (1.0f - (ar - erho)) + 1.5e-7f;
-                        ub2 = self ? 0.0f : ub2;
-                        if (valid && ub2 < kHistCap) {
-                            const int bin = (int)(ub2 * (float)kBins);
-                            atomicAdd(&sHist[(qcol[b] * kBins + bin) >> 1],
-                                      (bin & 1) ? 65536u : 1u);
-                        }
-                    } else {
-                        if (valid && (ar + erho >= qthr[b] || self)) {
-                            const int64_t s = s0 + qcol[b];
-                            const int32_t pos = atomicAdd(&A.cnt[s], 1);
-                            if (A.qmap) {
-                                if (pos < A.cap[s])
-                                    A.list[A.off[s] - A.off_base + pos] = j;
-                            } else if (pos < kCap) {
-                                A.list[s * kCap + pos] = j;
+                        // the pair's own row may be counted at its computed
+                        // bound or not at all: tau stays an upper bound
+                        const float rlo = __builtin_fmaf(ar, 1.0f - 7.2e-7f, -wE);
+                        if (valid && rlo > thr1) {
+                            // squared upper bound 1 - (|rho| lower bound), rounded up
+                            const float ub2 = (1.0f - rlo) + 1.5e-7f;
+                            if (ub2 < kHistCap) {
+                                const int bin = (int)(ub2 * (float)kBins);
+                                atomicAdd(&sHist[(qcol[b] * kBins + bin) >> 1],
+                                          (bin & 1) ? 65536u : 1u);
                             }
                         }
+                    } else {
+                        const float rhi = __builtin_fmaf(ar, 1.0f + 7.2e-7f, wE);
+                        if (valid && rhi >= qthr[b])
+                            emit[b] |= 1u << r;
                     }
                 }
+        }
+        if (PASS == 2) {
+            // one atomic per query column and wavefront: lanes l and l + 32
+            // share a column and split the reserved range
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                // the query's own row (distance 0) is always listed
+                const int32_t dr = qself[b] - c0 - rbase;
+                if (dr >= 0 && dr < 28 && (dr & 7) < 4 && qself[b] < qlim[b])
+                    emit[b] |= 1u << (4 * (dr >> 3) + (dr & 3));
+                const int c_me = __builtin_popcount(emit[b]);
+                const int c_other = __shfl_xor(c_me, 32, 64);
+                const int64_t s = s0 + qcol[b];
+                int base = 0;
+                if (lane < 32 && c_me + c_other > 0)
+                    base = atomicAdd(&A.cnt[s], c_me + c_other);
+                base = __shfl(base, lane & 31, 64);
+                int p = base + (lane < 32 ? 0 : c_other);
+                uint32_t mk = emit[b];
+                const int32_t cap = A.qmap ? A.cap[s] : kCap;
+                int32_t *dst = A.qmap ? A.list + (A.off[s] - A.off_base) : A.list + s * kCap;
+                while (mk) {
+                    const int r = __builtin_ctz(mk);
+                    mk &= mk - 1;
+                    if (p < cap)
+                        dst[p] = c0 + rbase + 8 * (r >> 2) + (r & 3);
+                    p++;
+                }
+            }
         }
     }
     if (PASS == 1) {
@@ -369,6 +429,35 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
             }
         }
     }
+}
+
+template <int PASS, int KT>
+static void launch_screen_kt(const ScreenArgs &A, unsigned n_seg, size_t lds, hipStream_t st)
+{
+    const dim3 grid((unsigned)A.n_tiles, n_seg);
+    switch (A.d) {
+    case 2:
+        screen_kernel<PASS, 2, KT><<<grid, kScreenThreads, lds, st>>>(A);
+        break;
+    case 3:
+        screen_kernel<PASS, 3, KT><<<grid, kScreenThreads, lds, st>>>(A);
+        break;
+    case 5:
+        screen_kernel<PASS, 5, KT><<<grid, kScreenThreads, lds, st>>>(A);
+        break;
+    default:
+        screen_kernel<PASS, 0, KT><<<grid, kScreenThreads, lds, st>>>(A);
+    }
+    CX_HIP(hipGetLastError());
+}
+
+template <int PASS> static void launch_screen(const ScreenArgs &A, unsigned n_seg, size_t lds,
+                                              hipStream_t st)
+{
+    if (A.kernel == 0)
+        launch_screen_kt<PASS, 0>(A, n_seg, lds, st);
+    else
+        launch_screen_kt<PASS, 1>(A, n_seg, lds, st);
 }
 
 // tau_q (squared scale): upper edge of the first bin where the cumulative count
@@ -727,7 +816,7 @@ __global__ void __launch_bounds__(kRsThreads) hard_merge_kernel(RescoreArgs A)
 // ---- hard queries ------------------------------------------------------------
 // A query whose list overflowed kCap (a wide screen interval, e.g. tiny p) is
 // listed again in full: its pass-2 count sizes a slice of a pool, the same
-// screen_kernel<2> re-runs over tiles of the hard queries only and writes
+// pass-2 screen re-runs over tiles of the hard queries only and writes
 // there, and rescore_kernel scores the slices.
 
 __global__ void hard_flag_kernel(const int32_t *__restrict__ cnt, int64_t nq, int32_t *flag)
@@ -758,14 +847,18 @@ __global__ void hard_scatter_kernel(const int32_t *__restrict__ cnt,
         *n_hard = pos[s] + flag[s];
 }
 
-// totals[0] = pool entries, totals[1] = slices over all hard queries
-__global__ void hard_totals_kernel(const int64_t *__restrict__ off,
+// totals = {hard queries, pool entries, slices} (the scans run over all nq
+// positions; those past the hard count hold zeros)
+__global__ void hard_totals_kernel(const int32_t *__restrict__ n_hard,
+                                   const int64_t *__restrict__ off,
                                    const int64_t *__restrict__ cnt64,
                                    const int64_t *__restrict__ choff,
-                                   const int64_t *__restrict__ chunks, int32_t n, int64_t *totals)
+                                   const int64_t *__restrict__ chunks, int64_t *totals)
 {
-    totals[0] = off[n - 1] + cnt64[n - 1];
-    totals[1] = choff[n - 1] + chunks[n - 1];
+    const int32_t h = *n_hard;
+    totals[0] = h;
+    totals[1] = h > 0 ? off[h - 1] + cnt64[h - 1] : 0;
+    totals[2] = h > 0 ? choff[h - 1] + chunks[h - 1] : 0;
 }
 
 // ---- driver -----------------------------------------------------------------
@@ -808,24 +901,27 @@ static void prepare_mixed(TreeState &t)
     t.isp_ds.alloc(npad);
     CX_HIP(hipMemsetAsync(t.x_ds.ptr, 0, t.x_ds.bytes(), st));
     CX_HIP(hipMemsetAsync(t.isp_ds.ptr, 0, t.isp_ds.bytes(), st));
+    MetaScale sc{};
+    for (int c = 0; c < t.d; c++)
+        sc.il[c] = t.inv_ls[c];
     gather_meta_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
-        t.raw.ptr, t.p.ptr, t.inv_order.ptr, n, t.d, t.x_ds.ptr, t.isp_ds.ptr);
+        t.raw.ptr, t.p.ptr, t.inv_order.ptr, n, t.d, sc, t.x_ds.ptr, t.isp_ds.ptr);
     CX_HIP(hipGetLastError());
     CX_HIP(hipStreamSynchronize(st));
     t.mixed_ready = true;
 }
 
-// read a few device words through the tree's pinned host buffer (synchronises)
-template <typename T> static T read_word(TreeState &t, const T *dptr)
+// read up to eight device words through the tree's pinned host buffer
+// (synchronises the tree's stream)
+template <typename T> static void read_words(TreeState &t, const T *dptr, int count, T *out)
 {
-    static_assert(sizeof(T) <= 8, "one word");
+    CX_REQUIRE(count >= 1 && (size_t)count * sizeof(T) <= 64, "read_words: at most 64 bytes");
     if (!t.host_word)
         CX_HIP(hipHostMalloc((void **)&t.host_word, 64, hipHostMallocDefault));
-    CX_HIP(hipMemcpyAsync(t.host_word, dptr, sizeof(T), hipMemcpyDeviceToHost, t.stream));
+    CX_HIP(hipMemcpyAsync(t.host_word, dptr, sizeof(T) * count, hipMemcpyDeviceToHost,
+                          t.stream));
     CX_HIP(hipStreamSynchronize(t.stream));
-    T v;
-    std::memcpy(&v, t.host_word, sizeof(T));
-    return v;
+    std::memcpy(out, t.host_word, sizeof(T) * count);
 }
 
 void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, int32_t predecessor,
@@ -893,8 +989,8 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     int32_t *cnt3 = (int32_t *)(ws + o_cnt3);
     int64_t *hch = (int64_t *)(ws + o_hch), *choff = (int64_t *)(ws + o_choff);
     // counters: ctr[0..1] rescore items (int64), ctr[2..3] hard slice items
-    // (int64), ctr[4] hard count, ctr[5] pool-size mismatches, ctr[8..11]
-    // pool entries and slices over all hard queries (int64)
+    // (int64), ctr[4] hard count, ctr[5] pool-size mismatches, ctr[8..13]
+    // hard count, pool entries and slices over all hard queries (int64)
     int64_t *rs_items = (int64_t *)ctr, *hs_items = (int64_t *)(ctr + 2);
     int32_t *n_hard_d = ctr + 4;
     int64_t *totals = (int64_t *)(ctr + 8);
@@ -912,7 +1008,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     ScreenArgs sa{};
     sa.vhi = t.vhi.ptr;
     sa.vlo = t.vlo.ptr;
-    sa.x = t.x_ds.ptr;
+    sa.x = t.x_ds.ptr; // scaled by 1/lengthscale
     sa.isp = t.isp_ds.ptr;
     sa.vnorm = t.vnorm.ptr;
     sa.qsorted = qs;
@@ -925,8 +1021,6 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     sa.pred = predecessor ? 1 : 0;
     sa.k = k;
     sa.var = t.var;
-    for (int c = 0; c < t.d; c++)
-        sa.inv_ls[c] = t.inv_ls[c];
     // |computed V.V - exact| <= cD |V_q||V_j|: the hi/lo split drops at most
     // 3.1 * 2^-18 of each product, the hi.hi FP32 accumulation adds gamma_mp,
     // the cross terms (2^-8 smaller) gamma_2mp, and the final sum one rounding.
@@ -934,7 +1028,19 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     const double g1 = (t.mp + 16) * u / (1.0 - (t.mp + 16) * u);
     const double g2 = (2 * t.mp + 16) * u / (1.0 - (2 * t.mp + 16) * u);
     sa.cD = (float)((3.1 / 262144.0 + g1 * 1.01 + g2 * 0.01 + 2 * u) * 1.05);
-    sa.eK = (float)(std::fabs((double)t.var) / 262144.0); // fast exp/kernel in FP32
+    // kernel value error: the FP32 kernel (exp2/sqrt, folded constants) stays
+    // within 32 u var; scaled coordinates |x/l| <= R add at most
+    // u var (1.3 R sqrt(d) + 0.4 (d + 2)) through d^2 (both kernels)
+    sa.eK = (float)(std::fabs((double)t.var) * u *
+                    (64.0 + 1.5 * t.xs_max * std::sqrt((double)t.d) + (t.d + 2.0)));
+    const double l2e = 1.4426950408889634;
+    if (t.kernel == 0) {
+        sa.kA = (float)(-0.5 * l2e);
+    } else {
+        sa.kA = (float)(5.0 * l2e * l2e);
+        sa.kB = (float)((double)t.var / l2e);
+        sa.kC = (float)((double)t.var / (3.0 * l2e * l2e));
+    }
     sa.tau = tau;
     sa.hist = hist;
     sa.cnt = cnt;
@@ -942,13 +1048,11 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     sa.n_tiles = (int32_t)((nq + kQT - 1) / kQT);
     const unsigned n_seg = (unsigned)((n + kSeg - 1) / kSeg);
     const size_t dyn_lds = sizeof(float) * 2 * (size_t)t.d * kCT;
-    screen_kernel<1><<<dim3((unsigned)sa.n_tiles, n_seg), kScreenThreads, dyn_lds, st>>>(sa);
-    CX_HIP(hipGetLastError());
+    launch_screen<1>(sa, n_seg, dyn_lds, st);
     prof_mark(t, 3);
     tau_kernel<<<qb, 256, 0, st>>>(hist, nq, k, tau);
     prof_mark(t, 4);
-    screen_kernel<2><<<dim3((unsigned)sa.n_tiles, n_seg), kScreenThreads, dyn_lds, st>>>(sa);
-    CX_HIP(hipGetLastError());
+    launch_screen<2>(sa, n_seg, dyn_lds, st);
     prof_mark(t, 5);
 
     RescoreArgs ra{};
@@ -984,21 +1088,22 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     CX_HIP(hipGetLastError());
 
     // hard queries: ordered list, pass-2 counts, pool offsets, slices
+    CX_HIP(hipMemsetAsync(hcnt64, 0, sizeof(int64_t) * nq, st));
+    CX_HIP(hipMemsetAsync(hch, 0, sizeof(int64_t) * nq, st));
     hard_flag_kernel<<<qb, 256, 0, st>>>(cnt, nq, flag);
     CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan32_bytes, flag, pos, (int)nq, st));
     hard_scatter_kernel<<<qb, 256, 0, st>>>(cnt, flag, pos, nq, hard, hcnt, hcnt64, hch,
                                             n_hard_d);
+    CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan64_bytes, hcnt64, hoff, (int)nq, st));
+    CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan64_bytes, hch, choff, (int)nq, st));
+    hard_totals_kernel<<<1, 1, 0, st>>>(n_hard_d, hoff, hcnt64, choff, hch, totals);
     prof_mark(t, 6);
-    const int32_t n_hard = read_word(t, n_hard_d);
+    int64_t tot[3];
+    read_words(t, totals, 3, tot);
+    const int32_t n_hard = (int32_t)tot[0];
     t.last_hard = n_hard;
     if (n_hard > 0) {
-        CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan64_bytes, hcnt64, hoff, n_hard,
-                                                st));
-        CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan64_bytes, hch, choff, n_hard,
-                                                st));
-        hard_totals_kernel<<<1, 1, 0, st>>>(hoff, hcnt64, choff, hch, n_hard, totals);
-        const int64_t pool_total = read_word(t, totals);
-        const int64_t chunk_total = read_word(t, totals + 1);
+        const int64_t pool_total = tot[1], chunk_total = tot[2];
         // batches of hard queries whose lists fit the pool and whose slice
         // lists fit the partial buffer
         const int64_t pool_max = (int64_t)1 << 28;
@@ -1049,9 +1154,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
             sh.off_base = base;
             sh.nq = hn;
             sh.n_tiles = (hn + kQT - 1) / kQT;
-            screen_kernel<2><<<dim3((unsigned)sh.n_tiles, n_seg), kScreenThreads, dyn_lds, st>>>(
-                sh);
-            CX_HIP(hipGetLastError());
+            launch_screen<2>(sh, n_seg, dyn_lds, st);
             RescoreArgs rh = ra;
             rh.h0 = h0;
             rh.hn = hn;
@@ -1075,7 +1178,8 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
             CX_HIP(hipGetLastError());
             h0 = h1;
         }
-        const int32_t bad = read_word(t, ctr + 5);
+        int32_t bad = 0;
+        read_words(t, ctr + 5, 1, &bad);
         CX_REQUIRE(bad == 0, "internal error: hard-query list size differs from its count");
     }
     prof_mark(t, 7);
