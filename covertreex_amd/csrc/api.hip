@@ -39,11 +39,13 @@ TreeState::~TreeState()
     vnorm.release();
     x_ds.release();
     isp_ds.release();
-    pool.release();
+    codes.release();
     part.release();
     work.release();
     if (host_word)
         (void)hipHostFree(host_word);
+    if (sync_event)
+        (void)hipEventDestroy(sync_event);
     for (int i = 0; i < kMaxMarks; i++)
         if (marks[i])
             (void)hipEventDestroy(marks[i]);

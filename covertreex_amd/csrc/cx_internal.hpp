@@ -114,13 +114,14 @@ struct TreeState {
     int32_t mp = 0;                // m padded to a multiple of 32
     DevBuf<float> vnorm;           // |V_i| rounded up, dataset order
     DevBuf<float> x_ds, isp_ds;    // raw coords and 1/sqrt(p) in dataset order
-    DevBuf<int32_t> pool;          // full candidate lists of overflowed queries
-    DevBuf<uint8_t> part;          // per-slice top-k lists of overflowed queries
+    DevBuf<uint8_t> codes;         // one byte per screened pair (a launch group)
+    DevBuf<uint8_t> part;          // per-segment top-k lists of overflowed queries
 
     // reusable query workspace
     DevBuf<uint8_t> work;
     // pinned host word for small device-to-host reads (hipHostMalloc)
     int32_t *host_word = nullptr;
+    hipEvent_t sync_event = nullptr; // marks a small device-to-host read
     int32_t last_hard = 0; // overflowed queries of the last mixed kNN
     float p_min = 1.0f;    // smallest p_diag, host-side at build
     double xs_max = 0.0;   // largest |coordinate / lengthscale|, host-side at build

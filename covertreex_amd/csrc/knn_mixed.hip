@@ -131,24 +131,16 @@ struct ScreenArgs {
     int32_t *hist;   // nq x kBins
     int32_t *cnt;    // nq
     int32_t *list;   // nq x kCap
-    int32_t n_tiles;
-    // optional remap (hard queries): tile position -> sorted query position,
-    // and each position's list capacity and offset (minus off_base)
-    const int32_t *qmap;
-    const int32_t *cap;
-    const int64_t *off;
-    int64_t off_base;
+    // this launch covers query tiles [tile0, tile0 + n_tiles) x all segments;
+    // codes hold one byte per (candidate, query) pair of those tiles
+    int32_t tile0, n_tiles, n_seg;
+    uint32_t *codes;
 };
-
-__device__ __forceinline__ int64_t screen_qpos(const ScreenArgs &A, int64_t t)
-{
-    return A.qmap ? (int64_t)A.qmap[t] : t;
-}
 
 // Kernel value in FP32 from the hardware exp2/sqrt (about 1 ulp each) with
 // the constants folded: RBF var * 2^(kA d2), kA = -log2(e)/2; Matern-5/2 with
 // a' = sqrt(kA d2) = log2(e) sqrt(5 d2): (var + kB a' + kC a'^2) 2^-a'.  The
-// error stays far below eK = var * 2^-18.
+// error stays far below eK.
 template <int KT> __device__ __forceinline__ float fast_kernel(float d2, const ScreenArgs &A)
 {
     if (KT == 0)
@@ -158,14 +150,61 @@ template <int KT> __device__ __forceinline__ float fast_kernel(float d2, const S
     return poly * __builtin_amdgcn_exp2f(-a);
 }
 
-// One workgroup = 128 candidates (MFMA rows) x 128 queries (MFMA columns) per
-// candidate tile, looping over a 2048-candidate segment.  Eight wavefronts:
-// wavefront w owns candidate rows 32*(w&3).. and query columns 64*(w>>2)..,
-// i.e. two 32x32 output blocks, each with a hi.hi and a cross accumulator.
+// First word of the packed codes of (tile block, candidate tile, wavefront,
+// lane): eight consecutive words per lane, word 2g + b holding candidate rows
+// rbase + 8g .. + 3 (one byte each) against query column qcol[b], so a
+// wavefront stores and reloads 2 KiB contiguous per candidate tile.
+__device__ __forceinline__ int64_t code_word(int64_t blk, int ct, int wid, int lane)
+{
+    return (((blk * (kSeg / kCT) + ct) * 8 + wid) * 64 + lane) * 8;
+}
+
+// Per-lane query columns of a 128-query tile (shared by the screen and the
+// filter): column, Vecchia limit on the candidate index, own dataset row.
+__device__ __forceinline__ void tile_queries(const ScreenArgs &A, int64_t s0, int qh, int lane,
+                                             int32_t *qcol, int32_t *qlim, int32_t *qself,
+                                             int32_t *qrowv)
+{
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+        qcol[b] = 64 * qh + 32 * b + (lane & 31);
+        const int64_t s = s0 + qcol[b];
+        const bool live = s < A.nq;
+        const int64_t qid = live ? A.qsorted[s] : 0;
+        const int32_t row = live ? A.qrow[s] : 0;
+        qlim[b] = !live ? 0 : (A.pred ? (qid < 0 ? A.n : (qid < A.n ? (int32_t)qid : A.n)) : A.n);
+        qself[b] = live ? row : -1;
+        qrowv[b] = row;
+    }
+}
+
+// candidate range [c_begin, c_end) of this (tile, segment); empty if none
+__device__ __forceinline__ void tile_range(const ScreenArgs &A, int64_t s0, int32_t *c_begin,
+                                           int32_t *c_end)
+{
+    int32_t e = A.n;
+    if (A.pred) {
+        // [0, largest query index) under the mask
+        const int64_t last = s0 + kQT - 1 < A.nq - 1 ? s0 + kQT - 1 : A.nq - 1;
+        const int64_t qmax = A.qsorted[last];
+        e = qmax < 0 ? A.n : (qmax < A.n ? (int32_t)qmax : A.n);
+    }
+    const int32_t b = (int32_t)blockIdx.y * (int32_t)kSeg;
+    *c_begin = b;
+    *c_end = e < b + (int32_t)kSeg ? e : b + (int32_t)kSeg;
+}
+
+// Pass 1.  One workgroup = 128 candidates (MFMA rows) x 128 queries (MFMA
+// columns) per candidate tile, looping over a 2048-candidate segment.  Eight
+// wavefronts: wavefront w owns candidate rows 32*(w&3).. and query columns
+// 64*(w>>2).., i.e. two 32x32 output blocks, each with a hi.hi and a cross
+// accumulator.  Per pair: the certified interval of |rho|; its upper end
+// feeds the query's histogram, its lower end, as a squared-distance lower
+// bound, is stored as a one-byte code for the filter.
 // DD > 0 fixes the coordinate dimension (query coordinates then live in
 // registers); DD == 0 reads them from LDS for any d <= kMaxD.  KT is the
 // kernel (0 RBF, 1 Matern-5/2).  Coordinates arrive scaled by 1/lengthscale.
-template <int PASS, int DD, int KT>
+template <int DD, int KT>
 __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
 {
     __shared__ __attribute__((aligned(16))) uint16_t sAh[kCT * kKP];
@@ -175,73 +214,52 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
     __shared__ __attribute__((aligned(16))) float sCw[kCT];
     __shared__ __attribute__((aligned(16))) float sCn[kCT];
     __shared__ int32_t sQrow[kQT];
-    // pass-1 histogram: two 16-bit bins per word (at most kSeg counts per bin)
-    __shared__ uint32_t sHist[PASS == 1 ? kQT * kBins / 2 : 1];
+    // histogram: two 16-bit bins per word (at most kSeg counts per bin)
+    __shared__ uint32_t sHist[kQT * kBins / 2];
     // coordinates, structure of arrays: candidates [d][kCT], then queries [d][kQT]
     extern __shared__ __attribute__((aligned(16))) float sDyn[];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int cb = wid & 3, qh = wid >> 2;
-    // blockIdx.x = query tile, blockIdx.y = candidate segment; heavy tiles first
-    const int32_t tile = A.n_tiles - 1 - (int32_t)blockIdx.x;
-    const int64_t s0 = (int64_t)tile * kQT;
-    const int32_t n = A.n, mp = A.mp;
+    // blockIdx.x = query tile within the launch, heavy tiles first
+    const int32_t tl = A.n_tiles - 1 - (int32_t)blockIdx.x;
+    const int64_t s0 = (int64_t)(A.tile0 + tl) * kQT;
+    const int64_t blk = (int64_t)tl * A.n_seg + blockIdx.y;
+    const int32_t mp = A.mp;
     const int32_t d = DD > 0 ? DD : A.d;
-    int32_t c_end = n;
-    if (A.pred) {
-        // candidate range of this tile: [0, largest query index) under the mask
-        const int64_t last = s0 + kQT - 1 < A.nq - 1 ? s0 + kQT - 1 : A.nq - 1;
-        const int64_t qmax = A.qsorted[screen_qpos(A, last)];
-        c_end = qmax < 0 ? n : (qmax < n ? (int32_t)qmax : n);
-    }
-    const int32_t c_begin = (int32_t)blockIdx.y * (int32_t)kSeg;
+    int32_t c_begin, c_end;
+    tile_range(A, s0, &c_begin, &c_end);
     if (c_begin >= c_end)
         return;
-    c_end = c_end < c_begin + (int32_t)kSeg ? c_end : c_begin + (int32_t)kSeg;
     float *sCx = sDyn, *sQx = sDyn + d * kCT;
 
     for (int i = tid; i < kQT; i += kScreenThreads) {
         const int64_t s = s0 + i;
-        const int32_t row = s < A.nq ? A.qrow[screen_qpos(A, s)] : 0;
+        const int32_t row = s < A.nq ? A.qrow[s] : 0;
         sQrow[i] = row;
         for (int c = 0; c < d; c++)
             sQx[c * kQT + i] = A.x[(int64_t)row * d + c];
     }
-    if (PASS == 1)
-        for (int i = tid; i < kQT * kBins / 2; i += kScreenThreads)
-            sHist[i] = 0u;
+    for (int i = tid; i < kQT * kBins / 2; i += kScreenThreads)
+        sHist[i] = 0u;
 
-    // this lane's two query columns: mask limit, own row, weight 1/sqrt(p),
-    // norm term, and (pass 2) the emission threshold on the |rho| upper bound
-    int32_t qcol[2], qlim[2], qself[2];
-    float qw[2], qcn[2], qthr[2];
+    // this lane's two query columns: mask limit, weight 1/sqrt(p), norm term
+    int32_t qcol[2], qlim[2], qself[2], qrowv[2];
+    tile_queries(A, s0, qh, lane, qcol, qlim, qself, qrowv);
+    float qw[2], qcn[2];
     float qx[2][DD > 0 ? DD : 1];
 #pragma unroll
     for (int b = 0; b < 2; b++) {
-        qcol[b] = 64 * qh + 32 * b + (lane & 31);
-        const int64_t s = s0 + qcol[b];
-        const bool live = s < A.nq;
-        const int64_t sp = live ? screen_qpos(A, s) : 0;
-        const int64_t qid = live ? A.qsorted[sp] : 0;
-        const int32_t row = live ? A.qrow[sp] : 0;
-        qlim[b] = !live ? 0 : (A.pred ? (qid < 0 ? n : (qid < n ? (int32_t)qid : n)) : n);
-        qself[b] = live ? row : -1;
-        qw[b] = A.isp[row];
-        qcn[b] = A.cD * 1.000002f * A.vnorm[row];
-        qthr[b] = 2.0f;
-        if (PASS == 2 && live) {
-            const float t = A.tau[sp];
-            // |rho| >= 1 - tau is implied by d^2 <= tau; 2^-22 covers rounding
-            qthr[b] = t >= 2.0f ? -1.0f : (1.0f - t) - 2.3841858e-7f;
-        }
+        qw[b] = A.isp[qrowv[b]];
+        qcn[b] = A.cD * 1.000002f * A.vnorm[qrowv[b]];
         if (DD > 0)
 #pragma unroll
             for (int c = 0; c < (DD > 0 ? DD : 1); c++)
-                qx[b][c] = A.x[(int64_t)row * d + c];
+                qx[b][c] = A.x[(int64_t)qrowv[b] * d + c];
     }
     const float eKp = A.eK * 1.000002f;
-    // pass 1 counts a pair when 1 - rlo + 1.5e-7 < kHistCap; screen first on
-    // rlo itself with a slightly lower threshold
+    // a pair counts when 1 - rlo + 1.5e-7 < kHistCap; screen first on rlo
+    // itself with a slightly lower threshold
     const float thr1 = 1.0f + 1.5e-7f - kHistCap - 1e-6f;
 
     const int rbase = 32 * cb + 4 * (lane >> 5);
@@ -249,7 +267,7 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
         __syncthreads(); // the previous tile's epilogue still reads sCx/sCw/sCn
         for (int i = tid; i < kCT; i += kScreenThreads) {
             const int32_t j = c0 + i;
-            const int32_t jr = j < n ? j : 0;
+            const int32_t jr = j < A.n ? j : 0;
             for (int c = 0; c < d; c++)
                 sCx[c * kCT + i] = A.x[(int64_t)jr * d + c];
             sCw[i] = A.isp[jr];
@@ -306,7 +324,8 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
         }
         // epilogue: output register r of block b is candidate row
         // rbase + 8*(r/4) + r%4, query column qcol[b]
-        uint32_t emit[2] = {0u, 0u};
+        const int ct = (c0 - c_begin) / kCT;
+        uint32_t words[8];
 #pragma unroll
         for (int g = 0; g < 4; g++) {
             const int rg = rbase + 8 * g;
@@ -348,7 +367,8 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
                 }
             }
 #pragma unroll
-            for (int b = 0; b < 2; b++)
+            for (int b = 0; b < 2; b++) {
+                uint32_t word = 0u;
 #pragma unroll
                 for (int rr = 0; rr < 4; rr++) {
                     const int r = 4 * g + rr;
@@ -365,108 +385,153 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
                     // for the roundings of num, w and ar
                     const float wE = w * __builtin_fmaf(qcn[b], cnv, eKp);
                     const bool valid = j < qlim[b];
-                    if (PASS == 1) {
-                        // the pair's own row may be counted at its computed
-                        // bound or not at all: tau stays an upper bound
-                        const float rlo = __builtin_fmaf(ar, 1.0f - 7.2e-7f, -wE);
-                        if (valid && rlo > thr1) {
-                            // squared upper bound 1 - (|rho| lower bound), rounded up
-                            const float ub2 = (1.0f - rlo) + 1.5e-7f;
-                            if (ub2 < kHistCap) {
-                                const int bin = (int)(ub2 * (float)kBins);
-                                atomicAdd(&sHist[(qcol[b] * kBins + bin) >> 1],
-                                          (bin & 1) ? 65536u : 1u);
-                            }
+                    // histogram of the squared upper bound 1 - (|rho| lower
+                    // bound); the pair's own row may be counted at its
+                    // computed bound or not at all, tau stays an upper bound
+                    const float rlo = __builtin_fmaf(ar, 1.0f - 7.2e-7f, -wE);
+                    if (valid && rlo > thr1) {
+                        const float ub2 = (1.0f - rlo) + 1.5e-7f;
+                        if (ub2 < kHistCap) {
+                            const int bin = (int)(ub2 * (float)kBins);
+                            atomicAdd(&sHist[(qcol[b] * kBins + bin) >> 1],
+                                      (bin & 1) ? 65536u : 1u);
                         }
-                    } else {
-                        const float rhi = __builtin_fmaf(ar, 1.0f + 7.2e-7f, wE);
-                        if (valid && rhi >= qthr[b])
-                            emit[b] |= 1u << r;
                     }
+                    // code = floor(256 * squared lower bound), 255 if invalid
+                    const float rhi = __builtin_fmaf(ar, 1.0f + 7.2e-7f, wE);
+                    const float lb2 = (1.0f - rhi) - 1.5e-7f;
+                    const float cf = __builtin_fminf(__builtin_fmaxf(lb2 * 256.0f, 0.0f), 255.0f);
+                    const uint32_t code = valid ? (uint32_t)cf : 255u;
+                    word |= code << (8 * rr);
                 }
-        }
-        if (PASS == 2) {
-            // one atomic per query column and wavefront: lanes l and l + 32
-            // share a column and split the reserved range
-#pragma unroll
-            for (int b = 0; b < 2; b++) {
-                // the query's own row (distance 0) is always listed
-                const int32_t dr = qself[b] - c0 - rbase;
-                if (dr >= 0 && dr < 28 && (dr & 7) < 4 && qself[b] < qlim[b])
-                    emit[b] |= 1u << (4 * (dr >> 3) + (dr & 3));
-                const int c_me = __builtin_popcount(emit[b]);
-                const int c_other = __shfl_xor(c_me, 32, 64);
-                const int64_t s = s0 + qcol[b];
-                int base = 0;
-                if (lane < 32 && c_me + c_other > 0)
-                    base = atomicAdd(&A.cnt[s], c_me + c_other);
-                base = __shfl(base, lane & 31, 64);
-                int p = base + (lane < 32 ? 0 : c_other);
-                uint32_t mk = emit[b];
-                const int32_t cap = A.qmap ? A.cap[s] : kCap;
-                int32_t *dst = A.qmap ? A.list + (A.off[s] - A.off_base) : A.list + s * kCap;
-                while (mk) {
-                    const int r = __builtin_ctz(mk);
-                    mk &= mk - 1;
-                    if (p < cap)
-                        dst[p] = c0 + rbase + 8 * (r >> 2) + (r & 3);
-                    p++;
-                }
+                words[2 * g + b] = word;
             }
         }
+        uint4 *cdst = (uint4 *)(A.codes + code_word(blk, ct, wid, lane));
+        cdst[0] = make_uint4(words[0], words[1], words[2], words[3]);
+        cdst[1] = make_uint4(words[4], words[5], words[6], words[7]);
     }
-    if (PASS == 1) {
-        __syncthreads();
-        for (int i = tid; i < kQT * kBins / 2; i += kScreenThreads) {
-            const int64_t s = s0 + (2 * i) / kBins;
-            const uint32_t h = sHist[i];
-            if (s < A.nq && h) {
-                const int bin = (2 * i) % kBins;
-                if (h & 0xffffu)
-                    atomicAdd(&A.hist[s * kBins + bin], (int32_t)(h & 0xffffu));
-                if (h >> 16)
-                    atomicAdd(&A.hist[s * kBins + bin + 1], (int32_t)(h >> 16));
+    __syncthreads();
+    for (int i = tid; i < kQT * kBins / 2; i += kScreenThreads) {
+        const int64_t s = s0 + (2 * i) / kBins;
+        const uint32_t h = sHist[i];
+        if (s < A.nq && h) {
+            const int bin = (2 * i) % kBins;
+            if (h & 0xffffu)
+                atomicAdd(&A.hist[s * kBins + bin], (int32_t)(h & 0xffffu));
+            if (h >> 16)
+                atomicAdd(&A.hist[s * kBins + bin + 1], (int32_t)(h >> 16));
+        }
+    }
+}
+
+// Pass 2: the same (tile, segment) grid and lane layout reads the codes back
+// and lists every valid candidate whose code is at most floor(256 tau_q)
+// (every pair with squared lower bound <= tau_q among them), plus the query's
+// own row.  One atomic per query column and wavefront: lanes l and l + 32
+// share a column and split the reserved range.
+__global__ void __launch_bounds__(kScreenThreads) filter_kernel(ScreenArgs A)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int cb = wid & 3, qh = wid >> 2;
+    const int32_t tl = A.n_tiles - 1 - (int32_t)blockIdx.x;
+    const int64_t s0 = (int64_t)(A.tile0 + tl) * kQT;
+    const int64_t blk = (int64_t)tl * A.n_seg + blockIdx.y;
+    int32_t c_begin, c_end;
+    tile_range(A, s0, &c_begin, &c_end);
+    if (c_begin >= c_end)
+        return;
+    int32_t qcol[2], qlim[2], qself[2], qrowv[2];
+    tile_queries(A, s0, qh, lane, qcol, qlim, qself, qrowv);
+    uint32_t qt[2];
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+        const int64_t s = s0 + qcol[b];
+        const float t = s < A.nq ? A.tau[s] : 0.0f;
+        qt[b] = t >= 2.0f ? 255u : (uint32_t)(t * 256.0f);
+    }
+    const int rbase = 32 * cb + 4 * (lane >> 5);
+    for (int32_t c0 = c_begin; c0 < c_end; c0 += kCT) {
+        const int ct = (c0 - c_begin) / kCT;
+        uint32_t emit[2] = {0u, 0u};
+        const uint4 *csrc = (const uint4 *)(A.codes + code_word(blk, ct, wid, lane));
+        const uint4 w0 = csrc[0], w1 = csrc[1];
+        const uint32_t words[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+        for (int g = 0; g < 4; g++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const uint32_t word = words[2 * g + b];
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) {
+                    const int32_t j = c0 + rbase + 8 * g + rr;
+                    const uint32_t code = (word >> (8 * rr)) & 255u;
+                    if (j < qlim[b] && code <= qt[b])
+                        emit[b] |= 1u << (4 * g + rr);
+                }
+            }
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            // the query's own row (distance 0) is always listed
+            const int32_t dr = qself[b] - c0 - rbase;
+            if (dr >= 0 && dr < 28 && (dr & 7) < 4 && qself[b] < qlim[b])
+                emit[b] |= 1u << (4 * (dr >> 3) + (dr & 3));
+            const int c_me = __builtin_popcount(emit[b]);
+            const int c_other = __shfl_xor(c_me, 32, 64);
+            const int64_t s = s0 + qcol[b];
+            int base = 0;
+            if (lane < 32 && c_me + c_other > 0)
+                base = atomicAdd(&A.cnt[s], c_me + c_other);
+            base = __shfl(base, lane & 31, 64);
+            int p = base + (lane < 32 ? 0 : c_other);
+            uint32_t mk = emit[b];
+            int32_t *dst = A.list + s * kCap;
+            while (mk) {
+                const int r = __builtin_ctz(mk);
+                mk &= mk - 1;
+                if (p < kCap)
+                    dst[p] = c0 + rbase + 8 * (r >> 2) + (r & 3);
+                p++;
             }
         }
     }
 }
 
-template <int PASS, int KT>
-static void launch_screen_kt(const ScreenArgs &A, unsigned n_seg, size_t lds, hipStream_t st)
+template <int KT> static void launch_screen_kt(const ScreenArgs &A, size_t lds, hipStream_t st)
 {
-    const dim3 grid((unsigned)A.n_tiles, n_seg);
+    const dim3 grid((unsigned)A.n_tiles, (unsigned)A.n_seg);
     switch (A.d) {
     case 2:
-        screen_kernel<PASS, 2, KT><<<grid, kScreenThreads, lds, st>>>(A);
+        screen_kernel<2, KT><<<grid, kScreenThreads, lds, st>>>(A);
         break;
     case 3:
-        screen_kernel<PASS, 3, KT><<<grid, kScreenThreads, lds, st>>>(A);
+        screen_kernel<3, KT><<<grid, kScreenThreads, lds, st>>>(A);
         break;
     case 5:
-        screen_kernel<PASS, 5, KT><<<grid, kScreenThreads, lds, st>>>(A);
+        screen_kernel<5, KT><<<grid, kScreenThreads, lds, st>>>(A);
         break;
     default:
-        screen_kernel<PASS, 0, KT><<<grid, kScreenThreads, lds, st>>>(A);
+        screen_kernel<0, KT><<<grid, kScreenThreads, lds, st>>>(A);
     }
     CX_HIP(hipGetLastError());
 }
 
-template <int PASS> static void launch_screen(const ScreenArgs &A, unsigned n_seg, size_t lds,
-                                              hipStream_t st)
+static void launch_screen(const ScreenArgs &A, size_t lds, hipStream_t st)
 {
     if (A.kernel == 0)
-        launch_screen_kt<PASS, 0>(A, n_seg, lds, st);
+        launch_screen_kt<0>(A, lds, st);
     else
-        launch_screen_kt<PASS, 1>(A, n_seg, lds, st);
+        launch_screen_kt<1>(A, lds, st);
 }
 
 // tau_q (squared scale): upper edge of the first bin where the cumulative count
 // of squared upper bounds reaches k; 2.0 (keep everything) when fewer than k
 // candidates have a squared upper bound below the histogram cap.
-__global__ void tau_kernel(const int32_t *__restrict__ hist, int64_t nq, int32_t k, float *tau)
+__global__ void tau_kernel(const int32_t *__restrict__ hist, int64_t s_begin, int64_t s_end,
+                           int32_t k, float *tau)
 {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nq)
+    const int64_t s = s_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= s_end)
         return;
     int cum = 0;
     float t = 2.0f;
@@ -485,6 +550,7 @@ __global__ void tau_kernel(const int32_t *__restrict__ hist, int64_t nq, int32_t
 constexpr int kRsThreads = 256;                 // one workgroup per query
 constexpr int kRsSort = 2048;                   // running top-k + one round, padded
 constexpr int kRsChunk = kRsSort - kMixMaxK;    // listed candidates per sort round
+constexpr int kHMaxM = 512;                     // query rows staged in LDS up to this rank
 
 struct HardEntry {
     float dh, dl, err;
@@ -499,20 +565,8 @@ struct RescoreArgs {
     const int64_t *qsorted;
     const int64_t *qperm; // sorted position -> original query position
     const int32_t *qrow;
-    const int32_t *cnt;   // listed candidates per sorted query (or per hard query)
-    const int32_t *list;  // kCap slots per sorted query (or the hard pool)
-    // hard queries (pool != 0): hard[h0 + i] is the sorted position of item i,
-    // its list is list[off[h0 + i] - off_base ..] with cap[h0 + i] entries
-    int32_t h0, hn;
-    const int32_t *hard, *cap;
-    const int64_t *off;
-    int64_t off_base;
-    // slices of kRsChunk: hard query h owns global slices [chunk_off[h],
-    // chunk_off[h + 1]); this batch holds n_chunks of them from chunk_base
-    const int64_t *chunk_off;
-    int64_t chunk_base, n_chunks;
-    HardEntry *part;      // n_chunks x k slice top-k lists
-    int32_t *mismatch;    // counts pool lists whose size differs from the pass-2 count
+    const int32_t *cnt;   // listed candidates per sorted query
+    const int32_t *list;  // kCap slots per sorted query
     int64_t n, nq;
     int32_t d, m, kernel, pred, k;
     float var;
@@ -623,7 +677,7 @@ __device__ __forceinline__ void group_dot(const float *__restrict__ vq,
     float s = 0.0f, c = 0.0f;
     sabs = 0.0f;
     if ((m & 3) == 0) {
-#pragma unroll 2
+#pragma unroll 4
         for (int32_t i = 4 * sub; i < m; i += 32) {
             const float4 a = *(const float4 *)(vq + i);
             const float4 b = *(const float4 *)(vj + i);
@@ -650,12 +704,11 @@ __device__ __forceinline__ void group_dot(const float *__restrict__ vq,
 // running top-k at [0, k): eight lanes per candidate for the dot (results
 // parked in the slots), then one lane per candidate for the distance.
 __device__ void score_slice(const RescoreArgs &A, const int32_t *__restrict__ lst, int32_t n_in,
-                            int32_t qrow, int64_t qt, float *sh, float *sl, float *se,
-                            int32_t *sj)
+                            int32_t qrow, int64_t qt, const float *vq, float *sh, float *sl,
+                            float *se, int32_t *sj)
 {
     const int tid = threadIdx.x, grp = tid >> 3, sub = tid & 7;
     const int32_t k = A.k, m = A.m;
-    const float *vq = A.v + qt * m;
     __syncthreads();
     for (int32_t e = grp; e < n_in; e += kRsThreads / 8) {
         const int32_t j = lst[e];
@@ -714,91 +767,281 @@ __device__ __forceinline__ void write_row(const RescoreArgs &A, int64_t pos_out,
     }
 }
 
-// MODE 0: one workgroup per query, its kCap-slot list scored in rounds of
-// kRsChunk, output row written.  MODE 1: one workgroup per kRsChunk slice of a
-// hard query's pool list, the slice's top-k written to A.part.
-template <int MODE> __global__ void __launch_bounds__(kRsThreads) rescore_kernel(RescoreArgs A)
+// One workgroup per query, its kCap-slot list scored in rounds of kRsChunk,
+// output row written.  Queries whose list overflowed are left to the hard
+// kernels.
+__global__ void __launch_bounds__(kRsThreads) rescore_kernel(RescoreArgs A)
 {
     __shared__ float sh[kRsSort], sl[kRsSort], se[kRsSort];
     __shared__ int32_t sj[kRsSort];
     __shared__ int64_t s_item;
-    const int32_t k = A.k;
-    const int64_t n_items = MODE ? A.n_chunks : A.nq;
+    __shared__ __attribute__((aligned(16))) float s_vq[kHMaxM];
+    const int32_t m = A.m;
     for (;;) {
         if (threadIdx.x == 0)
             s_item = (int64_t)atomicAdd((unsigned long long *)A.counter, 1ull);
         __syncthreads();
+        const int64_t s = s_item;
+        __syncthreads();
+        if (s >= A.nq)
+            break;
+        const int32_t count = A.cnt[s];
+        if (count > kCap)
+            continue; // scored over all its candidates by the hard kernels
+        const int32_t qrow = A.qrow[s];
+        const int64_t qt = A.inv_order[qrow];
+        // the query row in LDS when it fits (score_slice syncs before use)
+        const float *vq = A.v + qt * m;
+        if (m <= kHMaxM) {
+            for (int c = threadIdx.x; c < m; c += kRsThreads)
+                s_vq[c] = vq[c];
+            vq = s_vq;
+        }
+        clear_entries(sh, sl, se, sj);
+        for (int32_t c0 = 0; c0 < count; c0 += kRsChunk) {
+            const int32_t n_in = count - c0 < kRsChunk ? count - c0 : kRsChunk;
+            score_slice(A, A.list + s * kCap + c0, n_in, qrow, qt, vq, sh, sl, se, sj);
+        }
+        write_row(A, A.qperm[s], sh, sl, se, sj);
+        __syncthreads();
+    }
+}
+
+// ---- hard queries ------------------------------------------------------------
+// A query whose list overflowed kCap (a wide screen interval, e.g. tiny p) is
+// scored over all its valid candidates.  One workgroup = kHQ consecutive hard
+// queries x a kHSeg-candidate segment: every candidate row is read once for
+// all kHQ queries (eight lanes per candidate, kHQ compensated dots per lane
+// group), distances are finished one lane per pair, and each query keeps a
+// running top-k in LDS.  A merge kernel combines the segments.
+
+constexpr int kHQ = 4;        // hard queries per workgroup
+constexpr int kHSeg = 1024;   // candidates per workgroup
+constexpr int kHRound = 128;  // candidates per sort round
+constexpr int kHSort = 512;   // running top-k (k <= 256) + one round, padded
+
+struct HardArgs {
+    RescoreArgs r;
+    const int32_t *hard;  // sorted positions of the hard queries
+    int32_t n_hard;
+    int32_t g0, gn;       // this batch: query groups [g0, g0 + gn)
+    int32_t nseg;         // segments per group (ceil(n / kHSeg))
+    HardEntry *part;      // gn x nseg x kHQ x k segment top-k lists
+    int64_t *counter;
+};
+
+// ascending bitonic sort of the first `size` entries of each of `na` arrays
+// laid out [a][kHSort]
+__device__ void sort_arrays(float *h, float *l, float *e, int32_t *jv, int na, int size_total)
+{
+    const int half = size_total / 2;
+    for (int size = 2; size <= size_total; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (int t = threadIdx.x; t < na * half; t += blockDim.x) {
+                const int a = t / half, tt = t - a * half;
+                const int lo = a * kHSort + 2 * tt - (tt & (stride - 1)), hi = lo + stride;
+                const bool up = ((lo - a * kHSort) & size) == 0;
+                const bool sw = up ? entry_lt(h[hi], l[hi], jv[hi], h[lo], l[lo], jv[lo])
+                                   : entry_lt(h[lo], l[lo], jv[lo], h[hi], l[hi], jv[hi]);
+                if (sw) {
+                    float x = h[lo]; h[lo] = h[hi]; h[hi] = x;
+                    x = l[lo]; l[lo] = l[hi]; l[hi] = x;
+                    x = e[lo]; e[lo] = e[hi]; e[hi] = x;
+                    const int32_t y = jv[lo]; jv[lo] = jv[hi]; jv[hi] = y;
+                }
+            }
+        }
+    __syncthreads();
+}
+
+__device__ __forceinline__ int64_t hard_limit(const RescoreArgs &A, int64_t s)
+{
+    const int64_t qid = A.qsorted[s];
+    // Vecchia bound on the original index (negative -> unbounded)
+    return A.pred ? (qid < 0 ? A.n : (qid < A.n ? qid : A.n)) : A.n;
+}
+
+__global__ void __launch_bounds__(kRsThreads) hard_tile_kernel(HardArgs H)
+{
+    __shared__ float sh[kHQ * kHSort], sl[kHQ * kHSort], se[kHQ * kHSort];
+    __shared__ int32_t sj[kHQ * kHSort];
+    __shared__ int64_t s_item;
+    __shared__ int64_t s_lim[kHQ], s_qt[kHQ];
+    __shared__ int32_t s_qrow[kHQ];
+    __shared__ __attribute__((aligned(16))) float s_vq[kHQ * kHMaxM];
+    const RescoreArgs &A = H.r;
+    const int tid = threadIdx.x, grp = tid >> 3, sub = tid & 7;
+    const int32_t k = A.k, m = A.m;
+    // float4-aligned rows that fit LDS are staged
+    const bool staged = (m & 3) == 0 && m <= kHMaxM;
+    for (;;) {
+        if (tid == 0)
+            s_item = (int64_t)atomicAdd((unsigned long long *)H.counter, 1ull);
+        __syncthreads();
         const int64_t item = s_item;
         __syncthreads();
-        if (item >= n_items)
+        if (item >= (int64_t)H.gn * H.nseg)
             break;
-        if (MODE == 0) {
-            const int64_t s = item;
-            const int32_t count = A.cnt[s];
-            if (count > kCap)
-                continue; // listed again, in full, in the hard pool
-            const int32_t qrow = A.qrow[s];
-            const int64_t qt = A.inv_order[qrow];
-            clear_entries(sh, sl, se, sj);
-            for (int32_t c0 = 0; c0 < count; c0 += kRsChunk) {
-                const int32_t n_in = count - c0 < kRsChunk ? count - c0 : kRsChunk;
-                score_slice(A, A.list + s * kCap + c0, n_in, qrow, qt, sh, sl, se, sj);
+        const int32_t gl = (int32_t)(item / H.nseg), seg = (int32_t)(item % H.nseg);
+        const int32_t h0 = (H.g0 + gl) * kHQ;
+        const int nq_g = H.n_hard - h0 < kHQ ? H.n_hard - h0 : kHQ;
+        if (tid < kHQ) {
+            int64_t lim = 0, qtv = 0;
+            int32_t row = -1;
+            if (tid < nq_g) {
+                const int64_t s = H.hard[h0 + tid];
+                lim = hard_limit(A, s);
+                row = A.qrow[s];
+                qtv = A.inv_order[row];
             }
-            write_row(A, A.qperm[s], sh, sl, se, sj);
-        } else {
-            // chunk -> hard query: last h in [h0, h0 + hn) with chunk_off[h] <= item
-            const int64_t g = A.chunk_base + item;
-            int32_t lo = A.h0, hi = A.h0 + A.hn - 1;
-            while (lo < hi) {
-                const int32_t mid = (lo + hi + 1) >> 1;
-                if (A.chunk_off[mid] <= g)
-                    lo = mid;
-                else
-                    hi = mid - 1;
+            s_lim[tid] = lim;
+            s_qt[tid] = qtv;
+            s_qrow[tid] = row;
+        }
+        __syncthreads();
+        int64_t qt[kHQ];
+        int64_t lim_max = 0;
+#pragma unroll
+        for (int i = 0; i < kHQ; i++) {
+            qt[i] = s_qt[i];
+            lim_max = s_lim[i] > lim_max ? s_lim[i] : lim_max;
+        }
+        if (staged)
+            for (int t = tid; t < kHQ * m; t += kRsThreads) {
+                const int i = t / m, col = t - i * m;
+                s_vq[i * kHMaxM + col] = A.v[s_qt[i] * m + col];
             }
-            const int32_t h = lo;
-            const int64_t s = A.hard[h];
-            const int32_t got = A.cnt[h - A.h0], want = A.cap[h];
-            const int32_t total = got < want ? got : want;
-            const int64_t c0 = (g - A.chunk_off[h]) * kRsChunk;
-            if (c0 == 0 && got != want && threadIdx.x == 0)
-                atomicAdd(A.mismatch, 1);
-            const int32_t n_in =
-                c0 >= total ? 0 : (total - c0 < kRsChunk ? (int32_t)(total - c0) : kRsChunk);
-            const int32_t qrow = A.qrow[s];
-            const int64_t qt = A.inv_order[qrow];
-            clear_entries(sh, sl, se, sj);
-            if (n_in > 0)
-                score_slice(A, A.list + (A.off[h] - A.off_base) + c0, n_in, qrow, qt, sh, sl,
-                            se, sj);
-            HardEntry *out = A.part + item * k;
-            for (int t = threadIdx.x; t < k; t += kRsThreads)
-                out[t] = HardEntry{sh[t], sl[t], se[t], sj[t]};
+        const int64_t beg = (int64_t)seg * kHSeg;
+        if (beg >= lim_max)
+            continue; // no query of the group has candidates here
+        const int64_t end = beg + kHSeg < lim_max ? beg + kHSeg : lim_max;
+        for (int t = tid; t < kHQ * kHSort; t += kRsThreads) {
+            sh[t] = __builtin_huge_valf();
+            sl[t] = 0.0f;
+            se[t] = 0.0f;
+            sj[t] = 0x7fffffff;
+        }
+        for (int64_t c0 = beg; c0 < end; c0 += kHRound) {
+            const int n_in = end - c0 < kHRound ? (int)(end - c0) : kHRound;
+            __syncthreads();
+            // phase A: eight lanes per candidate, kHQ dots each
+            for (int e = grp; e < n_in; e += kRsThreads / 8) {
+                const int64_t j = c0 + e;
+                const float *vj = A.v + A.inv_order[j] * m;
+                float s_[kHQ], c_[kHQ], sa[kHQ];
+#pragma unroll
+                for (int i = 0; i < kHQ; i++) {
+                    s_[i] = 0.0f;
+                    c_[i] = 0.0f;
+                    sa[i] = 0.0f;
+                }
+                if (staged) {
+                    // query rows in LDS, candidate row in float4 pieces
+#pragma unroll 2
+                    for (int32_t col = 4 * sub; col < m; col += 32) {
+                        const float4 b = *(const float4 *)(vj + col);
+#pragma unroll
+                        for (int i = 0; i < kHQ; i++) {
+                            const float4 a = *(const float4 *)(s_vq + i * kHMaxM + col);
+                            dot2_step(a.x, b.x, s_[i], c_[i], sa[i]);
+                            dot2_step(a.y, b.y, s_[i], c_[i], sa[i]);
+                            dot2_step(a.z, b.z, s_[i], c_[i], sa[i]);
+                            dot2_step(a.w, b.w, s_[i], c_[i], sa[i]);
+                        }
+                    }
+                } else {
+                    for (int32_t col = sub; col < m; col += 8) {
+                        const float b = vj[col];
+#pragma unroll
+                        for (int i = 0; i < kHQ; i++)
+                            dot2_step(A.v[qt[i] * m + col], b, s_[i], c_[i], sa[i]);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < kHQ; i++) {
+                    df dot = two_sum(s_[i], c_[i]);
+                    float sab = sa[i];
+#pragma unroll
+                    for (int o = 1; o < 8; o <<= 1) {
+                        const df other = {__shfl_xor(dot.hi, o, 64), __shfl_xor(dot.lo, o, 64)};
+                        dot = df_add(dot, other);
+                        sab += __shfl_xor(sab, o, 64);
+                    }
+                    if (sub == 0) {
+                        const int slot = i * kHSort + k + e;
+                        sh[slot] = dot.hi;
+                        sl[slot] = dot.lo;
+                        se[slot] = sab;
+                        sj[slot] = (int32_t)j;
+                    }
+                }
+            }
+            __syncthreads();
+            // phase B: one lane per (query, candidate) pair
+            for (int t = tid; t < kHQ * n_in; t += kRsThreads) {
+                const int i = t / n_in, e = t - i * n_in;
+                const int slot = i * kHSort + k + e;
+                const int32_t j = sj[slot];
+                float dh = __builtin_huge_valf(), dl = 0.0f, er = 0.0f;
+                int32_t jj = 0x7fffffff;
+                if (i < nq_g && (int64_t)j < s_lim[i]) {
+                    jj = j;
+                    if (j == s_qrow[i]) {
+                        dh = 0.0f;
+                    } else {
+                        const df dd = finish_distance(A, s_qt[i], A.inv_order[j],
+                                                      df{sh[slot], sl[slot]}, se[slot], &er);
+                        dh = dd.hi;
+                        dl = dd.lo;
+                    }
+                }
+                sh[slot] = dh;
+                sl[slot] = dl;
+                se[slot] = er;
+                sj[slot] = jj;
+            }
+            int p2 = 2;
+            while (p2 < k + n_in)
+                p2 <<= 1;
+            // entries past k left by an earlier round rank behind the current
+            // top-k, so they never re-enter it
+            sort_arrays(sh, sl, se, sj, kHQ, p2);
+        }
+        HardEntry *out = H.part + ((int64_t)gl * H.nseg + seg) * kHQ * k;
+        for (int t = tid; t < kHQ * k; t += kRsThreads) {
+            const int i = t / k, r = t - i * k;
+            const int slot = i * kHSort + r;
+            out[t] = HardEntry{sh[slot], sl[slot], se[slot], sj[slot]};
         }
         __syncthreads();
     }
 }
 
-// Merge the per-slice top-k lists of each hard query and write its row.
-__global__ void __launch_bounds__(kRsThreads) hard_merge_kernel(RescoreArgs A)
+// Merge the segment lists of each hard query of the batch; write its row.
+__global__ void __launch_bounds__(kRsThreads) hard_merge_kernel(HardArgs H)
 {
     __shared__ float sh[kRsSort], sl[kRsSort], se[kRsSort];
     __shared__ int32_t sj[kRsSort];
+    const RescoreArgs &A = H.r;
     const int32_t k = A.k;
     const int32_t per = (kRsSort - k) / k; // lists merged per round
-    for (int32_t i = blockIdx.x; i < A.hn; i += gridDim.x) {
-        const int32_t h = A.h0 + i;
-        const int64_t first = A.chunk_off[h] - A.chunk_base;
-        const int32_t nl_all =
-            (int32_t)((h + 1 < A.h0 + A.hn ? A.chunk_off[h + 1] : A.chunk_base + A.n_chunks) -
-                      A.chunk_off[h]);
+    const int32_t h_begin = H.g0 * kHQ;
+    const int32_t h_end = (H.g0 + H.gn) * kHQ < H.n_hard ? (H.g0 + H.gn) * kHQ : H.n_hard;
+    for (int32_t h = h_begin + blockIdx.x; h < h_end; h += gridDim.x) {
+        const int64_t s = H.hard[h];
+        const int64_t limit = hard_limit(A, s);
+        const int32_t nl_all = (int32_t)((limit + kHSeg - 1) / kHSeg);
+        const int32_t gl = h / kHQ - H.g0, i = h % kHQ;
         __syncthreads();
         clear_entries(sh, sl, se, sj);
         for (int32_t g = 0; g < nl_all; g += per) {
             const int32_t nl = nl_all - g < per ? nl_all - g : per;
             __syncthreads();
             for (int t = threadIdx.x; t < nl * k; t += kRsThreads) {
-                const HardEntry e = A.part[(first + g + t / k) * k + t % k];
+                const int32_t seg = g + t / k;
+                const HardEntry e =
+                    H.part[(((int64_t)gl * H.nseg + seg) * kHQ + i) * k + t % k];
                 sh[k + t] = e.dh;
                 sl[k + t] = e.dl;
                 se[k + t] = e.err;
@@ -809,15 +1052,9 @@ __global__ void __launch_bounds__(kRsThreads) hard_merge_kernel(RescoreArgs A)
                 p2 <<= 1;
             sort_entries(sh, sl, se, sj, p2);
         }
-        write_row(A, A.qperm[A.hard[h]], sh, sl, se, sj);
+        write_row(A, A.qperm[s], sh, sl, se, sj);
     }
 }
-
-// ---- hard queries ------------------------------------------------------------
-// A query whose list overflowed kCap (a wide screen interval, e.g. tiny p) is
-// listed again in full: its pass-2 count sizes a slice of a pool, the same
-// pass-2 screen re-runs over tiles of the hard queries only and writes
-// there, and rescore_kernel scores the slices.
 
 __global__ void hard_flag_kernel(const int32_t *__restrict__ cnt, int64_t nq, int32_t *flag)
 {
@@ -826,39 +1063,18 @@ __global__ void hard_flag_kernel(const int32_t *__restrict__ cnt, int64_t nq, in
         flag[s] = cnt[s] > kCap ? 1 : 0;
 }
 
-// ordered compaction: hard[pos[s]] = s, its pass-2 count and kRsChunk
-// slices; the number of hard queries goes to n_hard
-__global__ void hard_scatter_kernel(const int32_t *__restrict__ cnt,
-                                    const int32_t *__restrict__ flag,
+// ordered compaction: hard[pos[s]] = s; the count goes to n_hard
+__global__ void hard_scatter_kernel(const int32_t *__restrict__ flag,
                                     const int32_t *__restrict__ pos, int64_t nq, int32_t *hard,
-                                    int32_t *hard_cnt, int64_t *hard_cnt64, int64_t *hard_chunks,
                                     int32_t *n_hard)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nq)
         return;
-    if (flag[s]) {
+    if (flag[s])
         hard[pos[s]] = (int32_t)s;
-        hard_cnt[pos[s]] = cnt[s];
-        hard_cnt64[pos[s]] = cnt[s];
-        hard_chunks[pos[s]] = (cnt[s] + kRsChunk - 1) / kRsChunk;
-    }
     if (s == nq - 1)
         *n_hard = pos[s] + flag[s];
-}
-
-// totals = {hard queries, pool entries, slices} (the scans run over all nq
-// positions; those past the hard count hold zeros)
-__global__ void hard_totals_kernel(const int32_t *__restrict__ n_hard,
-                                   const int64_t *__restrict__ off,
-                                   const int64_t *__restrict__ cnt64,
-                                   const int64_t *__restrict__ choff,
-                                   const int64_t *__restrict__ chunks, int64_t *totals)
-{
-    const int32_t h = *n_hard;
-    totals[0] = h;
-    totals[1] = h > 0 ? off[h - 1] + cnt64[h - 1] : 0;
-    totals[2] = h > 0 ? choff[h - 1] + chunks[h - 1] : 0;
 }
 
 // ---- driver -----------------------------------------------------------------
@@ -911,18 +1127,10 @@ static void prepare_mixed(TreeState &t)
     t.mixed_ready = true;
 }
 
-// read up to eight device words through the tree's pinned host buffer
-// (synchronises the tree's stream)
-template <typename T> static void read_words(TreeState &t, const T *dptr, int count, T *out)
-{
-    CX_REQUIRE(count >= 1 && (size_t)count * sizeof(T) <= 64, "read_words: at most 64 bytes");
-    if (!t.host_word)
-        CX_HIP(hipHostMalloc((void **)&t.host_word, 64, hipHostMallocDefault));
-    CX_HIP(hipMemcpyAsync(t.host_word, dptr, sizeof(T) * count, hipMemcpyDeviceToHost,
-                          t.stream));
-    CX_HIP(hipStreamSynchronize(t.stream));
-    std::memcpy(out, t.host_word, sizeof(T) * count);
-}
+// pair codes kept per launch group of query tiles (bytes)
+constexpr int64_t kCodeBudget = (int64_t)4 << 30;
+// segment top-k lists of the hard queries per launch batch (bytes)
+constexpr int64_t kPartBudget = (int64_t)512 << 20;
 
 void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, int32_t predecessor,
                int64_t *d_idx, float *d_dist, float *d_err)
@@ -939,8 +1147,8 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     hipStream_t st = t.stream;
     const int64_t n = t.n;
 
-    // workspace: sorted queries, permutation, rows, tau, counts, lists, and
-    // the hard-query bookkeeping
+    // workspace: sorted queries, permutation, rows, tau, counts, histograms,
+    // lists, and the hard-query bookkeeping
     auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -960,22 +1168,13 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     const size_t o_flag = take(sizeof(int32_t) * nq);
     const size_t o_pos = take(sizeof(int32_t) * nq);
     const size_t o_hard = take(sizeof(int32_t) * nq);
-    const size_t o_hcnt = take(sizeof(int32_t) * nq);
-    const size_t o_hcnt64 = take(sizeof(int64_t) * nq);
-    const size_t o_hoff = take(sizeof(int64_t) * nq);
-    const size_t o_cnt3 = take(sizeof(int32_t) * nq);
-    const size_t o_hch = take(sizeof(int64_t) * nq);
-    const size_t o_choff = take(sizeof(int64_t) * nq);
-    size_t sort_bytes = 0, scan32_bytes = 0, scan64_bytes = 0;
+    size_t sort_bytes = 0, scan_bytes = 0;
     CX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int64_t *)nullptr,
                                               (int64_t *)nullptr, (const int64_t *)nullptr,
                                               (int64_t *)nullptr, (int)nq, 0, 64, st));
-    CX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan32_bytes, (const int32_t *)nullptr,
+    CX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t *)nullptr,
                                             (int32_t *)nullptr, (int)nq, st));
-    CX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan64_bytes, (const int64_t *)nullptr,
-                                            (int64_t *)nullptr, (int)nq, st));
-    const size_t tmp_bytes = std::max(sort_bytes, std::max(scan32_bytes, scan64_bytes));
-    const size_t o_tmp = take(tmp_bytes);
+    const size_t o_tmp = take(std::max(sort_bytes, scan_bytes));
     uint8_t *ws = workspace(t, off);
     int64_t *qs = (int64_t *)(ws + o_qs), *perm = (int64_t *)(ws + o_perm);
     int64_t *iota = (int64_t *)(ws + o_iota);
@@ -984,16 +1183,11 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     float *tau = (float *)(ws + o_tau);
     int32_t *hist = (int32_t *)(ws + o_hist);
     int32_t *flag = (int32_t *)(ws + o_flag), *pos = (int32_t *)(ws + o_pos);
-    int32_t *hard = (int32_t *)(ws + o_hard), *hcnt = (int32_t *)(ws + o_hcnt);
-    int64_t *hcnt64 = (int64_t *)(ws + o_hcnt64), *hoff = (int64_t *)(ws + o_hoff);
-    int32_t *cnt3 = (int32_t *)(ws + o_cnt3);
-    int64_t *hch = (int64_t *)(ws + o_hch), *choff = (int64_t *)(ws + o_choff);
-    // counters: ctr[0..1] rescore items (int64), ctr[2..3] hard slice items
-    // (int64), ctr[4] hard count, ctr[5] pool-size mismatches, ctr[8..13]
-    // hard count, pool entries and slices over all hard queries (int64)
+    int32_t *hard = (int32_t *)(ws + o_hard);
+    // counters: ctr[0..1] rescore items (int64), ctr[2..3] hard items (int64),
+    // ctr[4] hard count
     int64_t *rs_items = (int64_t *)ctr, *hs_items = (int64_t *)(ctr + 2);
     int32_t *n_hard_d = ctr + 4;
-    int64_t *totals = (int64_t *)(ctr + 8);
 
     const unsigned qb = (unsigned)((nq + 255) / 256);
     iota64_kernel<<<qb, 256, 0, st>>>(iota, nq);
@@ -1045,14 +1239,39 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     sa.hist = hist;
     sa.cnt = cnt;
     sa.list = list;
-    sa.n_tiles = (int32_t)((nq + kQT - 1) / kQT);
-    const unsigned n_seg = (unsigned)((n + kSeg - 1) / kSeg);
+    const int32_t n_tiles = (int32_t)((nq + kQT - 1) / kQT);
+    sa.n_seg = (int32_t)((n + kSeg - 1) / kSeg);
     const size_t dyn_lds = sizeof(float) * 2 * (size_t)t.d * kCT;
-    launch_screen<1>(sa, n_seg, dyn_lds, st);
-    prof_mark(t, 3);
-    tau_kernel<<<qb, 256, 0, st>>>(hist, nq, k, tau);
-    prof_mark(t, 4);
-    launch_screen<2>(sa, n_seg, dyn_lds, st);
+    // launch groups of query tiles whose pair codes fit the code budget
+    const int64_t tile_bytes = (int64_t)sa.n_seg * kSeg * kQT;
+    const int32_t group =
+        (int32_t)std::max<int64_t>(1, std::min<int64_t>(n_tiles, kCodeBudget / tile_bytes));
+    if (t.codes.count < (size_t)(group * tile_bytes)) {
+        CX_HIP(hipStreamSynchronize(st));
+        t.codes.alloc((size_t)(group * tile_bytes));
+    }
+    sa.codes = (uint32_t *)t.codes.ptr;
+    const bool one_group = group >= n_tiles;
+    for (int32_t t0 = 0; t0 < n_tiles; t0 += group) {
+        sa.tile0 = t0;
+        sa.n_tiles = std::min(group, n_tiles - t0);
+        launch_screen(sa, dyn_lds, st);
+        if (one_group)
+            prof_mark(t, 3);
+        const int64_t s_begin = (int64_t)t0 * kQT;
+        const int64_t s_end = std::min<int64_t>(nq, (int64_t)(t0 + sa.n_tiles) * kQT);
+        tau_kernel<<<(unsigned)((s_end - s_begin + 255) / 256), 256, 0, st>>>(hist, s_begin,
+                                                                               s_end, k, tau);
+        if (one_group)
+            prof_mark(t, 4);
+        filter_kernel<<<dim3((unsigned)sa.n_tiles, (unsigned)sa.n_seg), kScreenThreads, 0, st>>>(
+            sa);
+        CX_HIP(hipGetLastError());
+    }
+    if (!one_group) {
+        prof_mark(t, 3);
+        prof_mark(t, 4);
+    }
     prof_mark(t, 5);
 
     RescoreArgs ra{};
@@ -1082,105 +1301,53 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     ra.out_dist = d_dist;
     ra.out_err = d_err;
     ra.counter = rs_items;
-    ra.mismatch = ctr + 5;
-    const int64_t blocks = nq < 2048 ? nq : 2048;
-    rescore_kernel<0><<<(unsigned)blocks, kRsThreads, 0, st>>>(ra);
-    CX_HIP(hipGetLastError());
 
-    // hard queries: ordered list, pass-2 counts, pool offsets, slices
-    CX_HIP(hipMemsetAsync(hcnt64, 0, sizeof(int64_t) * nq, st));
-    CX_HIP(hipMemsetAsync(hch, 0, sizeof(int64_t) * nq, st));
+    // hard queries: ordered list (read back once) ...
     hard_flag_kernel<<<qb, 256, 0, st>>>(cnt, nq, flag);
-    CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan32_bytes, flag, pos, (int)nq, st));
-    hard_scatter_kernel<<<qb, 256, 0, st>>>(cnt, flag, pos, nq, hard, hcnt, hcnt64, hch,
-                                            n_hard_d);
-    CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan64_bytes, hcnt64, hoff, (int)nq, st));
-    CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan64_bytes, hch, choff, (int)nq, st));
-    hard_totals_kernel<<<1, 1, 0, st>>>(n_hard_d, hoff, hcnt64, choff, hch, totals);
+    CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan_bytes, flag, pos, (int)nq, st));
+    hard_scatter_kernel<<<qb, 256, 0, st>>>(flag, pos, nq, hard, n_hard_d);
+    if (!t.host_word)
+        CX_HIP(hipHostMalloc((void **)&t.host_word, 64, hipHostMallocDefault));
+    CX_HIP(hipMemcpyAsync(t.host_word, n_hard_d, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (!t.sync_event)
+        CX_HIP(hipEventCreateWithFlags(&t.sync_event, hipEventDisableTiming));
+    hipEvent_t ev = t.sync_event;
+    CX_HIP(hipEventRecord(ev, st));
+    // ... while the listed queries are rescored
+    const int64_t blocks = nq < 2048 ? nq : 2048;
+    rescore_kernel<<<(unsigned)blocks, kRsThreads, 0, st>>>(ra);
+    CX_HIP(hipGetLastError());
     prof_mark(t, 6);
-    int64_t tot[3];
-    read_words(t, totals, 3, tot);
-    const int32_t n_hard = (int32_t)tot[0];
+    CX_HIP(hipEventSynchronize(ev));
+    const int32_t n_hard = *t.host_word;
     t.last_hard = n_hard;
     if (n_hard > 0) {
-        const int64_t pool_total = tot[1], chunk_total = tot[2];
-        // batches of hard queries whose lists fit the pool and whose slice
-        // lists fit the partial buffer
-        const int64_t pool_max = (int64_t)1 << 28;
-        const int64_t chunk_max = (int64_t)1 << 16;
-        const bool split = pool_total > pool_max || chunk_total > chunk_max;
-        std::vector<int64_t> h_off, h_ch;
-        if (split) {
-            h_off.resize((size_t)n_hard + 1);
-            h_ch.resize((size_t)n_hard + 1);
-            CX_HIP(hipMemcpy(h_off.data(), hoff, sizeof(int64_t) * n_hard,
-                             hipMemcpyDeviceToHost));
-            CX_HIP(hipMemcpy(h_ch.data(), choff, sizeof(int64_t) * n_hard,
-                             hipMemcpyDeviceToHost));
-            h_off[(size_t)n_hard] = pool_total;
-            h_ch[(size_t)n_hard] = chunk_total;
-        }
-        const size_t pool_need = (size_t)std::min(pool_total, pool_max);
-        const size_t part_need = sizeof(HardEntry) * (size_t)std::min(chunk_total, chunk_max) * k;
-        if (t.pool.count < pool_need || t.part.count < part_need) {
+        HardArgs ha{};
+        ha.r = ra;
+        ha.hard = hard;
+        ha.n_hard = n_hard;
+        ha.nseg = (int32_t)((n + kHSeg - 1) / kHSeg);
+        ha.counter = hs_items;
+        const int32_t n_groups = (n_hard + kHQ - 1) / kHQ;
+        const int64_t group_bytes = (int64_t)ha.nseg * kHQ * k * sizeof(HardEntry);
+        const int32_t batch = (int32_t)std::max<int64_t>(
+            1, std::min<int64_t>(n_groups, kPartBudget / group_bytes));
+        if (t.part.count < (size_t)(batch * group_bytes)) {
             CX_HIP(hipStreamSynchronize(st));
-            if (t.pool.count < pool_need)
-                t.pool.alloc(pool_need);
-            if (t.part.count < part_need)
-                t.part.alloc(part_need);
+            t.part.alloc((size_t)(batch * group_bytes));
         }
-        ScreenArgs sh = sa;
-        sh.list = t.pool.ptr;
-        sh.cnt = cnt3;
-        for (int32_t h0 = 0; h0 < n_hard;) {
-            int32_t h1 = n_hard;
-            if (split) {
-                h1 = h0 + 1;
-                while (h1 < n_hard && h_off[(size_t)h1 + 1] - h_off[(size_t)h0] <= pool_max &&
-                       h_ch[(size_t)h1 + 1] - h_ch[(size_t)h0] <= chunk_max)
-                    h1++;
-                CX_REQUIRE(h_off[(size_t)h1] - h_off[(size_t)h0] <= pool_max &&
-                               h_ch[(size_t)h1] - h_ch[(size_t)h0] <= chunk_max,
-                           "a single query's candidate list exceeds the hard pool");
-            }
-            const int32_t hn = h1 - h0;
-            const int64_t base = split ? h_off[(size_t)h0] : 0;
-            const int64_t cbase = split ? h_ch[(size_t)h0] : 0;
-            const int64_t nch = split ? h_ch[(size_t)h1] - h_ch[(size_t)h0] : chunk_total;
-            CX_HIP(hipMemsetAsync(cnt3, 0, sizeof(int32_t) * hn, st));
-            sh.qmap = hard + h0;
-            sh.cap = hcnt + h0;
-            sh.off = hoff + h0;
-            sh.off_base = base;
-            sh.nq = hn;
-            sh.n_tiles = (hn + kQT - 1) / kQT;
-            launch_screen<2>(sh, n_seg, dyn_lds, st);
-            RescoreArgs rh = ra;
-            rh.h0 = h0;
-            rh.hn = hn;
-            rh.hard = hard;
-            rh.cap = hcnt;
-            rh.off = hoff;
-            rh.off_base = base;
-            rh.chunk_off = choff;
-            rh.chunk_base = cbase;
-            rh.n_chunks = nch;
-            rh.part = (HardEntry *)t.part.ptr;
-            rh.cnt = cnt3;
-            rh.list = t.pool.ptr;
-            rh.counter = hs_items;
+        ha.part = (HardEntry *)t.part.ptr;
+        for (int32_t g0 = 0; g0 < n_groups; g0 += batch) {
+            ha.g0 = g0;
+            ha.gn = std::min(batch, n_groups - g0);
             CX_HIP(hipMemsetAsync(hs_items, 0, sizeof(int64_t), st));
-            const int64_t hb = nch < 2048 ? nch : 2048;
-            rescore_kernel<1><<<(unsigned)hb, kRsThreads, 0, st>>>(rh);
+            const int64_t items = (int64_t)ha.gn * ha.nseg;
+            hard_tile_kernel<<<(unsigned)std::min<int64_t>(items, 2048), kRsThreads, 0, st>>>(ha);
             CX_HIP(hipGetLastError());
-            const int64_t mb = hn < 2048 ? hn : 2048;
-            hard_merge_kernel<<<(unsigned)mb, kRsThreads, 0, st>>>(rh);
+            const int64_t hq = std::min<int64_t>((int64_t)ha.gn * kHQ, 2048);
+            hard_merge_kernel<<<(unsigned)hq, kRsThreads, 0, st>>>(ha);
             CX_HIP(hipGetLastError());
-            h0 = h1;
         }
-        int32_t bad = 0;
-        read_words(t, ctr + 5, 1, &bad);
-        CX_REQUIRE(bad == 0, "internal error: hard-query list size differs from its count");
     }
     prof_mark(t, 7);
 }
