@@ -29,6 +29,22 @@ CX_KNN_MIXED = 1
 CX_KNN_REFERENCE_F32_BUDGETED = 2
 CX_KNN_EXACT = 1
 
+# per-batch build telemetry fields (include/covertreex_hip.h CX_BF_*)
+CX_BATCH_FIELDS = 16
+BATCH_FIELDS = ("start", "size", "leaders", "survivors", "orphans", "far_levels",
+                "scan_evals", "mis_evals", "nonleader_evals", "grid_edges", "distance_edges",
+                "scan_ns", "mis_ns", "nonleader_ns", "total_ns")
+# query telemetry layout (CX_QT_*)
+CX_QT_SIZE = 160
+CX_QT_LEVELS = 64
+QT_SCALARS = ("mode", "queries", "distance_evals", "prunes_lower_bound",
+              "prunes_lower_bound_chunks", "predecessor_filtered", "subtrees_pruned",
+              "budget_escalations", "budget_early_terminate", "listed_candidates",
+              "hard_queries", "hard_pairs", "max_list", "n_levels")
+CX_QT_FRONTIER = 14
+CX_QT_EXPANDED = 78
+CX_QT_BLOCKS = 142
+
 _i64 = C.c_int64
 _i32 = C.c_int32
 _f32 = C.c_float
@@ -60,6 +76,12 @@ SIGNATURES = {
                                    C.c_double, _p, _p]),
     "cx_tree_set_profiling": (_i32, [_p, _i32]),
     "cx_tree_last_profile": (_i32, [_p, _p, _i32, _p]),
+    "cx_tree_insert_residual": (_i32, [_p, _p, _p, _p, _i64, _i32]),
+    "cx_tree_insert_euclidean": (_i32, [_p, _p, _i64, _i32]),
+    "cx_tree_build_telemetry": (_i32, [_p, _p, _i64, _p]),
+    "cx_tree_set_query_telemetry": (_i32, [_p, _i32]),
+    "cx_tree_last_query_telemetry": (_i32, [_p, _p, _i32, _p]),
+    "cx_tree_clear_last_query_telemetry": (_i32, [_p]),
     "cx_tree_free": (None, [_p]),
 }
 

@@ -107,6 +107,40 @@ static void copy_to_host(const TreeState &t, void *dst, const void *src, size_t 
     CX_HIP(hipStreamSynchronize(t.stream));
 }
 
+// zeroed device counters for this call when query telemetry is on
+void begin_query_telemetry(TreeState &t, int32_t mode, int64_t nq)
+{
+    t.last_telem.clear();
+    t.telem = nullptr;
+    if (!t.query_telemetry)
+        return;
+    if (!t.telem_buf.ptr)
+        t.telem_buf.alloc(CX_QT_SIZE);
+    CX_HIP(hipMemsetAsync(t.telem_buf.ptr, 0, sizeof(int64_t) * CX_QT_SIZE, t.stream));
+    t.telem = t.telem_buf.ptr;
+    t.telem_mode = mode;
+    t.telem_nq = nq;
+}
+
+void end_query_telemetry(TreeState &t)
+{
+    if (!t.telem)
+        return;
+    t.last_telem.assign(CX_QT_SIZE, 0);
+    CX_HIP(hipMemcpyAsync(t.last_telem.data(), t.telem, sizeof(int64_t) * CX_QT_SIZE,
+                          hipMemcpyDeviceToHost, t.stream));
+    CX_HIP(hipStreamSynchronize(t.stream));
+    t.last_telem[CX_QT_MODE] = t.telem_mode;
+    t.last_telem[CX_QT_QUERIES] = t.telem_nq;
+    int64_t lv = 0;
+    for (int i = 0; i < CX_QT_LEVELS; i++)
+        if (t.last_telem[CX_QT_FRONTIER + i] || t.last_telem[CX_QT_EXPANDED + i])
+            lv = i + 1;
+    t.last_telem[CX_QT_N_LEVELS] = lv;
+    t.telem = nullptr;
+}
+
+
 } // namespace cx
 
 using namespace cx;
@@ -184,6 +218,7 @@ cx_status cx_tree_build_residual(const float *V, const float *p, const float *co
     });
 }
 
+
 cx_status cx_tree_knn_residual_device(cx_tree *t, const int64_t *d_queries, int64_t nq, int32_t k,
                                       int32_t predecessor, int32_t mode, int64_t *d_out_idx,
                                       float *d_out_dist, float *d_out_err)
@@ -200,6 +235,7 @@ cx_status cx_tree_knn_residual_device(cx_tree *t, const int64_t *d_queries, int6
             return;
         t->s.n_marks = 0;
         t->s.last_mode = mode;
+        begin_query_telemetry(t->s, mode, nq);
         if (mode == CX_KNN_REFERENCE_F32 || mode == CX_KNN_REFERENCE_F32_BUDGETED)
             knn_reference_f32(t->s, d_queries, nq, k, predecessor,
                               mode == CX_KNN_REFERENCE_F32_BUDGETED, d_out_idx, d_out_dist,
@@ -208,6 +244,7 @@ cx_status cx_tree_knn_residual_device(cx_tree *t, const int64_t *d_queries, int6
             knn_mixed(t->s, d_queries, nq, k, predecessor, d_out_idx, d_out_dist, d_out_err);
         else
             throw Error(CX_ERR_INVALID, "unknown kNN mode");
+        end_query_telemetry(t->s);
         CX_HIP(hipStreamSynchronize(t->s.stream));
     });
 }
@@ -412,6 +449,101 @@ cx_status cx_tree_last_profile(cx_tree *t, float *ms_out, int32_t cap, int32_t *
             CX_HIP(hipEventElapsedTime(&ms, s.marks[i], s.marks[i + 1]));
             ms_out[i] = ms;
         }
+    });
+}
+
+cx_status cx_tree_insert_residual(cx_tree *t, const float *V, const float *p,
+                                  const float *coords, int64_t n_new, int32_t chunk)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        CX_REQUIRE(n_new >= 0, "n_new must be >= 0");
+        CX_REQUIRE(n_new == 0 || (V && p && coords), "V, p and coords must be non-NULL");
+        std::lock_guard<std::mutex> lock(t->s.mu);
+        TreeState &s = t->s;
+        CX_REQUIRE(s.metric == 0, "insert_residual on a Euclidean tree");
+        CX_REQUIRE(s.n > 0, "insert into an empty tree: build it with the rows instead");
+        CX_HIP(hipSetDevice(s.device));
+        if (n_new == 0)
+            return;
+        for (int64_t i = 0; i < n_new; i++)
+            s.p_min = p[i] < s.p_min ? p[i] : s.p_min;
+        for (int64_t i = 0; i < n_new; i++)
+            for (int32_t c = 0; c < s.d; c++) {
+                const double v = std::fabs((double)coords[i * s.d + c] * (double)s.inv_ls[c]);
+                s.xs_max = v > s.xs_max ? v : s.xs_max;
+            }
+        insert_residual_rows(s, V, p, coords, n_new, chunk);
+    });
+}
+
+cx_status cx_tree_insert_euclidean(cx_tree *t, const float *points, int64_t n_new, int32_t chunk)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        CX_REQUIRE(n_new >= 0, "n_new must be >= 0");
+        CX_REQUIRE(n_new == 0 || points, "points must be non-NULL");
+        std::lock_guard<std::mutex> lock(t->s.mu);
+        TreeState &s = t->s;
+        CX_REQUIRE(s.metric == 1, "insert_euclidean on a residual tree");
+        CX_REQUIRE(s.n > 0, "insert into an empty tree: build it with the points instead");
+        CX_HIP(hipSetDevice(s.device));
+        if (n_new == 0)
+            return;
+        insert_euclidean_rows(s, points, n_new, chunk);
+    });
+}
+
+cx_status cx_tree_build_telemetry(const cx_tree *t, int64_t *out, int64_t cap_batches,
+                                  int64_t *n_batches)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr && n_batches != nullptr, "NULL pointer");
+        const TreeState &s = t->s;
+        const int64_t nb = (int64_t)s.batch_stats.size() / CX_BATCH_FIELDS;
+        *n_batches = nb;
+        if (!out)
+            return;
+        int khz = 0;
+        CX_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s.device));
+        const double ns_per_tick = khz > 0 ? 1e6 / (double)khz : 0.0;
+        for (int64_t b = 0; b < nb && b < cap_batches; b++)
+            for (int f = 0; f < CX_BATCH_FIELDS; f++) {
+                int64_t v = s.batch_stats[(size_t)(b * CX_BATCH_FIELDS + f)];
+                if (f >= CX_BF_NS_SCAN && f <= CX_BF_NS_TOTAL)
+                    v = (int64_t)((double)v * ns_per_tick);
+                out[b * CX_BATCH_FIELDS + f] = v;
+            }
+    });
+}
+
+cx_status cx_tree_set_query_telemetry(cx_tree *t, int32_t enable)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        std::lock_guard<std::mutex> lock(t->s.mu);
+        t->s.query_telemetry = enable != 0;
+    });
+}
+
+cx_status cx_tree_last_query_telemetry(const cx_tree *t, int64_t *out, int32_t cap,
+                                       int32_t *n_out)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr && n_out != nullptr, "NULL pointer");
+        const TreeState &s = t->s;
+        *n_out = (int32_t)s.last_telem.size();
+        for (int32_t i = 0; out && i < cap && i < (int32_t)s.last_telem.size(); i++)
+            out[i] = s.last_telem[(size_t)i];
+    });
+}
+
+cx_status cx_tree_clear_last_query_telemetry(cx_tree *t)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        std::lock_guard<std::mutex> lock(t->s.mu);
+        t->s.last_telem.clear();
     });
 }
 

@@ -134,12 +134,17 @@ __device__ __forceinline__ int block_min_int(int v, int *s_red)
     return r;
 }
 
-// stats: [0] near/far scan evaluations, [1] MIS evaluations, [2] non-leader
-// evaluations, [3] leaders, [4] orphans dropped, [5] levels with a far set
+// Inserts rows [s_begin, n) in batches of `chunk`.  Per batch b (counted from
+// s_begin) it records bstats[b * kBatchFields + f] (see cx_internal.hpp):
+// start row, size, leaders, bottom-level survivors, dropped orphans, levels
+// with a far set, distance evaluations of the near/far scan, the MIS and the
+// non-leader step, conflict edges found through the grid and through
+// distance, and wall-clock ticks of the three steps and of the whole batch.
 template <int KIND>
 __global__ void __launch_bounds__(kBiThreads)
     batch_insert_kernel(MetricView mv, TreeLinks t, const float *__restrict__ raw, int32_t gd,
-                        float gs, int32_t use_grid, int64_t n, int64_t chunk, int64_t *stats)
+                        float gs, int32_t use_grid, int64_t s_begin, int64_t n, int64_t chunk,
+                        int64_t *bstats)
 {
     __shared__ int32_t s_ca[kBiMaxChunk], s_cb[kBiMaxChunk];
     __shared__ int32_t s_va[kBiMaxChunk], s_vb[kBiMaxChunk];
@@ -150,11 +155,14 @@ __global__ void __launch_bounds__(kBiThreads)
     __shared__ int s_nl;
 
     const int tid = threadIdx.x;
-    int64_t ev_scan = 0, ev_mis = 0, ev_nl = 0, n_leaders = 0, n_orphans = 0, n_far_levels = 0;
 
-    for (int64_t s = 0; s < n; s += chunk) {
+    for (int64_t s = s_begin; s < n; s += chunk) {
         const int64_t e = s + chunk < n ? s + chunk : n;
         const int nb = (int)(e - s);
+        int64_t ev_scan = 0, ev_mis = 0, ev_nl = 0, n_orphans = 0, e_grid = 0, e_dist = 0;
+        int64_t n_leaders = 0, n_far_levels = 0;
+        uint64_t tk_scan = 0, tk_mis = 0, tk_nl = 0;
+        const uint64_t tk_begin = wall_clock64();
         // add_point(row, i32::MIN, -1), src/algo/batch/mod.rs:511-513
         for (int i = tid; i < nb; i += kBiThreads) {
             st_rx(&t.parents[s + i], (int64_t)-1);
@@ -183,6 +191,7 @@ __global__ void __launch_bounds__(kBiThreads)
         for (int32_t level = 9; level >= -10 && nc > 0; --level) {
             const float r = ldexpf(1.0f, level);
             const float r2 = r * r;
+            const uint64_t tk0 = wall_clock64();
             // near/far over each candidate's cover and the cover's children (580-627)
             if (tid < nc) {
                 const int64_t q = cur_c[tid], p = cur_v[tid];
@@ -213,6 +222,8 @@ __global__ void __launch_bounds__(kBiThreads)
                 s_best[tid] = (int32_t)bn;
             }
             __syncthreads();
+            const uint64_t tk1 = wall_clock64();
+            tk_scan += tk1 - tk0;
             bool active = tid < nc;
             bool near = active && s_best[tid] >= 0;
             int nn = 0;
@@ -248,9 +259,12 @@ __global__ void __launch_bounds__(kBiThreads)
                         bool adj = false;
                         if (use_grid)
                             adj = grid_adjacent(raw, gd, gs, a, b);
-                        if (!adj) {
+                        if (adj) {
+                            e_grid++;
+                        } else {
                             adj = build_dist_sq<KIND>(mv, a, b) <= r2;
                             ev_mis++;
+                            e_dist += adj ? 1 : 0;
                         }
                         if (adj)
                             s_state[tid] = -1;
@@ -275,6 +289,8 @@ __global__ void __launch_bounds__(kBiThreads)
                     n_leaders += nl;
                 }
                 __syncthreads();
+                const uint64_t tk2 = wall_clock64();
+                tk_mis += tk2 - tk1;
                 const int nl = s_nl;
                 // non-leaders: nearest leader within r, strict <, first wins (716-769)
                 if (tid < nf) {
@@ -305,6 +321,7 @@ __global__ void __launch_bounds__(kBiThreads)
                 }
                 nn += nk;
                 __syncthreads();
+                tk_nl += wall_clock64() - tk2;
             }
             int32_t *tc = cur_c, *tv = cur_v;
             cur_c = nxt_c;
@@ -324,16 +341,26 @@ __global__ void __launch_bounds__(kBiThreads)
             }
         }
         __syncthreads();
-    }
-    if (stats) {
-        atomicAdd((unsigned long long *)&stats[0], (unsigned long long)ev_scan);
-        atomicAdd((unsigned long long *)&stats[1], (unsigned long long)ev_mis);
-        atomicAdd((unsigned long long *)&stats[2], (unsigned long long)ev_nl);
-        if (tid == 0) {
-            atomicAdd((unsigned long long *)&stats[3], (unsigned long long)n_leaders);
-            atomicAdd((unsigned long long *)&stats[5], (unsigned long long)n_far_levels);
+        if (bstats) {
+            int64_t *bs = bstats + ((s - s_begin) / chunk) * kBatchFields;
+            atomicAdd((unsigned long long *)&bs[6], (unsigned long long)ev_scan);
+            atomicAdd((unsigned long long *)&bs[7], (unsigned long long)ev_mis);
+            atomicAdd((unsigned long long *)&bs[8], (unsigned long long)ev_nl);
+            atomicAdd((unsigned long long *)&bs[4], (unsigned long long)n_orphans);
+            atomicAdd((unsigned long long *)&bs[9], (unsigned long long)e_grid);
+            atomicAdd((unsigned long long *)&bs[10], (unsigned long long)e_dist);
+            if (tid == 0) {
+                bs[0] = s;
+                bs[1] = nb;
+                bs[2] = n_leaders;
+                bs[3] = nc;
+                bs[5] = n_far_levels;
+                bs[11] = (int64_t)tk_scan;
+                bs[12] = (int64_t)tk_mis;
+                bs[13] = (int64_t)tk_nl;
+                bs[14] = (int64_t)(wall_clock64() - tk_begin);
+            }
         }
-        atomicAdd((unsigned long long *)&stats[4], (unsigned long long)n_orphans);
     }
 }
 
@@ -629,74 +656,51 @@ static int32_t compute_depths(const int64_t *parents, int64_t n, int32_t *depth,
     return h;
 }
 
-void build_residual_tree(TreeState &t, const float *V, const float *p, const float *coords,
-                         int32_t chunk, int32_t order_mode, float grid_scale)
+// Runs the batch insert of rows [s_begin, t.n) and appends the per-batch
+// telemetry to t.batch_stats.
+template <int KIND>
+static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, float grid_scale,
+                       int64_t s_begin, int32_t chunk, hipStream_t st)
 {
     const int64_t n = t.n;
-    const int32_t m = t.m, d = t.d;
-    hipStream_t st = t.stream;
-    CX_REQUIRE(chunk > 0 && chunk <= kBiMaxChunk, "chunk must be in [1, 1024]");
-    // raw inputs to HBM
-    DevBuf<float> v_in((size_t)n * m), c_in((size_t)n * d), p_in(n);
-    CX_HIP(hipMemcpyAsync(v_in.ptr, V, sizeof(float) * n * m, hipMemcpyHostToDevice, st));
-    CX_HIP(hipMemcpyAsync(c_in.ptr, coords, sizeof(float) * n * d, hipMemcpyHostToDevice, st));
-    CX_HIP(hipMemcpyAsync(p_in.ptr, p, sizeof(float) * n, hipMemcpyHostToDevice, st));
-
-    t.order.alloc(n);
-    t.inv_order.alloc(n);
-    if (order_mode == CX_ORDER_NATURAL || d != 3) {
-        iota_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.order.ptr, n);
-        CX_HIP(hipGetLastError());
-    } else {
-        morton_order_on_stream(c_in.ptr, n, d, t.order.ptr, st);
-    }
-    inverse_perm_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.order.ptr, n, t.inv_order.ptr);
-
-    t.v.alloc((size_t)n * m);
-    t.raw.alloc((size_t)n * d);
-    t.scaled.alloc((size_t)n * d);
-    t.norms.alloc(n);
-    t.p.alloc(n);
-    gather_rows_kernel<<<(unsigned)n, 64, 0, st>>>(v_in.ptr, t.order.ptr, n, m, t.v.ptr);
-    gather_rows_kernel<<<(unsigned)n, 64, 0, st>>>(c_in.ptr, t.order.ptr, n, d, t.raw.ptr);
-    gather_rows_kernel<<<(unsigned)n, 64, 0, st>>>(p_in.ptr, t.order.ptr, n, 1, t.p.ptr);
-    DevBuf<float> inv_ls(d);
-    CX_HIP(hipMemcpyAsync(inv_ls.ptr, t.inv_ls.data(), sizeof(float) * d, hipMemcpyHostToDevice,
-                          st));
-    scale_rows_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.raw.ptr, n, d, inv_ls.ptr, t.scaled.ptr,
-                                                       t.norms.ptr);
-    CX_HIP(hipGetLastError());
-
-    t.parents.alloc(n);
-    t.children.alloc(n);
-    t.next.alloc(n);
-    t.levels.alloc(n);
-    MetricView mv{t.scaled.ptr, t.norms.ptr, t.v.ptr, t.p.ptr, d, m, t.var, t.kernel};
+    if (s_begin >= n)
+        return;
+    const int64_t nb = (n - s_begin + chunk - 1) / chunk;
+    DevBuf<int64_t> bst((size_t)nb * kBatchFields);
+    CX_HIP(hipMemsetAsync(bst.ptr, 0, bst.bytes(), st));
     TreeLinks links{t.parents.ptr, t.children.ptr, t.next.ptr, t.levels.ptr};
-    batch_insert_kernel<kMetricResidual><<<1, kBiThreads, 0, st>>>(
-        mv, links, t.raw.ptr, d, grid_scale, 1, n, chunk, nullptr);
+    batch_insert_kernel<KIND><<<1, kBiThreads, 0, st>>>(mv, links, t.raw.ptr, t.d, grid_scale,
+                                                        use_grid, s_begin, n, chunk, bst.ptr);
     CX_HIP(hipGetLastError());
+    const size_t old = t.batch_stats.size();
+    t.batch_stats.resize(old + (size_t)nb * kBatchFields);
+    CX_HIP(hipMemcpyAsync(t.batch_stats.data() + old, bst.ptr, bst.bytes(), hipMemcpyDeviceToHost,
+                          st));
+    CX_HIP(hipStreamSynchronize(st));
+}
+
+// Arrays derived from the links after inserts: child CSR, depths, si_cache
+// (bottom-up one depth at a time), subtree index bounds and the roots in
+// index order.
+template <int KIND> static void finalize_tree(TreeState &t, const MetricView &mv, hipStream_t st)
+{
+    const int64_t n = t.n;
     t.min_level = n > 0 ? -10 : -20;
     t.max_level = n > 0 ? 10 : 20;
     build_child_csr(t, st);
-
-    // depth from each node to its root, then si_cache and subtree bounds
-    // bottom-up one depth at a time
     t.depth.alloc(n);
     DevBuf<int32_t> top(n);
     t.max_depth = compute_depths(t.parents.ptr, n, t.depth.ptr, top.ptr, st);
     t.si.alloc(n);
     CX_HIP(hipMemsetAsync(t.si.ptr, 0, sizeof(float) * n, st));
     for (int32_t lv = t.max_depth; lv >= 1; lv--)
-        si_level_kernel<kMetricResidual><<<nblocks(n, 256), 256, 0, st>>>(
-            mv, t.parents.ptr, t.depth.ptr, top.ptr, n, lv, t.si.ptr);
+        si_level_kernel<KIND><<<nblocks(n, 256), 256, 0, st>>>(mv, t.parents.ptr, t.depth.ptr,
+                                                               top.ptr, n, lv, t.si.ptr);
     CX_HIP(hipGetLastError());
     t.smin.alloc(n);
     t.smax.alloc(n);
     subtree_bounds_on_stream(t.parents.ptr, t.order.ptr, t.depth.ptr, t.max_depth, n, t.smin.ptr,
                              t.smax.ptr, st);
-
-    // roots in index order
     DevBuf<int32_t> flags(n), nsel(1);
     t.roots.alloc(n);
     root_flags_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.parents.ptr, n, flags.ptr);
@@ -713,6 +717,118 @@ void build_residual_tree(TreeState &t, const float *V, const float *p, const flo
     t.n_roots = nr;
 }
 
+static MetricView residual_view(const TreeState &t)
+{
+    return MetricView{t.scaled.ptr, t.norms.ptr, t.v.ptr, t.p.ptr, t.d, t.m, t.var, t.kernel};
+}
+
+// Rows [n0, n0 + nn) of the tree arrays from host V, p, coords (nn rows):
+// tree row n0 + i takes staged row sel[i]; coordinates are also scaled by
+// 1/lengthscale.
+static void load_residual_rows(TreeState &t, const float *V, const float *p, const float *coords,
+                               int64_t n0, int64_t nn, const int64_t *sel, hipStream_t st)
+{
+    const int32_t m = t.m, d = t.d;
+    DevBuf<float> v_in((size_t)nn * m), c_in((size_t)nn * d), p_in(nn);
+    CX_HIP(hipMemcpyAsync(v_in.ptr, V, sizeof(float) * nn * m, hipMemcpyHostToDevice, st));
+    CX_HIP(hipMemcpyAsync(c_in.ptr, coords, sizeof(float) * nn * d, hipMemcpyHostToDevice, st));
+    CX_HIP(hipMemcpyAsync(p_in.ptr, p, sizeof(float) * nn, hipMemcpyHostToDevice, st));
+    gather_rows_kernel<<<(unsigned)nn, 64, 0, st>>>(v_in.ptr, sel, nn, m, t.v.ptr + n0 * m);
+    gather_rows_kernel<<<(unsigned)nn, 64, 0, st>>>(c_in.ptr, sel, nn, d, t.raw.ptr + n0 * d);
+    gather_rows_kernel<<<(unsigned)nn, 64, 0, st>>>(p_in.ptr, sel, nn, 1, t.p.ptr + n0);
+    DevBuf<float> inv_ls(d);
+    CX_HIP(hipMemcpyAsync(inv_ls.ptr, t.inv_ls.data(), sizeof(float) * d, hipMemcpyHostToDevice,
+                          st));
+    scale_rows_kernel<<<nblocks(nn, 256), 256, 0, st>>>(t.raw.ptr + n0 * d, nn, d, inv_ls.ptr,
+                                                        t.scaled.ptr + n0 * d, t.norms.ptr + n0);
+    CX_HIP(hipGetLastError());
+    CX_HIP(hipStreamSynchronize(st)); // the staging buffers go out of scope
+}
+
+void build_residual_tree(TreeState &t, const float *V, const float *p, const float *coords,
+                         int32_t chunk, int32_t order_mode, float grid_scale)
+{
+    const int64_t n = t.n;
+    const int32_t m = t.m, d = t.d;
+    hipStream_t st = t.stream;
+    CX_REQUIRE(chunk > 0 && chunk <= kBiMaxChunk, "chunk must be in [1, 1024]");
+    t.chunk = chunk;
+    t.grid_scale = grid_scale;
+
+    t.order.alloc(n);
+    t.inv_order.alloc(n);
+    if (order_mode == CX_ORDER_NATURAL || d != 3) {
+        iota_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.order.ptr, n);
+        CX_HIP(hipGetLastError());
+    } else {
+        DevBuf<float> c_in((size_t)n * d);
+        CX_HIP(hipMemcpyAsync(c_in.ptr, coords, sizeof(float) * n * d, hipMemcpyHostToDevice,
+                              st));
+        morton_order_on_stream(c_in.ptr, n, d, t.order.ptr, st);
+        CX_HIP(hipStreamSynchronize(st));
+    }
+    inverse_perm_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.order.ptr, n, t.inv_order.ptr);
+
+    t.v.alloc((size_t)n * m);
+    t.raw.alloc((size_t)n * d);
+    t.scaled.alloc((size_t)n * d);
+    t.norms.alloc(n);
+    t.p.alloc(n);
+    load_residual_rows(t, V, p, coords, 0, n, t.order.ptr, st);
+
+    t.parents.alloc(n);
+    t.children.alloc(n);
+    t.next.alloc(n);
+    t.levels.alloc(n);
+    const MetricView mv = residual_view(t);
+    run_insert<kMetricResidual>(t, mv, 1, grid_scale, 0, chunk, st);
+    finalize_tree<kMetricResidual>(t, mv, st);
+}
+
+__global__ void iota_from_kernel(int64_t *out, int64_t n0, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[n0 + i] = n0 + i;
+}
+
+// Appends rows to the tree's arrays: the new rows keep their given order
+// (node n0 + i is dataset row n0 + i) and are batch-inserted with the grid
+// scale of the build, as CoverTreeWrapper.insert_residual does with its
+// batch (src/lib.rs:254-300).
+void insert_residual_rows(TreeState &t, const float *V, const float *p, const float *coords,
+                          int64_t n_new, int32_t chunk)
+{
+    hipStream_t st = t.stream;
+    CX_REQUIRE(chunk > 0 && chunk <= kBiMaxChunk, "chunk must be in [1, 1024]");
+    const int64_t n0 = t.n, n1 = t.n + n_new;
+    const int32_t m = t.m, d = t.d;
+    CX_REQUIRE(n1 < (int64_t)1 << 31, "tree size must stay below 2^31");
+    t.v.grow_keep((size_t)n1 * m, st);
+    t.raw.grow_keep((size_t)n1 * d, st);
+    t.scaled.grow_keep((size_t)n1 * d, st);
+    t.norms.grow_keep(n1, st);
+    t.p.grow_keep(n1, st);
+    t.order.grow_keep(n1, st);
+    t.inv_order.grow_keep(n1, st);
+    t.parents.grow_keep(n1, st);
+    t.children.grow_keep(n1, st);
+    t.next.grow_keep(n1, st);
+    t.levels.grow_keep(n1, st);
+    iota_from_kernel<<<nblocks(n_new, 256), 256, 0, st>>>(t.order.ptr, n0, n_new);
+    iota_from_kernel<<<nblocks(n_new, 256), 256, 0, st>>>(t.inv_order.ptr, n0, n_new);
+    // the staged rows are taken in their given order
+    DevBuf<int64_t> ident(n_new);
+    iota_kernel<<<nblocks(n_new, 256), 256, 0, st>>>(ident.ptr, n_new);
+    CX_HIP(hipGetLastError());
+    load_residual_rows(t, V, p, coords, n0, n_new, ident.ptr, st);
+    t.n = n1;
+    const MetricView mv = residual_view(t);
+    run_insert<kMetricResidual>(t, mv, 1, t.grid_scale, n0, chunk, st);
+    finalize_tree<kMetricResidual>(t, mv, st);
+    t.mixed_ready = false;
+}
+
 // Euclidean tree: chunked batch_insert_collect with the Euclidean metric in
 // natural order and no grid edges (src/lib.rs:207-223, src/algo/batch/mod.rs:
 // 402-409), plus the same cover-radius recurrence with Euclidean distances.
@@ -722,6 +838,7 @@ void build_euclidean_tree(TreeState &t, const float *points, int32_t chunk)
     const int32_t d = t.d;
     hipStream_t st = t.stream;
     CX_REQUIRE(chunk > 0 && chunk <= kBiMaxChunk, "chunk must be in [1, 1024]");
+    t.chunk = chunk;
     t.raw.alloc((size_t)n * d);
     CX_HIP(hipMemcpyAsync(t.raw.ptr, points, sizeof(float) * n * d, hipMemcpyHostToDevice, st));
     t.order.alloc(n);
@@ -732,41 +849,36 @@ void build_euclidean_tree(TreeState &t, const float *points, int32_t chunk)
     t.children.alloc(n);
     t.next.alloc(n);
     t.levels.alloc(n);
-    MetricView mv{t.raw.ptr, nullptr, nullptr, nullptr, d, 0, 1.0f, 0};
-    TreeLinks links{t.parents.ptr, t.children.ptr, t.next.ptr, t.levels.ptr};
-    batch_insert_kernel<kMetricEuclidean><<<1, kBiThreads, 0, st>>>(mv, links, t.raw.ptr, d, 1.0f,
-                                                                    0, n, chunk, nullptr);
+    const MetricView mv{t.raw.ptr, nullptr, nullptr, nullptr, d, 0, 1.0f, 0};
+    run_insert<kMetricEuclidean>(t, mv, 0, 1.0f, 0, chunk, st);
+    finalize_tree<kMetricEuclidean>(t, mv, st);
+}
+
+// CoverTreeWrapper.insert (src/lib.rs:207-223) on a device-resident
+// Euclidean tree: the new points become nodes n0.. in their given order.
+void insert_euclidean_rows(TreeState &t, const float *points, int64_t n_new, int32_t chunk)
+{
+    hipStream_t st = t.stream;
+    CX_REQUIRE(chunk > 0 && chunk <= kBiMaxChunk, "chunk must be in [1, 1024]");
+    const int64_t n0 = t.n, n1 = t.n + n_new;
+    const int32_t d = t.d;
+    CX_REQUIRE(n1 < (int64_t)1 << 31, "tree size must stay below 2^31");
+    t.raw.grow_keep((size_t)n1 * d, st);
+    t.order.grow_keep(n1, st);
+    t.inv_order.grow_keep(n1, st);
+    t.parents.grow_keep(n1, st);
+    t.children.grow_keep(n1, st);
+    t.next.grow_keep(n1, st);
+    t.levels.grow_keep(n1, st);
+    CX_HIP(hipMemcpyAsync(t.raw.ptr + n0 * d, points, sizeof(float) * n_new * d,
+                          hipMemcpyHostToDevice, st));
+    iota_from_kernel<<<nblocks(n_new, 256), 256, 0, st>>>(t.order.ptr, n0, n_new);
+    iota_from_kernel<<<nblocks(n_new, 256), 256, 0, st>>>(t.inv_order.ptr, n0, n_new);
     CX_HIP(hipGetLastError());
-    t.min_level = n > 0 ? -10 : -20;
-    t.max_level = n > 0 ? 10 : 20;
-    build_child_csr(t, st);
-    t.depth.alloc(n);
-    DevBuf<int32_t> top(n);
-    t.max_depth = compute_depths(t.parents.ptr, n, t.depth.ptr, top.ptr, st);
-    t.si.alloc(n);
-    CX_HIP(hipMemsetAsync(t.si.ptr, 0, sizeof(float) * n, st));
-    for (int32_t lv = t.max_depth; lv >= 1; lv--)
-        si_level_kernel<kMetricEuclidean><<<nblocks(n, 256), 256, 0, st>>>(
-            mv, t.parents.ptr, t.depth.ptr, top.ptr, n, lv, t.si.ptr);
-    CX_HIP(hipGetLastError());
-    t.smin.alloc(n);
-    t.smax.alloc(n);
-    subtree_bounds_on_stream(t.parents.ptr, t.order.ptr, t.depth.ptr, t.max_depth, n, t.smin.ptr,
-                             t.smax.ptr, st);
-    DevBuf<int32_t> flags(n), nsel(1);
-    t.roots.alloc(n);
-    root_flags_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.parents.ptr, n, flags.ptr);
-    hipcub::CountingInputIterator<int32_t> it(0);
-    size_t tmp_bytes = 0;
-    CX_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp_bytes, it, flags.ptr, t.roots.ptr, nsel.ptr,
-                                         (int)n, st));
-    DevBuf<uint8_t> tmp(tmp_bytes);
-    CX_HIP(hipcub::DeviceSelect::Flagged(tmp.ptr, tmp_bytes, it, flags.ptr, t.roots.ptr, nsel.ptr,
-                                         (int)n, st));
-    int32_t nr = 0;
-    CX_HIP(hipMemcpyAsync(&nr, nsel.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    CX_HIP(hipStreamSynchronize(st));
-    t.n_roots = nr;
+    t.n = n1;
+    const MetricView mv{t.raw.ptr, nullptr, nullptr, nullptr, d, 0, 1.0f, 0};
+    run_insert<kMetricEuclidean>(t, mv, 0, 1.0f, n0, chunk, st);
+    finalize_tree<kMetricEuclidean>(t, mv, st);
 }
 
 void morton_order_device(const float *coords_host, int64_t n, int32_t d, int64_t *order_host)

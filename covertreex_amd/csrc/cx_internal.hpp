@@ -75,6 +75,41 @@ template <typename T> struct DevBuf {
         count = 0;
     }
     size_t bytes() const { return count * sizeof(T); }
+    // reallocate to n elements keeping the first min(count, n) (stream-ordered
+    // copy, then the old buffer is freed after the stream drains)
+    void grow_keep(size_t n, hipStream_t st)
+    {
+        T *np = nullptr;
+        if (n)
+            CX_HIP(hipMalloc((void **)&np, n * sizeof(T) + 256));
+        const size_t keep = count < n ? count : n;
+        if (keep)
+            CX_HIP(hipMemcpyAsync(np, ptr, keep * sizeof(T), hipMemcpyDeviceToDevice, st));
+        CX_HIP(hipStreamSynchronize(st));
+        release();
+        ptr = np;
+        count = n;
+    }
+};
+
+// Per-batch build telemetry (int64 fields per batch in TreeState::batch_stats)
+constexpr int kBatchFields = 16;
+enum BatchField {
+    kBfStart = 0,     // first row (tree order) of the batch
+    kBfSize = 1,      // rows in the batch
+    kBfLeaders = 2,   // MIS-selected leaders over all levels
+    kBfSurvivors = 3, // rows attached at the bottom level
+    kBfOrphans = 4,   // far non-leaders dropped (left as roots)
+    kBfFarLevels = 5, // levels with a non-empty far set
+    kBfScanEvals = 6, // distance evaluations: near/far scan
+    kBfMisEvals = 7,  // distance evaluations: MIS conflicts
+    kBfNlEvals = 8,   // distance evaluations: non-leader attachment
+    kBfGridEdges = 9, // conflicts found through the grid
+    kBfDistEdges = 10, // conflicts found through d^2 <= r^2
+    kBfTicksScan = 11, // wall-clock ticks: near/far scan
+    kBfTicksMis = 12,  // wall-clock ticks: MIS and leader linking
+    kBfTicksNl = 13,   // wall-clock ticks: non-leader attachment
+    kBfTicksTotal = 14 // wall-clock ticks: whole batch
 };
 
 // Device-resident tree and metric data (all arrays in tree order).
@@ -117,6 +152,22 @@ struct TreeState {
     DevBuf<uint8_t> codes;         // one byte per screened pair (a launch group)
     DevBuf<uint8_t> part;          // per-segment top-k lists of overflowed queries
 
+    // query telemetry: when enabled, kNN kernels add their counters into
+    // telem (CX_QT_SIZE int64, zeroed per call; null when disabled) and the
+    // entry point copies them to last_telem
+    bool query_telemetry = false;
+    DevBuf<int64_t> telem_buf;
+    int64_t *telem = nullptr;
+    std::vector<int64_t> last_telem;
+    int32_t telem_mode = 0;
+    int64_t telem_nq = 0;
+
+    // build parameters kept for inserts, and the per-batch telemetry of every
+    // insert so far (kBatchFields int64 per batch)
+    int32_t chunk = 512;
+    float grid_scale = 1.0f;
+    std::vector<int64_t> batch_stats;
+
     // reusable query workspace
     DevBuf<uint8_t> work;
     // pinned host word for small device-to-host reads (hipHostMalloc)
@@ -141,11 +192,16 @@ struct TreeState {
 void build_residual_tree(TreeState &t, const float *V, const float *p, const float *coords,
                          int32_t chunk, int32_t order_mode, float grid_scale);
 void build_euclidean_tree(TreeState &t, const float *points, int32_t chunk);
+void insert_residual_rows(TreeState &t, const float *V, const float *p, const float *coords,
+                          int64_t n_new, int32_t chunk);
+void insert_euclidean_rows(TreeState &t, const float *points, int64_t n_new, int32_t chunk);
 void morton_order_device(const float *coords_host, int64_t n, int32_t d, int64_t *order_host);
 void subtree_bounds_device(const int64_t *parents_host, const int64_t *n2d_host, int64_t n,
                            int64_t *min_host, int64_t *max_host);
 
 // knn_ref.hip
+void begin_query_telemetry(TreeState &t, int32_t mode, int64_t nq);
+void end_query_telemetry(TreeState &t);
 void knn_reference_f32(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k,
                        int32_t predecessor, int32_t budgeted, int64_t *d_idx, float *d_dist,
                        float *d_err);

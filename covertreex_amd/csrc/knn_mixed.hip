@@ -1077,6 +1077,42 @@ __global__ void hard_scatter_kernel(const int32_t *__restrict__ flag,
         *n_hard = pos[s] + flag[s];
 }
 
+// Query telemetry of a mixed call: pairs screened (every valid candidate of
+// every query), candidates listed for rescoring, hard queries and their
+// pairs, the longest list.
+__global__ void mixed_telemetry_kernel(const int64_t *__restrict__ qsorted,
+                                       const int32_t *__restrict__ cnt, int64_t nq, int64_t n,
+                                       int32_t pred, int64_t *telem)
+{
+    __shared__ unsigned long long s_acc[4];
+    __shared__ int s_max;
+    if (threadIdx.x < 4)
+        s_acc[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0)
+        s_max = 0;
+    __syncthreads();
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nq) {
+        const int64_t qid = qsorted[s];
+        const int64_t lim = pred ? (qid < 0 ? n : (qid < n ? qid : n)) : n;
+        const int32_t c = cnt[s];
+        const bool hard = c > kCap;
+        atomicAdd(&s_acc[0], (unsigned long long)lim);
+        atomicAdd(&s_acc[1], hard ? 0ull : (unsigned long long)c);
+        atomicAdd(&s_acc[2], hard ? 1ull : 0ull);
+        atomicAdd(&s_acc[3], hard ? (unsigned long long)lim : 0ull);
+        atomicMax(&s_max, c);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd((unsigned long long *)&telem[CX_QT_DISTANCE_EVALS], s_acc[0]);
+        atomicAdd((unsigned long long *)&telem[CX_QT_LISTED], s_acc[1]);
+        atomicAdd((unsigned long long *)&telem[CX_QT_HARD_QUERIES], s_acc[2]);
+        atomicAdd((unsigned long long *)&telem[CX_QT_HARD_PAIRS], s_acc[3]);
+        atomicMax((unsigned long long *)&telem[CX_QT_MAX_LIST], (unsigned long long)s_max);
+    }
+}
+
 // ---- driver -----------------------------------------------------------------
 
 __global__ void query_rows_kernel(const int64_t *__restrict__ qsorted, int64_t nq, int64_t n,
@@ -1349,6 +1385,8 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
             CX_HIP(hipGetLastError());
         }
     }
+    if (t.telem)
+        mixed_telemetry_kernel<<<qb, 256, 0, st>>>(qs, cnt, nq, n, predecessor ? 1 : 0, t.telem);
     prof_mark(t, 7);
 }
 

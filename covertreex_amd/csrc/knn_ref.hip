@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(64)
     knn_ref_kernel(MetricView mv, RefTree tr, const int64_t *__restrict__ queries, int64_t nq,
                    int32_t k, int32_t pred, int64_t *out_idx, float *out_dist, float *out_err,
                    int32_t *counter, int32_t *ws_node, float *ws_val, int64_t fr_cap,
-                   int64_t ch_cap, Budget bud)
+                   int64_t ch_cap, Budget bud, int64_t *telem)
 {
     __shared__ float s_td[kRefMaxK];
     __shared__ int32_t s_tn[kRefMaxK];
@@ -157,6 +157,10 @@ __global__ void __launch_bounds__(64)
         if (n == 0 || (pred && q_ds == 0))
             continue;
 
+        // telemetry counters (wave-uniform), added to telem at the end
+        int64_t t_evals = 0, t_lb = 0, t_lbc = 0, t_pred = 0, t_sub = 0, t_esc = 0, t_early = 0;
+        int32_t t_level = 0;
+
         // ---- roots (688-734) ------------------------------------------------
         int32_t budget_idx = 0;
         int64_t budget_limit = bud.n > 0 ? bud.sched[0] : INT64_MAX;
@@ -188,6 +192,8 @@ __global__ void __launch_bounds__(64)
             }
             fr_stored += __popcll(smask);
             fr_count += __popcll(__ballot(live));
+            t_evals += __popcll(__ballot(live));
+            t_pred += __popcll(__ballot(live && !valid));
             uint64_t vmask = __ballot(valid);
             heap_len += __popcll(vmask);
             survivors += __popcll(vmask);
@@ -216,13 +222,18 @@ __global__ void __launch_bounds__(64)
         while (!early && fr_count > 0 &&
                (survivors < budget_limit || (pred && heap_len < k))) {
             int64_t ch_count = 0, nfr_count = 0, nfr_stored = 0;
+            if (telem && lane == 0 && t_level < CX_QT_LEVELS)
+                atomicAdd((unsigned long long *)&telem[CX_QT_FRONTIER + t_level],
+                          (unsigned long long)fr_count);
             // gather (862-995)
             for (int64_t f = 0; f < fr_stored; f++) {
                 const int32_t parent = fa_n[f];
                 const float pdist = fa_d[f];
                 const float plb = pdist - radius_of(tr, parent);
-                if (heap_len == k && plb > kth)
+                if (heap_len == k && plb > kth) {
+                    t_lb++;
                     continue;
+                }
                 const int32_t c0 = tr.child_off[parent], c1 = tr.child_off[parent + 1];
                 for (int32_t cb = c0; cb < c1; cb += 64) {
                     const int32_t ci = cb + lane;
@@ -243,6 +254,9 @@ __global__ void __launch_bounds__(64)
                     }
                     nfr_stored += __popcll(fm);
                     nfr_count += __popcll(__ballot(failed));
+                    t_evals += __popcll(__ballot(failed));
+                    t_pred += __popcll(__ballot(failed));
+                    t_sub += __popcll(__ballot(live && pred && tr.smin[ch] >= max_idx));
                     uint64_t om = __ballot(ok);
                     int op = __popcll(om & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
                     if (ok) {
@@ -252,6 +266,10 @@ __global__ void __launch_bounds__(64)
                     ch_count += __popcll(om);
                 }
             }
+            if (telem && lane == 0 && t_level < CX_QT_LEVELS)
+                atomicAdd((unsigned long long *)&telem[CX_QT_EXPANDED + t_level],
+                          (unsigned long long)ch_count);
+            t_level++;
             // tiles (997-1222)
             int64_t start = 0;
             while (start < ch_count && (survivors < budget_limit || (pred && heap_len < k))) {
@@ -273,14 +291,21 @@ __global__ void __launch_bounds__(64)
                         mn = u < mn ? u : mn;
                     }
                     if (mn > kth) {
+                        t_lbc++;
                         start = end;
                         continue;
                     }
                 }
+                if (telem && lane == 0)
+                    atomicAdd((unsigned long long *)&telem[CX_QT_BLOCKS +
+                                                           (tile == 16 ? 0 : tile == 32 ? 1 : 2)],
+                              1ull);
                 bool eval = live && !(heap_len == k && lb > kth); // cached_lb prune (1053-1066)
                 float dist = eval ? dist_batch_ref(mv, q, node) : __builtin_huge_valf();
                 int pos = lane;
                 const int n_ev = __popcll(__ballot(eval));
+                t_evals += n_ev;
+                t_lb += __popcll(__ballot(live && !eval));
                 // stable sort by distance (1103-1109): bitonic on (dist, position)
                 for (int size = 2; size <= 64; size <<= 1) {
                     for (int j = size >> 1; j > 0; j >>= 1) {
@@ -308,8 +333,10 @@ __global__ void __launch_bounds__(64)
                     const float crad = radius_of(tr, child);
                     if (heap_len == k) {
                         float lbv = fminf(plbc, dv - crad);
-                        if (lbv > kth)
+                        if (lbv > kth) {
+                            t_lb++;
                             continue;
+                        }
                     }
                     topk_insert(tk, dv, child);
                     if (heap_len < k)
@@ -338,13 +365,16 @@ __global__ void __launch_bounds__(64)
                         if (budget_idx + 1 < bud.n) {
                             budget_idx++;
                             budget_limit = bud.sched[budget_idx];
+                            t_esc++;
                         }
                         low_yield = 0;
                     } else if (ratio < bud.down) {
                         low_yield++;
                         const bool need_more = pred && heap_len < k;
-                        if (low_yield >= 2 && !need_more)
+                        if (low_yield >= 2 && !need_more) {
                             stop = true;
+                            t_early++;
+                        }
                     } else {
                         low_yield = 0;
                     }
@@ -362,6 +392,19 @@ __global__ void __launch_bounds__(64)
             fr_count = nfr_count;
             fr_stored = nfr_stored;
             __syncthreads();
+        }
+
+        if (telem && lane == 0) {
+            atomicAdd((unsigned long long *)&telem[CX_QT_DISTANCE_EVALS], (unsigned long long)t_evals);
+            atomicAdd((unsigned long long *)&telem[CX_QT_PRUNES_LB], (unsigned long long)t_lb);
+            atomicAdd((unsigned long long *)&telem[CX_QT_PRUNES_LB_CHUNKS],
+                      (unsigned long long)t_lbc);
+            atomicAdd((unsigned long long *)&telem[CX_QT_PRED_FILTERED], (unsigned long long)t_pred);
+            atomicAdd((unsigned long long *)&telem[CX_QT_SUBTREES_PRUNED],
+                      (unsigned long long)t_sub);
+            atomicAdd((unsigned long long *)&telem[CX_QT_BUDGET_ESC], (unsigned long long)t_esc);
+            atomicAdd((unsigned long long *)&telem[CX_QT_BUDGET_EARLY],
+                      (unsigned long long)t_early);
         }
 
         // ---- output: ascending (into_sorted_vec, 1232-1240), truncated to k
@@ -413,7 +456,7 @@ void knn_reference_f32(TreeState &t, const int64_t *d_queries, int64_t nq, int32
     }
     knn_ref_kernel<<<(unsigned)waves, 64, 0, st>>>(mv, tr, d_queries, nq, k, predecessor, d_idx,
                                                    d_dist, d_err, counter, ws_node, ws_val,
-                                                   fr_cap, ch_cap, bud);
+                                                   fr_cap, ch_cap, bud, t.telem);
     CX_HIP(hipGetLastError());
     prof_mark(t, 1);
 }

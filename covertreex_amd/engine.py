@@ -9,6 +9,7 @@ through ``libcovertreex_hip.so``.
 
 from __future__ import annotations
 
+import os
 import time
 from dataclasses import dataclass, field
 from typing import Any, Callable, Dict, Mapping, Protocol
@@ -96,6 +97,16 @@ class HipHandle:
     subtree_min_bounds: np.ndarray | None
 
 
+def _query_telemetry_enabled() -> bool:
+    """COVERTREEX_HIP_QUERY_TELEMETRY (or the reference's
+    COVERTREEX_RUST_QUERY_TELEMETRY, src/lib.rs:404-406) set to 1/true."""
+    for key in ("COVERTREEX_HIP_QUERY_TELEMETRY", "COVERTREEX_RUST_QUERY_TELEMETRY"):
+        value = os.environ.get(key, "")
+        if value == "1" or value.lower() == "true":
+            return True
+    return False
+
+
 class HipEngine:
     """Residual-correlation cover tree built and queried on an MI355X."""
 
@@ -134,6 +145,11 @@ class HipEngine:
                                       grid_whiten_scale=runtime.residual_grid_whiten_scale,
                                       kernel_type=kernel_type)
         build_seconds = time.perf_counter() - start
+        if log_writer is not None:
+            # one benchmark-batch record per device batch (src/lib.rs:1274-1371)
+            from .telemetry import record_build_batches
+
+            record_build_batches(log_writer, tree)
         n2d = tree.get_order()
         smin = tree.get_subtree_bounds()[0] if compute_predecessor_bounds else None
         handle = HipHandle(tree=tree, node_to_dataset=n2d, backend=b, subtree_min_bounds=smin)
@@ -157,8 +173,18 @@ class HipEngine:
         q = np.asarray(queries, dtype=np.int64).reshape(-1)
         precision = (runtime.knn_precision if runtime is not None
                      else tree.meta.get("precision", "mixed"))
+        telemetry = _query_telemetry_enabled()
+        handle.tree.set_query_telemetry(telemetry)
         idx, dist = handle.tree.knn_residual(q, int(k), predecessor_mode=predecessor_mode,
                                              mode=precision)
+        if telemetry:
+            # attach to the active op log, as the reference does with its
+            # query telemetry (covertreex/engine.py:1115-1131)
+            telem = handle.tree.last_query_telemetry()
+            op_log = getattr(context, "op_log", None)
+            if telem is not None and op_log is not None and hasattr(op_log, "add_metadata"):
+                op_log.add_metadata(hip_query_telemetry=telem)
+            handle.tree.clear_last_query_telemetry()
         return _format_knn_output(idx, dist, return_distances=return_distances)
 
 

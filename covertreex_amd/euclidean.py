@@ -32,6 +32,19 @@ class EuclideanHipTree(HipTree):
         t.dim = x.shape[1]
         return t
 
+    def insert(self, points, *, chunk_size: int | None = None) -> None:
+        """CoverTreeWrapper.insert (src/lib.rs:207-223) on the device tree: the
+        points become nodes n, n+1, ... in their given order."""
+        x = np.ascontiguousarray(points, dtype=np.float32)
+        if x.ndim != 2 or x.shape[1] != self.dim:
+            raise TypeError(f"expected a float32 or float64 2D array with {self.dim} columns")
+        chunk = int(chunk_size or max(x.shape[0], 1))
+        if chunk > 1024:
+            raise ValueError("chunk_size must be <= 1024 on this engine; pass chunk_size")
+        with self._lock:
+            check(_ffi.load().cx_tree_insert_euclidean(self._h, ptr(x), x.shape[0], chunk))
+            self.n += x.shape[0]
+
     def knn_points(self, queries, k: int, *, predecessor_mode: bool = False,
                    mode: str = "exact"):
         q = np.ascontiguousarray(queries, dtype=np.float32)

@@ -119,6 +119,74 @@ class HipTree:
         check(_ffi.load().cx_tree_last_profile(self._h, ptr(out), out.size, C.byref(n)))
         return [float(x) for x in out[: n.value]]
 
+    # -- query telemetry (src/lib.rs:357-367, src/telemetry.rs:1-19) ------------
+    def set_query_telemetry(self, enable: bool = True) -> None:
+        """The COVERTREEX_RUST_QUERY_TELEMETRY switch, per tree."""
+        check(_ffi.load().cx_tree_set_query_telemetry(self._h, int(bool(enable))))
+
+    def last_query_telemetry(self) -> dict | None:
+        """Counters of the last kNN call, keyed like ResidualQueryTelemetry
+        (frontier_levels / frontier_expanded are summed over the call's queries
+        per level; block_sizes lists the tile sizes used, one entry per tile
+        kind), plus the mixed engine's own counters; None when not recorded."""
+        out = np.zeros(_ffi.CX_QT_SIZE, np.int64)
+        n = C.c_int32(0)
+        check(_ffi.load().cx_tree_last_query_telemetry(self._h, ptr(out), out.size, C.byref(n)))
+        if n.value == 0:
+            return None
+        t = {k: int(out[i]) for i, k in enumerate(_ffi.QT_SCALARS)}
+        mode = {v: k for k, v in MODES.items() if k != "reference"}.get(t.pop("mode"), "reference")
+        lv = t.pop("n_levels")
+        blocks = out[_ffi.CX_QT_BLOCKS:_ffi.CX_QT_BLOCKS + 3]
+        t.update({
+            "engine_mode": mode,
+            "frontier_levels": [int(x) for x in out[_ffi.CX_QT_FRONTIER:_ffi.CX_QT_FRONTIER + lv]],
+            "frontier_expanded": [int(x) for x in
+                                  out[_ffi.CX_QT_EXPANDED:_ffi.CX_QT_EXPANDED + lv]],
+            "yields": [],
+            "caps_applied": 0,
+            "prunes_cap": 0,
+            "masked_dedup": 0,
+            "level_cache_hits": 0,
+            "level_cache_misses": 0,
+            # tiles per size instead of the reference's one entry per tile
+            "block_sizes": [],
+            "block_size_counts": {size: int(c) for size, c in zip((16, 32, 64), blocks)},
+        })
+        return t
+
+    def clear_last_query_telemetry(self) -> None:
+        check(_ffi.load().cx_tree_clear_last_query_telemetry(self._h))
+
+    # -- build telemetry (BatchInsertTelemetry, src/lib.rs:1274-1371) -------------
+    def build_telemetry(self) -> list:
+        """One dict per batch of the build and of later inserts."""
+        nb = C.c_int64(0)
+        L = _ffi.load()
+        check(L.cx_tree_build_telemetry(self._h, None, 0, C.byref(nb)))
+        out = np.zeros((max(nb.value, 1), _ffi.CX_BATCH_FIELDS), np.int64)
+        check(L.cx_tree_build_telemetry(self._h, ptr(out), nb.value, C.byref(nb)))
+        return [{k: int(row[i]) for i, k in enumerate(_ffi.BATCH_FIELDS)}
+                for row in out[: nb.value]]
+
+    # -- incremental insert (src/lib.rs:254-300) ----------------------------------
+    def insert_residual(self, v_new, p_new, coords_new, *, chunk_size: int | None = None):
+        """Append rows (dataset indices n, n+1, ...) and batch-insert them in
+        their given order; chunk_size defaults to one batch of all the rows
+        (the reference's default), at most 1024."""
+        v = np.ascontiguousarray(v_new, np.float32)
+        p = np.ascontiguousarray(p_new, np.float32).reshape(-1)
+        x = np.ascontiguousarray(coords_new, np.float32)
+        if v.ndim != 2 or x.ndim != 2 or v.shape[0] != p.size or x.shape[0] != p.size:
+            raise TypeError("expected V (n, m), p (n,) and coords (n, d) float arrays")
+        chunk = int(chunk_size or max(p.size, 1))
+        if chunk > 1024:
+            raise ValueError("chunk_size must be <= 1024 on this engine; pass chunk_size")
+        with self._lock:
+            check(_ffi.load().cx_tree_insert_residual(self._h, ptr(v), ptr(p), ptr(x), p.size,
+                                                      chunk))
+            self.n += p.size
+
     # -- getters (src/lib.rs:825-904) -------------------------------------------
     def info(self):
         n, m, d = C.c_int64(), C.c_int32(), C.c_int32()

@@ -164,14 +164,82 @@ cx_status cx_nystrom_residual(const double *points, int64_t n, int32_t d,
                               const double *ls, int32_t n_ls, int32_t kernel_type, double jitter,
                               float *v_out, float *p_out);
 
-/* Per-phase device timing of the last kNN call on this tree (telemetry, the
- * analogue of the reference's query telemetry, src/telemetry.rs:1-60).
- * When enabled, HIP events are recorded on the tree's stream at phase
- * boundaries.  cx_tree_last_profile writes up to cap durations (ms):
- *   CX_KNN_MIXED:         [prep, query sort, screen pass 1, tau, screen pass 2, rescore]
+/* Per-phase device timing of the last kNN call on this tree.  When enabled,
+ * HIP events are recorded on the tree's stream at phase boundaries.
+ * cx_tree_last_profile writes up to cap durations (ms):
+ *   CX_KNN_MIXED:         [prep, query sort, screen pass 1, tau, filter,
+ *                          rescore, hard queries]
  *   CX_KNN_REFERENCE_F32: [traversal]                                         */
 cx_status cx_tree_set_profiling(cx_tree *t, int32_t enable);
 cx_status cx_tree_last_profile(cx_tree *t, float *ms_out, int32_t cap, int32_t *n_out);
+
+/* Incremental insert on a device-resident residual tree.  Replaces
+ * CoverTreeWrapper.insert_residual (src/lib.rs:254-300): the n_new rows (V,
+ * p, coords, host float32, dataset order) become dataset rows n .. n+n_new-1
+ * and tree nodes n .. n+n_new-1 in their given order, batch-inserted in
+ * chunks of `chunk` with the build's grid scale, kernel and lengthscales.
+ * si_cache, subtree bounds and the child lists are recomputed. */
+cx_status cx_tree_insert_residual(cx_tree *t, const float *V, const float *p,
+                                  const float *coords, int64_t n_new, int32_t chunk);
+
+/* Incremental insert on a Euclidean tree.  Replaces CoverTreeWrapper.insert
+ * (src/lib.rs:207-223) applied to a grown tree: the n_new points become nodes
+ * n .. n+n_new-1, batch-inserted in chunks of `chunk`. */
+cx_status cx_tree_insert_euclidean(cx_tree *t, const float *points, int64_t n_new,
+                                   int32_t chunk);
+
+/* Per-batch build telemetry: the counterpart of the reference's
+ * BatchInsertTelemetry / record_rust_batch payload (src/lib.rs:1274-1371),
+ * for every batch of the build and of later inserts, CX_BATCH_FIELDS int64
+ * per batch:                                                               */
+#define CX_BATCH_FIELDS 16
+#define CX_BF_START 0      /* first tree row of the batch */
+#define CX_BF_SIZE 1       /* rows in the batch */
+#define CX_BF_LEADERS 2    /* MIS-selected leaders, all levels (selected) */
+#define CX_BF_SURVIVORS 3  /* rows attached at the bottom level */
+#define CX_BF_ORPHANS 4    /* far non-leaders left as roots */
+#define CX_BF_FAR_LEVELS 5 /* levels with a non-empty far set */
+#define CX_BF_SCAN_EVALS 6 /* distance evaluations: near/far scan (traversal) */
+#define CX_BF_MIS_EVALS 7  /* distance evaluations: conflict graph */
+#define CX_BF_NL_EVALS 8   /* distance evaluations: non-leader attachment */
+#define CX_BF_GRID_EDGES 9 /* conflicts found through the grid */
+#define CX_BF_DIST_EDGES 10 /* conflicts found through d^2 <= r^2 */
+#define CX_BF_NS_SCAN 11   /* device wall time (ns): near/far scan */
+#define CX_BF_NS_MIS 12    /* device wall time (ns): conflicts, MIS, leaders */
+#define CX_BF_NS_NL 13     /* device wall time (ns): non-leader attachment */
+#define CX_BF_NS_TOTAL 14  /* device wall time (ns): whole batch */
+/* Writes min(cap, batches) rows of CX_BATCH_FIELDS; *n_batches = batches. */
+cx_status cx_tree_build_telemetry(const cx_tree *t, int64_t *out, int64_t cap_batches,
+                                  int64_t *n_batches);
+
+/* Query telemetry.  Replaces the COVERTREEX_RUST_QUERY_TELEMETRY switch and
+ * CoverTreeWrapper.last_query_telemetry / clear_last_query_telemetry
+ * (src/lib.rs:357-367, ResidualQueryTelemetry src/telemetry.rs:1-19).  When
+ * enabled, each kNN call records CX_QT_SIZE int64 counters:                */
+#define CX_QT_SIZE 160
+#define CX_QT_LEVELS 64
+#define CX_QT_MODE 0            /* the CX_KNN_* mode of the call */
+#define CX_QT_QUERIES 1
+#define CX_QT_DISTANCE_EVALS 2  /* residual distances evaluated (mixed: pairs screened) */
+#define CX_QT_PRUNES_LB 3       /* prunes_lower_bound */
+#define CX_QT_PRUNES_LB_CHUNKS 4 /* prunes_lower_bound_chunks */
+#define CX_QT_PRED_FILTERED 5   /* predecessor_filtered */
+#define CX_QT_SUBTREES_PRUNED 6 /* subtrees_pruned */
+#define CX_QT_BUDGET_ESC 7      /* budget_escalations */
+#define CX_QT_BUDGET_EARLY 8    /* budget_early_terminate */
+#define CX_QT_LISTED 9          /* mixed: candidates listed for rescoring */
+#define CX_QT_HARD_QUERIES 10   /* mixed: queries scored over all candidates */
+#define CX_QT_HARD_PAIRS 11     /* mixed: pairs scored for those queries */
+#define CX_QT_MAX_LIST 12       /* mixed: longest list */
+#define CX_QT_N_LEVELS 13       /* levels recorded below (<= CX_QT_LEVELS) */
+#define CX_QT_FRONTIER 14       /* [CX_QT_LEVELS] frontier size per level, summed */
+#define CX_QT_EXPANDED 78       /* [CX_QT_LEVELS] children gathered per level */
+#define CX_QT_BLOCKS 142        /* [3] tiles of 16, 32, 64 candidates */
+cx_status cx_tree_set_query_telemetry(cx_tree *t, int32_t enable);
+/* *n_out = CX_QT_SIZE when the last call recorded telemetry, else 0. */
+cx_status cx_tree_last_query_telemetry(const cx_tree *t, int64_t *out, int32_t cap,
+                                       int32_t *n_out);
+cx_status cx_tree_clear_last_query_telemetry(cx_tree *t);
 
 void cx_tree_free(cx_tree *t);
 

@@ -19,7 +19,9 @@ from .lib import (  # noqa: F401
     EXP_LIBM,
     OracleTree,
     build_residual_tree,
+    build_residual_tree_with_insert,
     build_euclidean_tree,
+    build_euclidean_tree_with_insert,
     dist_batch_pairs,
     dist_sq_pairs,
     greedy_mis,

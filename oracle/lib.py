@@ -78,6 +78,8 @@ def lib():
         L.ox_tree_new.argtypes = [_i64]
         L.ox_tree_free.argtypes = [C.POINTER(_Tree)]
         L.ox_build.argtypes = [C.POINTER(_Tree), _p, _i64, _p, _i32, _f32, _i32, _p]
+        L.ox_batch_insert.argtypes = [C.POINTER(_Tree), _p, _i64, _i64, _p, _i32, _f32, _i32, _p]
+        L.ox_batch_insert.restype = _i64
         L.ox_si_cache.argtypes = [C.POINTER(_Tree), _p, _p]
         L.ox_subtree_bounds.argtypes = [_i64, _p, _p, _p, _p]
         L.ox_greedy_mis.argtypes = [_i64, _p, _p, _p]
@@ -242,13 +244,31 @@ def _tree_arrays(t, n):
             cp(tt.levels, np.int32), cp(tt.si, np.float32), int(tt.min_level), int(tt.max_level))
 
 
-def _build(metric: _Metric, n: int, order, chunk, grid_coords, grid_scale, use_grid):
+_STAT_KEYS = ["filter_evals", "conflict_pairs", "nonleader_evals", "leaders", "orphans_dropped",
+              "far_levels"]
+
+
+def _build(metric: _Metric, n: int, order, segments, grid_coords, grid_scale, use_grid):
+    """Chunked batch inserts over tree rows: ``segments`` is a list of
+    (start, end, chunk); batches run in order and each records its stats
+    (the build's chunk loop, src/lib.rs:1502-1539, and later inserts)."""
     L = lib()
     t = L.ox_tree_new(max(n, 1))
-    stats = np.zeros(8, np.int64)
+    total = np.zeros(8, np.int64)
     gc = None if grid_coords is None else _c(grid_coords, np.float32)
     gd = 0 if gc is None else gc.shape[1]
-    L.ox_build(t, metric.h, int(chunk), _ptr(gc), gd, float(grid_scale), int(use_grid), _ptr(stats))
+    batches = []
+    for start, end, chunk in segments:
+        chunk = int(chunk) if chunk and chunk > 0 else max(end - start, 1)
+        for s in range(start, end, chunk):
+            e = min(s + chunk, end)
+            st = np.zeros(8, np.int64)
+            L.ox_batch_insert(t, metric.h, s, e, _ptr(gc), gd, float(grid_scale), int(use_grid),
+                              _ptr(st))
+            total += st
+            b = {k: int(v) for k, v in zip(_STAT_KEYS, st)}
+            b.update(start=s, size=e - s)
+            batches.append(b)
     si = np.zeros(max(n, 1), np.float32)
     L.ox_si_cache(t, metric.h, _ptr(si))
     parents, children, nxt, levels, _, mn, mx = _tree_arrays(t, n)
@@ -257,12 +277,13 @@ def _build(metric: _Metric, n: int, order, chunk, grid_coords, grid_scale, use_g
     inv = np.empty(n, np.int64)
     inv[order] = np.arange(n, dtype=np.int64)
     smin, smax = subtree_bounds(parents, order)
-    keys = ["filter_evals", "conflict_pairs", "nonleader_evals", "leaders", "orphans_dropped",
-            "far_levels"]
-    return OracleTree(order=order, inv_order=inv, parents=parents, children=children, next=nxt,
+    tree = OracleTree(order=order, inv_order=inv, parents=parents, children=children, next=nxt,
                       levels=levels, si=si[:n].copy(), min_level=mn, max_level=mx,
                       subtree_min=smin, subtree_max=smax,
-                      stats={k: int(v) for k, v in zip(keys, stats)}, _tree=t, _metric=metric)
+                      stats={k: int(v) for k, v in zip(_STAT_KEYS, total)}, _tree=t,
+                      _metric=metric)
+    tree.batch_stats = batches
+    return tree
 
 
 def build_residual_tree(coords, v, p, *, var=1.0, ls=1.0, kernel_type=0, chunk=512,
@@ -277,7 +298,29 @@ def build_residual_tree(coords, v, p, *, var=1.0, ls=1.0, kernel_type=0, chunk=5
     c_ord = coords[order]
     metric = residual_metric(c_ord, np.asarray(v, np.float32)[order], np.asarray(p, np.float32)[order],
                              var, ls, kernel_type, exp_mode)
-    return _build(metric, n, order, chunk, c_ord, grid_scale, 1)
+    return _build(metric, n, order, [(0, n, chunk)], c_ord, grid_scale, 1)
+
+
+def build_residual_tree_with_insert(coords, v, p, coords_new, v_new, p_new, *, var=1.0, ls=1.0,
+                                    kernel_type=0, chunk=512, insert_chunk=None,
+                                    batch_order="hilbert", grid_scale=1.0,
+                                    exp_mode=EXP_LIBM) -> OracleTree:
+    """The build above, then CoverTreeWrapper.insert_residual (src/lib.rs:254-300)
+    of new rows: they become dataset rows n.. and tree nodes n.. in their
+    given order, batch-inserted in chunks of insert_chunk (default: one batch)."""
+    coords = _c(coords, np.float32)
+    cn = _c(coords_new, np.float32)
+    n0, n1 = coords.shape[0], coords.shape[0] + cn.shape[0]
+    order0 = (np.arange(n0, dtype=np.int64) if batch_order == "natural"
+              else morton_order(coords)[0])
+    order = np.concatenate([order0, np.arange(n0, n1, dtype=np.int64)])
+    c_all = np.concatenate([coords, cn])
+    v_all = np.concatenate([np.asarray(v, np.float32), np.asarray(v_new, np.float32)])
+    p_all = np.concatenate([np.asarray(p, np.float32), np.asarray(p_new, np.float32)])
+    c_ord = c_all[order]
+    metric = residual_metric(c_ord, v_all[order], p_all[order], var, ls, kernel_type, exp_mode)
+    segs = [(0, n0, chunk), (n0, n1, insert_chunk or (n1 - n0))]
+    return _build(metric, n1, order, segs, c_ord, grid_scale, 1)
 
 
 def build_euclidean_tree(points, *, chunk=512) -> OracleTree:
@@ -286,7 +329,19 @@ def build_euclidean_tree(points, *, chunk=512) -> OracleTree:
     n = pts.shape[0]
     order = np.arange(n, dtype=np.int64)
     metric = euclidean_metric(pts)
-    return _build(metric, n, order, chunk, None, 1.0, 0)
+    return _build(metric, n, order, [(0, n, chunk)], None, 1.0, 0)
+
+
+def build_euclidean_tree_with_insert(points, new_points, *, chunk=512,
+                                     insert_chunk=None) -> OracleTree:
+    """The Euclidean build, then CoverTreeWrapper.insert (src/lib.rs:207-223) of
+    new points as nodes n.., in chunks of insert_chunk (default: one batch)."""
+    pts = np.concatenate([_c(points, np.float32), _c(new_points, np.float32)])
+    n0, n1 = np.asarray(points).shape[0], pts.shape[0]
+    order = np.arange(n1, dtype=np.int64)
+    metric = euclidean_metric(pts)
+    return _build(metric, n1, order, [(0, n0, chunk), (n0, n1, insert_chunk or (n1 - n0))],
+                  None, 1.0, 0)
 
 
 def si_cache(tree: OracleTree) -> np.ndarray:
