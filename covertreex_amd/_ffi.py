@@ -78,6 +78,7 @@ SIGNATURES = {
     "cx_tree_last_profile": (_i32, [_p, _p, _i32, _p]),
     "cx_tree_insert_residual": (_i32, [_p, _p, _p, _p, _i64, _i32]),
     "cx_tree_insert_euclidean": (_i32, [_p, _p, _i64, _i32]),
+    "cx_tree_delete": (_i32, [_p, _p, _i64]),
     "cx_tree_build_telemetry": (_i32, [_p, _p, _i64, _p]),
     "cx_tree_set_query_telemetry": (_i32, [_p, _i32]),
     "cx_tree_last_query_telemetry": (_i32, [_p, _p, _i32, _p]),
@@ -92,6 +93,10 @@ class HipError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"covertreex_hip error {status}: {message}")
         self.status = status
+
+
+class HipValueError(HipError, ValueError):
+    """CX_ERR_INVALID: an argument the library rejected."""
 
 
 _lib = None
@@ -119,7 +124,8 @@ def load(path: str | None = None):
 def check(status: int) -> None:
     if status != CX_OK:
         msg = load().cx_last_error()
-        raise HipError(int(status), msg.decode() if msg else "")
+        cls = HipValueError if status == CX_ERR_INVALID else HipError
+        raise cls(int(status), msg.decode() if msg else "")
 
 
 def ptr(a: np.ndarray | None):

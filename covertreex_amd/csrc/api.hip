@@ -494,6 +494,21 @@ cx_status cx_tree_insert_euclidean(cx_tree *t, const float *points, int64_t n_ne
     });
 }
 
+cx_status cx_tree_delete(cx_tree *t, const int64_t *indices, int64_t n_remove)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        CX_REQUIRE(n_remove >= 0 && (n_remove == 0 || indices), "NULL indices");
+        std::lock_guard<std::mutex> lock(t->s.mu);
+        TreeState &s = t->s;
+        CX_REQUIRE(n_remove < s.n, "cannot delete every point of a tree");
+        CX_HIP(hipSetDevice(s.device));
+        if (n_remove == 0)
+            return;
+        delete_rows(s, indices, n_remove, s.metric == 1);
+    });
+}
+
 cx_status cx_tree_build_telemetry(const cx_tree *t, int64_t *out, int64_t cap_batches,
                                   int64_t *n_batches)
 {

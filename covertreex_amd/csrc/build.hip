@@ -16,6 +16,8 @@
 // (device_common.hpp) so every comparison matches the reference bit for bit.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "cx_internal.hpp"
 #include "device_common.hpp"
 
@@ -656,13 +658,13 @@ static int32_t compute_depths(const int64_t *parents, int64_t n, int32_t *depth,
     return h;
 }
 
-// Runs the batch insert of rows [s_begin, t.n) and appends the per-batch
-// telemetry to t.batch_stats.
+// Runs the batch insert of rows [s_begin, s_end) (default: to t.n) and
+// appends the per-batch telemetry to t.batch_stats.
 template <int KIND>
 static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, float grid_scale,
-                       int64_t s_begin, int32_t chunk, hipStream_t st)
+                       int64_t s_begin, int32_t chunk, hipStream_t st, int64_t s_end = -1)
 {
-    const int64_t n = t.n;
+    const int64_t n = s_end < 0 ? t.n : s_end;
     if (s_begin >= n)
         return;
     const int64_t nb = (n - s_begin + chunk - 1) / chunk;
@@ -879,6 +881,169 @@ void insert_euclidean_rows(TreeState &t, const float *points, int64_t n_new, int
     const MetricView mv{t.raw.ptr, nullptr, nullptr, nullptr, d, 0, 1.0f, 0};
     run_insert<kMetricEuclidean>(t, mv, 0, 1.0f, n0, chunk, st);
     finalize_tree<kMetricEuclidean>(t, mv, st);
+}
+
+// ---- delete -------------------------------------------------------------------
+
+// Removes dataset rows from a device-resident tree, following the reference's
+// batch_delete (covertreex/algo/batch_delete.py:105-525): every descendant of
+// a removed node is detached, the remaining tree keeps its links (each kept
+// node's ancestors are kept), and the detached nodes are inserted again in
+// groups of equal max(level, 0), lowest level first, each group in
+// batches of t.chunk.  Here the re-insertion is this tree's batch insert.
+// Remaining dataset rows are renumbered in their order (removed rows close
+// up); tree order becomes [kept nodes in tree order, detached nodes by group].
+// `remove` holds dataset rows, or tree nodes when by_node.
+void delete_rows(TreeState &t, const int64_t *remove, int64_t n_remove, bool by_node)
+{
+    hipStream_t st = t.stream;
+    const int64_t n = t.n;
+    std::vector<int64_t> par(n), chi(n), nxt(n), ord(n), inv(n);
+    std::vector<int32_t> lev(n);
+    CX_HIP(hipMemcpyAsync(par.data(), t.parents.ptr, 8 * n, hipMemcpyDeviceToHost, st));
+    CX_HIP(hipMemcpyAsync(chi.data(), t.children.ptr, 8 * n, hipMemcpyDeviceToHost, st));
+    CX_HIP(hipMemcpyAsync(nxt.data(), t.next.ptr, 8 * n, hipMemcpyDeviceToHost, st));
+    CX_HIP(hipMemcpyAsync(ord.data(), t.order.ptr, 8 * n, hipMemcpyDeviceToHost, st));
+    CX_HIP(hipMemcpyAsync(inv.data(), t.inv_order.ptr, 8 * n, hipMemcpyDeviceToHost, st));
+    CX_HIP(hipMemcpyAsync(lev.data(), t.levels.ptr, 4 * n, hipMemcpyDeviceToHost, st));
+    CX_HIP(hipStreamSynchronize(st));
+
+    // 1 removed, 2 detached (a descendant of a removed node)
+    std::vector<uint8_t> state(n, 0), removed_ds(n, 0);
+    std::vector<int64_t> stack;
+    for (int64_t i = 0; i < n_remove; i++) {
+        const int64_t x = remove[i];
+        CX_REQUIRE(x >= 0 && x < n, "Deletion index out of range.");
+        const int64_t r = by_node ? ord[x] : x;
+        CX_REQUIRE(!removed_ds[r], "Duplicate deletion indices are not allowed.");
+        removed_ds[r] = 1;
+        state[inv[r]] = 1;
+        stack.push_back(inv[r]);
+    }
+    auto for_children = [&](int64_t u, auto &&fn) {
+        const int64_t first = chi[u];
+        for (int64_t c = first, steps = 0; c != -1 && steps < n; steps++) {
+            fn(c);
+            const int64_t nx = nxt[c];
+            if (nx == c || nx == first)
+                break;
+            c = nx;
+        }
+    };
+    while (!stack.empty()) {
+        const int64_t u = stack.back();
+        stack.pop_back();
+        for_children(u, [&](int64_t c) {
+            if (state[c] == 0) {
+                state[c] = 2;
+                stack.push_back(c);
+            }
+        });
+    }
+    // new tree positions: kept nodes in tree order, then detached nodes by
+    // (max(level, 0), tree order)
+    std::vector<int64_t> newpos(n, -1), src;
+    src.reserve(n);
+    for (int64_t u = 0; u < n; u++)
+        if (state[u] == 0) {
+            newpos[u] = (int64_t)src.size();
+            src.push_back(u);
+        }
+    const int64_t kept = (int64_t)src.size();
+    std::vector<int64_t> det;
+    for (int64_t u = 0; u < n; u++)
+        if (state[u] == 2)
+            det.push_back(u);
+    auto glevel = [&](int64_t u) { return lev[u] > 0 ? lev[u] : 0; };
+    std::stable_sort(det.begin(), det.end(),
+                     [&](int64_t a, int64_t b) { return glevel(a) < glevel(b); });
+    std::vector<int64_t> group_end;
+    for (size_t i = 0; i < det.size(); i++) {
+        newpos[det[i]] = (int64_t)src.size();
+        src.push_back(det[i]);
+        if (i + 1 == det.size() || glevel(det[i + 1]) != glevel(det[i]))
+            group_end.push_back((int64_t)src.size());
+    }
+    const int64_t n1 = (int64_t)src.size();
+    // dataset rows close up over the removed ones
+    std::vector<int64_t> below(n + 1, 0);
+    for (int64_t r = 0; r < n; r++)
+        below[r + 1] = below[r] + removed_ds[r];
+    std::vector<int64_t> ord1(n1), par1(n1, -1), chi1(n1, -1), nxt1(n1, -1);
+    std::vector<int32_t> lev1(n1, INT32_MIN);
+    for (int64_t i = 0; i < n1; i++)
+        ord1[i] = ord[src[i]] - below[ord[src[i]]];
+    for (int64_t i = 0; i < kept; i++) {
+        const int64_t u = src[i];
+        par1[i] = par[u] >= 0 ? newpos[par[u]] : -1;
+        lev1[i] = lev[u];
+        int64_t last = -1;
+        for_children(u, [&](int64_t c) {
+            if (state[c] != 0)
+                return;
+            if (last < 0)
+                chi1[i] = newpos[c];
+            else
+                nxt1[last] = newpos[c];
+            last = newpos[c];
+        });
+    }
+
+    // device arrays in the new tree order
+    DevBuf<int64_t> sel(n1 > 0 ? n1 : 1);
+    CX_HIP(hipMemcpyAsync(sel.ptr, src.data(), 8 * n1, hipMemcpyHostToDevice, st));
+    auto regather = [&](DevBuf<float> &b, int32_t w) {
+        if (!b.ptr)
+            return;
+        DevBuf<float> nb((size_t)(n1 > 0 ? n1 : 1) * w);
+        if (n1 > 0)
+            gather_rows_kernel<<<(unsigned)n1, 64, 0, st>>>(b.ptr, sel.ptr, n1, w, nb.ptr);
+        CX_HIP(hipGetLastError());
+        CX_HIP(hipStreamSynchronize(st));
+        b = std::move(nb);
+    };
+    regather(t.raw, t.d);
+    if (t.metric == 0) {
+        regather(t.v, t.m);
+        regather(t.scaled, t.d);
+        regather(t.norms, 1);
+        regather(t.p, 1);
+    }
+    auto upload = [&](auto &b, const auto &h) {
+        b.alloc(n1 > 0 ? n1 : 1);
+        CX_HIP(hipMemcpyAsync(b.ptr, h.data(), sizeof(h[0]) * n1, hipMemcpyHostToDevice, st));
+    };
+    upload(t.order, ord1);
+    upload(t.parents, par1);
+    upload(t.children, chi1);
+    upload(t.next, nxt1);
+    upload(t.levels, lev1);
+    t.inv_order.alloc(n1 > 0 ? n1 : 1);
+    if (n1 > 0)
+        inverse_perm_kernel<<<nblocks(n1, 256), 256, 0, st>>>(t.order.ptr, n1, t.inv_order.ptr);
+    CX_HIP(hipGetLastError());
+    CX_HIP(hipStreamSynchronize(st));
+    t.n = n1;
+    t.mixed_ready = false;
+
+    // re-insert the detached groups, then the derived arrays
+    if (t.metric == 0) {
+        const MetricView mv = residual_view(t);
+        int64_t g0 = kept;
+        for (int64_t g1 : group_end) {
+            run_insert<kMetricResidual>(t, mv, 1, t.grid_scale, g0, t.chunk, st, g1);
+            g0 = g1;
+        }
+        finalize_tree<kMetricResidual>(t, mv, st);
+    } else {
+        const MetricView mv{t.raw.ptr, nullptr, nullptr, nullptr, t.d, 0, 1.0f, 0};
+        int64_t g0 = kept;
+        for (int64_t g1 : group_end) {
+            run_insert<kMetricEuclidean>(t, mv, 0, 1.0f, g0, t.chunk, st, g1);
+            g0 = g1;
+        }
+        finalize_tree<kMetricEuclidean>(t, mv, st);
+    }
 }
 
 void morton_order_device(const float *coords_host, int64_t n, int32_t d, int64_t *order_host)

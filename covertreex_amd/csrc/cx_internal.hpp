@@ -195,6 +195,7 @@ void build_euclidean_tree(TreeState &t, const float *points, int32_t chunk);
 void insert_residual_rows(TreeState &t, const float *V, const float *p, const float *coords,
                           int64_t n_new, int32_t chunk);
 void insert_euclidean_rows(TreeState &t, const float *points, int64_t n_new, int32_t chunk);
+void delete_rows(TreeState &t, const int64_t *remove, int64_t n_remove, bool by_node);
 void morton_order_device(const float *coords_host, int64_t n, int32_t d, int64_t *order_host);
 void subtree_bounds_device(const int64_t *parents_host, const int64_t *n2d_host, int64_t n,
                            int64_t *min_host, int64_t *max_host);

@@ -187,6 +187,16 @@ class HipTree:
                                                       chunk))
             self.n += p.size
 
+    # -- delete (covertreex/algo/batch_delete.py:628-689) --------------------------
+    def delete(self, indices) -> None:
+        """Remove points (dataset rows of a residual tree; node indices of a
+        Euclidean tree): their descendants are re-inserted, remaining rows are
+        renumbered in order.  ValueError on duplicates or out-of-range rows."""
+        idx = np.ascontiguousarray(np.asarray(indices, dtype=np.int64).reshape(-1))
+        with self._lock:
+            check(_ffi.load().cx_tree_delete(self._h, ptr(idx), idx.size))
+            self.n -= idx.size
+
     # -- getters (src/lib.rs:825-904) -------------------------------------------
     def info(self):
         n, m, d = C.c_int64(), C.c_int32(), C.c_int32()

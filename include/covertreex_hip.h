@@ -188,6 +188,19 @@ cx_status cx_tree_insert_residual(cx_tree *t, const float *V, const float *p,
 cx_status cx_tree_insert_euclidean(cx_tree *t, const float *points, int64_t n_new,
                                    int32_t chunk);
 
+/* Point deletion on a device-resident tree.  Replaces batch_delete
+ * (covertreex/algo/batch_delete.py:628-689, computation 105-525): the points
+ * and every descendant of them are detached, the rest of the tree keeps its
+ * links, and the detached points (not the removed ones) are inserted again
+ * in groups of equal max(level, 0), lowest first, each in batches of the
+ * build's chunk (this library's batch insert, not the reference's python
+ * planner).  indices: dataset rows for a residual tree, node indices (as the
+ * Euclidean kNN returns them) for a Euclidean tree; distinct, in range, fewer
+ * than the tree's size.  Remaining dataset rows are renumbered in order (the
+ * removed rows close up); tree order becomes [kept nodes in their order,
+ * re-inserted nodes by group]. */
+cx_status cx_tree_delete(cx_tree *t, const int64_t *indices, int64_t n_remove);
+
 /* Per-batch build telemetry: the counterpart of the reference's
  * BatchInsertTelemetry / record_rust_batch payload (src/lib.rs:1274-1371),
  * for every batch of the build and of later inserts, CX_BATCH_FIELDS int64

@@ -11,7 +11,6 @@ Contents
   reference's f32 evaluation order (see ``oracle/oracle.h``).
 - :mod:`oracle.fp64`: FP64 NumPy Nystrom factor and FP64 brute-force residual
   kNN evaluated on the same stored FP32 inputs.
-- :mod:`oracle.invariants`: cover-tree structure checks.
 """
 
 from .lib import (  # noqa: F401
@@ -22,6 +21,7 @@ from .lib import (  # noqa: F401
     build_residual_tree_with_insert,
     build_euclidean_tree,
     build_euclidean_tree_with_insert,
+    delete_rows,
     dist_batch_pairs,
     dist_sq_pairs,
     greedy_mis,

@@ -248,16 +248,14 @@ _STAT_KEYS = ["filter_evals", "conflict_pairs", "nonleader_evals", "leaders", "o
               "far_levels"]
 
 
-def _build(metric: _Metric, n: int, order, segments, grid_coords, grid_scale, use_grid):
+def _insert_segments(t, metric: _Metric, segments, grid_coords, grid_scale, use_grid):
     """Chunked batch inserts over tree rows: ``segments`` is a list of
-    (start, end, chunk); batches run in order and each records its stats
-    (the build's chunk loop, src/lib.rs:1502-1539, and later inserts)."""
+    (start, end, chunk); batches run in order and each records its stats."""
     L = lib()
-    t = L.ox_tree_new(max(n, 1))
-    total = np.zeros(8, np.int64)
     gc = None if grid_coords is None else _c(grid_coords, np.float32)
     gd = 0 if gc is None else gc.shape[1]
     batches = []
+    total = np.zeros(8, np.int64)
     for start, end, chunk in segments:
         chunk = int(chunk) if chunk and chunk > 0 else max(end - start, 1)
         for s in range(start, end, chunk):
@@ -269,6 +267,11 @@ def _build(metric: _Metric, n: int, order, segments, grid_coords, grid_scale, us
             b = {k: int(v) for k, v in zip(_STAT_KEYS, st)}
             b.update(start=s, size=e - s)
             batches.append(b)
+    return batches, total
+
+
+def _finish(t, metric: _Metric, n: int, order, batches, total) -> "OracleTree":
+    L = lib()
     si = np.zeros(max(n, 1), np.float32)
     L.ox_si_cache(t, metric.h, _ptr(si))
     parents, children, nxt, levels, _, mn, mx = _tree_arrays(t, n)
@@ -284,6 +287,112 @@ def _build(metric: _Metric, n: int, order, segments, grid_coords, grid_scale, us
                       _metric=metric)
     tree.batch_stats = batches
     return tree
+
+
+def _build(metric: _Metric, n: int, order, segments, grid_coords, grid_scale, use_grid):
+    """The build's chunk loop (src/lib.rs:1502-1539) and later inserts."""
+    t = lib().ox_tree_new(max(n, 1))
+    batches, total = _insert_segments(t, metric, segments, grid_coords, grid_scale, use_grid)
+    return _finish(t, metric, n, order, batches, total)
+
+
+def _child_walk(children, nxt, u, n):
+    """Children of u in list order (prepend lists, src/tree.rs:98-103)."""
+    out = []
+    c = int(children[u])
+    first = c
+    steps = 0
+    while c != -1 and steps < n:
+        out.append(c)
+        nx = int(nxt[c])
+        if nx == c or nx == first:
+            break
+        c = nx
+        steps += 1
+    return out
+
+
+def delete_rows(tree: "OracleTree", data, remove, *, v=None, p=None, var=1.0, ls=1.0,
+                kernel_type=0, chunk=512, grid_scale=1.0, exp_mode=EXP_LIBM,
+                euclidean=False) -> "OracleTree":
+    """batch_delete's procedure (covertreex/algo/batch_delete.py:105-525) on a
+    tree of this oracle: removed nodes and all their descendants leave the
+    tree; the remaining links stay; the detached (not removed) nodes go back
+    in groups of equal max(level, 0), lowest first, each through chunked
+    batch inserts.  ``data`` (and v, p for a residual tree) are the tree's
+    dataset rows; ``remove`` holds dataset rows (residual) or node indices
+    (Euclidean).  Remaining dataset rows are renumbered in order; the new tree
+    order is [kept nodes, detached nodes by group]."""
+    order, inv = tree.order, tree.inv_order
+    parents, children, nxt, levels = tree.parents, tree.children, tree.next, tree.levels
+    n = parents.size
+    state = np.zeros(n, np.int8)
+    removed_ds = np.zeros(n, bool)
+    stack = []
+    for x in np.asarray(remove, np.int64).reshape(-1):
+        if x < 0 or x >= n:
+            raise ValueError("Deletion index out of range.")
+        r = int(order[x]) if euclidean else int(x)
+        if removed_ds[r]:
+            raise ValueError("Duplicate deletion indices are not allowed.")
+        removed_ds[r] = True
+        state[inv[r]] = 1
+        stack.append(int(inv[r]))
+    while stack:
+        u = stack.pop()
+        for c in _child_walk(children, nxt, u, n):
+            if state[c] == 0:
+                state[c] = 2
+                stack.append(c)
+    kept = [u for u in range(n) if state[u] == 0]
+    glevel = lambda u: max(int(levels[u]), 0)  # noqa: E731
+    det = sorted((u for u in range(n) if state[u] == 2), key=glevel)  # stable: tree order
+    src = np.asarray(kept + det, np.int64)
+    newpos = -np.ones(n, np.int64)
+    newpos[src] = np.arange(src.size)
+    n1, k0 = src.size, len(kept)
+    ends = [k0 + i + 1 for i in range(len(det)) if i + 1 == len(det)
+            or glevel(det[i + 1]) != glevel(det[i])]
+    below = np.concatenate([[0], np.cumsum(removed_ds)])
+    order1 = order[src] - below[order[src]]
+    par1 = np.full(n1, -1, np.int64)
+    chi1 = np.full(n1, -1, np.int64)
+    nxt1 = np.full(n1, -1, np.int64)
+    lev1 = np.full(n1, np.iinfo(np.int32).min, np.int32)
+    for i, u in enumerate(kept):
+        par1[i] = newpos[parents[u]] if parents[u] >= 0 else -1
+        lev1[i] = levels[u]
+        kids = [newpos[c] for c in _child_walk(children, nxt, u, n) if state[c] == 0]
+        if kids:
+            chi1[i] = kids[0]
+            for a, b in zip(kids[:-1], kids[1:]):
+                nxt1[a] = b
+    # dataset rows in the new tree order (the old tree order, then src)
+    rows = np.asarray(data, np.float32)[order][src]
+    if euclidean:
+        metric = euclidean_metric(rows)
+        gc, use_grid = None, 0
+    else:
+        metric = residual_metric(rows, np.asarray(v, np.float32)[order][src],
+                                 np.asarray(p, np.float32)[order][src], var, ls, kernel_type,
+                                 exp_mode)
+        gc, use_grid = rows, 1
+    L = lib()
+    t = L.ox_tree_new(max(n1, 1))
+    tt = t.contents
+    for dst, arr in ((tt.parents, par1), (tt.children, chi1), (tt.next, nxt1)):
+        C.memmove(dst, np.ascontiguousarray(arr).ctypes.data, 8 * n1)
+    C.memmove(tt.levels, np.ascontiguousarray(lev1).ctypes.data, 4 * n1)
+    # the tree keeps its level range: bottom survivors take min_level (mod.rs:791-800)
+    tt.min_level, tt.max_level = tree.min_level, tree.max_level
+    tt.n = k0
+    segs, g0 = [], k0
+    for g1 in ends:
+        segs.append((g0, g1, chunk))
+        g0 = g1
+    batches, total = _insert_segments(t, metric, segs, gc, grid_scale, use_grid)
+    tt.n = n1
+    return _finish(t, metric, n1, order1, batches, total)
 
 
 def build_residual_tree(coords, v, p, *, var=1.0, ls=1.0, kernel_type=0, chunk=512,

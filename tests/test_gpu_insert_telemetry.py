@@ -137,3 +137,56 @@ def test_engine_writes_batch_records(hip, tmp_path):
         assert rec["schema_id"] == SCHEMA_ID and rec["batch_index"] == i
         assert rec["selected"] == batches[i]["leaders"]
         assert rec["candidates"] == batches[i]["size"]
+
+
+@pytest.mark.parametrize("seed,nrem,gs", [(0, 60, 6.0), (1, 1, 1.0), (2, 700, 6.0)])
+def test_residual_delete_matches_oracle(hip, seed, nrem, gs):
+    from covertreex_amd.native import HipTree
+
+    x, v, p = _data(2500, 2, 32, 0, seed=seed)
+    ht = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), chunk_size=256,
+                                grid_whiten_scale=gs, kernel_type=0)
+    ot = oracle.build_residual_tree(x, v, p, kernel_type=0, chunk=256, grid_scale=gs,
+                                    exp_mode=oracle.EXP_CR)
+    rem = np.random.default_rng(seed + 10).choice(2500, nrem, replace=False)
+    ht.delete(rem)
+    od = oracle.delete_rows(ot, x, rem, v=v, p=p, kernel_type=0, chunk=256, grid_scale=gs,
+                            exp_mode=oracle.EXP_CR)
+    assert ht.n == 2500 - nrem
+    _same_tree(ht, od)
+    keep = np.setdiff1d(np.arange(2500), rem)
+    xk, vk, pk = x[keep], v[keep], p[keep]
+    q = np.arange(0, keep.size, 5)
+    oi, odist, _ = oracle.knn_residual(od, q, 12, predecessor=False, budget=())
+    hi, hd = ht.knn_residual(q, 12, predecessor_mode=False, mode="reference")
+    assert np.array_equal(hd.view(np.uint32), odist.view(np.uint32))
+    mi, md = ht.knn_residual(q, 12, predecessor_mode=True, mode="mixed")
+    bi, bd, _ = fp64.brute_knn_fp64(xk, vk, pk, q, 12, predecessor=True, kernel_type=0)
+    for r in range(q.size):
+        got, ref = mi[r][mi[r] >= 0], bi[r][bi[r] >= 0]
+        assert got.size == ref.size
+        assert np.all(np.abs(md[r][: got.size] - bd[r][: ref.size]) <= 1e-5)
+    with pytest.raises(ValueError):
+        ht.delete([0, 0])
+
+
+def test_euclidean_delete_matches_oracle(hip):
+    from covertreex_amd.euclidean import EuclideanHipTree
+
+    rng = np.random.default_rng(8)
+    pts = rng.random((3000, 4)).astype(np.float32)
+    ht = EuclideanHipTree.build(pts, chunk_size=512)
+    ot = oracle.build_euclidean_tree(pts, chunk=512)
+    rem = rng.choice(3000, 250, replace=False)
+    ht.delete(rem)
+    od = oracle.delete_rows(ot, pts, rem, chunk=512, euclidean=True)
+    assert np.array_equal(ht.get_order(), od.order)
+    assert np.array_equal(ht.get_parents(), od.parents)
+    assert np.array_equal(ht.get_children(), od.children)
+    assert np.array_equal(ht.get_next_node(), od.next)
+    assert np.array_equal(ht.get_levels(), od.levels)
+    qs = rng.random((150, 4)).astype(np.float32)
+    oi, odist = oracle.knn_euclidean(od, qs, 6)
+    hi, hd = ht.knn_points(qs, 6, mode="reference")
+    assert np.array_equal(hd.view(np.uint32), odist.view(np.uint32))
+    assert np.array_equal(hi, oi)
