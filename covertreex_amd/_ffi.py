@@ -44,6 +44,7 @@ QT_SCALARS = ("mode", "queries", "distance_evals", "prunes_lower_bound",
 CX_QT_FRONTIER = 14
 CX_QT_EXPANDED = 78
 CX_QT_BLOCKS = 142
+CX_QT_HARD_LISTED = 145
 
 _i64 = C.c_int64
 _i32 = C.c_int32

@@ -545,17 +545,34 @@ __global__ void tau_kernel(const int32_t *__restrict__ hist, int64_t s_begin, in
     tau[s] = t;
 }
 
-// ---- rescore -----------------------------------------------------------------
+// ---- exact scoring -------------------------------------------------------------
+// Listed candidates (rescore) and the candidates of hard queries are scored
+// exactly: FP64 dot, FP64 distance (finish64), and each query keeps its top-k
+// in the registers of one wavefront (WaveTopK).  Every kernel here runs its
+// wavefronts independently: no workgroup barriers.
 
-constexpr int kRsThreads = 256;                 // one workgroup per query
-constexpr int kRsSort = 2048;                   // running top-k + one round, padded
-constexpr int kRsChunk = kRsSort - kMixMaxK;    // listed candidates per sort round
-constexpr int kHMaxM = 512;                     // query rows staged in LDS up to this rank
+constexpr int kWaves = 4;                       // wavefronts per workgroup
+constexpr int kRsThreads = 64 * kWaves;
 
-struct HardEntry {
-    float dh, dl, err;
+// Workgroup b of a launch runs on XCD b % kXcds (round-robin dispatch); the
+// persistent kernels give each XCD its own work counter and work range.
+constexpr int kXcds = 8;
+constexpr int64_t kXcdQueries = 256; // consecutive sorted queries per XCD chunk
+
+// persistent grid: up to 2048 workgroups, a multiple of kXcds
+static inline unsigned xcd_grid(int64_t items)
+{
+    const int64_t b = items < 2048 ? items : 2048;
+    return (unsigned)((b + kXcds - 1) / kXcds * kXcds);
+}
+
+// one scored pair: distance, candidate (dataset index), error bound
+struct Ent {
+    double d;
     int32_t j;
+    float e;
 };
+using HardEntry = Ent;
 
 struct RescoreArgs {
     const float *v;       // n x m tree order
@@ -567,222 +584,476 @@ struct RescoreArgs {
     const int32_t *qrow;
     const int32_t *cnt;   // listed candidates per sorted query
     const int32_t *list;  // kCap slots per sorted query
+    const float *vnorm;   // |V_i| rounded up, dataset order
     int64_t n, nq;
     int32_t d, m, kernel, pred, k;
     float var;
-    float inv_ls_hi[kMaxD], inv_ls_lo[kMaxD];
+    double inv_ls[kMaxD]; // 1 / lengthscale
     int64_t *out_idx;
     float *out_dist, *out_err;
     int64_t *counter;
 };
 
-// The residual distance from a compensated dot V_q.V_j (double-float) and
-// sum|V_q[c] V_j[c]|: kernel value, numerator, division and square root in
-// double-float.  Returns the distance and a bound on its error.
-__device__ __forceinline__ df finish_distance(const RescoreArgs &A, int64_t qt, int64_t jt, df dot,
-                                              float sabs, float *err)
+// e^x for x <= 0 in FP64, from the mathematics: x = k ln2 + r with k the
+// integer nearest x / ln2 (r by two FMAs against ln 2 split into its nearest
+// double and the remainder, so |r| <= 0.3466 and r is accurate to about
+// 2^-53 |r|), e^r by its Taylor series through r^13/13! in Horner form
+// (truncation below 6e-18 relative; the Horner sums add at most
+// 26 * 2^-53 * e^|r| / e^r < 5.8e-15 relative), then the exact scaling by
+// 2^k.  Relative error below 7.1e-15; results under e^-690 are returned as 0.
+__device__ __forceinline__ double exp_neg64(double x)
 {
-    const int32_t d = A.d, m = A.m;
+    if (x < -690.0)
+        return 0.0;
+    constexpr double kLn2Hi = 0x1.62e42fefa39efp-1;  // ln 2 rounded to double
+    constexpr double kLn2Lo = 0x1.abc9e3b39803fp-56; // ln 2 - kLn2Hi
+    constexpr double kInvLn2 = 0x1.71547652b82fep+0;
+    const double kd = __builtin_rint(x * kInvLn2);
+    double r = __builtin_fma(-kd, kLn2Hi, x);
+    r = __builtin_fma(-kd, kLn2Lo, r);
+    double e = 1.0 / 6227020800.0; // 1/13!
+    e = __builtin_fma(e, r, 1.0 / 479001600.0);
+    e = __builtin_fma(e, r, 1.0 / 39916800.0);
+    e = __builtin_fma(e, r, 1.0 / 3628800.0);
+    e = __builtin_fma(e, r, 1.0 / 362880.0);
+    e = __builtin_fma(e, r, 1.0 / 40320.0);
+    e = __builtin_fma(e, r, 1.0 / 5040.0);
+    e = __builtin_fma(e, r, 1.0 / 720.0);
+    e = __builtin_fma(e, r, 1.0 / 120.0);
+    e = __builtin_fma(e, r, 1.0 / 24.0);
+    e = __builtin_fma(e, r, 1.0 / 6.0);
+    e = __builtin_fma(e, r, 0.5);
+    e = __builtin_fma(e, r, 1.0);
+    e = __builtin_fma(e, r, 1.0);
+    return __builtin_ldexp(e, (int)kd);
+}
+
+// The residual distance in FP64 from the dot V_q.V_j and a bound on that
+// dot's error; returns the distance and a bound on its error (err, which also
+// covers rounding the distance to FP32).  With u = 2^-53 and IEEE FP64
+// division, the square roots taken within 2u:
+// - d^2 = sum ((x_q - x_j) / l)^2: relative (d + 7) u;
+// - K: RBF var e^(-d^2/2), absolute var (0.37 (d + 7) u + u + 7.1e-15)
+//   (max x e^-x = 1/e); Matern-5/2 var (1 + a + a^2/3) e^-a, a = sqrt(5 d^2)
+//   with relative error ((d + 8)/2 + 2) u and max (1 + a + a^2/3)(2 + a) e^-a
+//   < 2.6: absolute var ((1.3 d + 21) u + 7.1e-15); both within
+//   eK = |var| ((1.3 d + 24) u + 7.2e-15);
+// - num = K - dot: eK + dot_err + u |num|;
+// - rho = num / sqrt(max(p_q p_j, 1e-18)) (the product exact): e_num / den
+//   plus 4u |rho|;
+// - s = 1 - |rho| (clamped to [0, 1]): plus u s;
+// - dist = sqrt(s): |sqrt(s) - sqrt(s')| <= min(sqrt(e_s), e_s / (sqrt(s) +
+//   sqrt(max(s - e_s, 0)))), plus 2u dist for the root itself.
+__device__ __forceinline__ double finish64(const RescoreArgs &A, int64_t qt, int64_t jt,
+                                          double dot, float dot_err, float *err)
+{
+    constexpr double u = 0x1p-53;
+    const int32_t d = A.d;
     const float *xq = A.raw + qt * d, *xj = A.raw + jt * d;
-    df d2 = {0.0f, 0.0f};
+    double d2 = 0.0;
     for (int c = 0; c < d; c++) {
-        df diff = two_sum(xq[c], -xj[c]);
-        df s = df_mul(diff, df{A.inv_ls_hi[c], A.inv_ls_lo[c]});
-        d2 = df_add(d2, df_mul(s, s));
+        const double t = ((double)xq[c] - (double)xj[c]) * A.inv_ls[c];
+        d2 = __builtin_fma(t, t, d2);
     }
-    df kv;
+    double kv;
     if (A.kernel == 0) {
-        kv = df_exp(df_mul_f(d2, -0.5f));
+        kv = exp_neg64(-0.5 * d2);
     } else {
-        df a = df_sqrt(df_mul_f(d2, 5.0f));
-        constexpr DfConst third = split_const(1.0 / 3.0);
-        df poly = df_add(df_add(df{1.0f, 0.0f}, a), df_mul(df_mul(a, a), df{third.hi, third.lo}));
-        kv = df_mul(poly, df_exp(df_neg(a)));
+        const double a = __builtin_sqrt(5.0 * d2);
+        kv = __builtin_fma(a, __builtin_fma(a, 1.0 / 3.0, 1.0), 1.0) * exp_neg64(-a);
     }
-    kv = df_mul_f(kv, A.var);
-    df num = df_sub(kv, dot);
-    df pp = two_prod(A.p[qt], A.p[jt]);
-    if (pp.hi < 1e-18f)
-        pp = {1e-18f, 0.0f};
-    df den = df_sqrt(pp);
-    df rho = df_div(num, den);
-    if (rho.hi < 0.0f)
-        rho = df_neg(rho);
-    if (rho.hi > 1.0f || (rho.hi == 1.0f && rho.lo > 0.0f)) // |rho| clamped to 1
-        rho = {1.0f, 0.0f};
-    df s = df_sub(df{1.0f, 0.0f}, rho);
-    df dist = df_sqrt(s);
-    // error: dot accumulation (m+8) * 2^-46 * sum|products|, kernel 2^-44,
-    // then through the division and the square root; the output rounding to
-    // f32 adds half an ulp.
-    const float e_num = ((float)(m + 8) * 1.5e-14f * (sabs * 1.01f + __builtin_fabsf(kv.hi)) +
-                         __builtin_fabsf(kv.hi) * 6e-14f);
-    const float e_rho = e_num / den.hi * 1.001f + rho.hi * 3e-14f + 1e-37f;
-    const float sd = dist.hi;
-    float e_d = sd > 0.0f ? e_rho / (2.0f * sd) : __builtin_sqrtf(e_rho);
-    const float e_sq = __builtin_sqrtf(e_rho);
+    kv *= (double)A.var;
+    const double num = kv - dot;
+    double pp = (double)A.p[qt] * (double)A.p[jt];
+    pp = pp < 1e-18 ? 1e-18 : pp;
+    const double den = __builtin_sqrt(pp);
+    double rho = __builtin_fabs(num / den);
+    rho = rho > 1.0 ? 1.0 : rho;
+    const double sv = 1.0 - rho;
+    const double dist = __builtin_sqrt(sv);
+    const double e_k =
+        __builtin_fabs((double)A.var) * ((1.3 * d + 24.0) * u + 7.2e-15);
+    const double e_num = e_k + (double)dot_err + u * __builtin_fabs(num);
+    const double e_s = (e_num / den) * (1.0 + 8.0 * u) + 4.0 * u * rho + u * sv + 1e-300;
+    const double lo = sv - e_s > 0.0 ? __builtin_sqrt(sv - e_s) : 0.0;
+    double e_d = dist > 0.0 ? e_s / (dist + lo) : __builtin_sqrt(e_s);
+    const double e_sq = __builtin_sqrt(e_s);
     e_d = e_d < e_sq ? e_d : e_sq;
-    *err = e_d * 1.01f + sd * 6e-8f;
+    *err = (float)((e_d + 2.0 * u * dist) * 1.01 + dist * 6e-8);
     return dist;
 }
 
-__device__ __forceinline__ bool entry_lt(float ah, float al, int32_t aj, float bh, float bl,
-                                         int32_t bj)
+__device__ __forceinline__ float dot64_error(int32_t m, float norm_q, float norm_j)
 {
-    return ah < bh || (ah == bh && (al < bl || (al == bl && aj < bj)));
+    return ((float)(m / 8 + 6) * 1.12e-16f + 3.56e-15f) * norm_q * norm_j;
 }
 
-// ascending bitonic sort of the first `size` (a power of two) entries
-// (distance hi, lo, index; error rides along) in LDS
-__device__ void sort_entries(float *h, float *l, float *e, int32_t *jv, int size_total)
+// Dot V_q . V_j in FP64, eight lanes per row pair.  A product of two f32
+// values is exact in FP64, so the only rounding is in the sums: each lane
+// runs FMA chains over its columns and the group adds its lanes in three
+// steps.  With u = 2^-53 the error is at most (m/8 + 5) u sum|V_q[c] V_j[c]|
+// <= (m/8 + 5) u |V_q| |V_j|, which dot64_error bounds.  This form reads the
+// query row from memory (ranks that do not fit the staged forms).
+__device__ __forceinline__ double group_dot64(const float *__restrict__ vq,
+                                              const float *__restrict__ vj, int32_t m, int sub)
 {
-    for (int size = 2; size <= size_total; size <<= 1)
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            __syncthreads();
-            for (int t = threadIdx.x; t < size_total / 2; t += blockDim.x) {
-                const int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
-                const bool up = (lo & size) == 0;
-                const bool sw = up ? entry_lt(h[hi], l[hi], jv[hi], h[lo], l[lo], jv[lo])
-                                   : entry_lt(h[lo], l[lo], jv[lo], h[hi], l[hi], jv[hi]);
-                if (sw) {
-                    float a = h[lo]; h[lo] = h[hi]; h[hi] = a;
-                    a = l[lo]; l[lo] = l[hi]; l[hi] = a;
-                    a = e[lo]; e[lo] = e[hi]; e[hi] = a;
-                    const int32_t b = jv[lo]; jv[lo] = jv[hi]; jv[hi] = b;
-                }
-            }
+    double s0 = 0.0;
+    for (int32_t i = sub; i < m; i += 8)
+        s0 = __builtin_fma((double)vq[i], (double)vj[i], s0);
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1)
+        s0 += __shfl_xor(s0, o, 64);
+    return s0;
+}
+
+// row[col .. col + 3] as a float4, zero when col >= m (m % 4 == 0): the load
+// itself is unconditional (column 0 when past m) so that it stays one
+// 16-byte load rather than four predicated ones
+__device__ __forceinline__ float4 load_col4(const float *__restrict__ row, int32_t col, int32_t m)
+{
+    const bool in = col < m;
+    const float4 v = *(const float4 *)(row + (in ? col : 0));
+    return in ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Two FP64 dots per eight-lane group at once: rows vj0 and vj1 against the
+// staged query rows qa and qb (QT = double, or float; zero past m up to a
+// multiple of 64 columns; rows float4-aligned), every load of both rows
+// issued before the first FMA of a 64-column step.  Error as group_dot64's.
+template <typename QT>
+__device__ __forceinline__ void dot_pair(const QT *qa, const QT *qb, const float *vj0,
+                                         const float *vj1, int32_t m, int sub, double &t0,
+                                         double &t1)
+{
+    double p0 = 0.0, p1 = 0.0, r0 = 0.0, r1 = 0.0;
+#pragma unroll 1
+    for (int32_t cb = 0; cb < m; cb += 64) {
+        float4 x[2], y[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int32_t col = cb + 4 * sub + 32 * t;
+            x[t] = load_col4(vj0, col, m);
+            y[t] = load_col4(vj1, col, m);
         }
-    __syncthreads();
-}
-
-// Compensated dot of V rows q and j over this lane's columns (Dot2 within a
-// lane: TwoProduct + TwoSum per term), eight lanes per row pair.  Rows are
-// float4-aligned when m % 4 == 0; the lanes of a group then read 128
-// contiguous bytes per load.
-__device__ __forceinline__ void dot2_step(float a, float b, float &s, float &c, float &sabs)
-{
-    const float ph = a * b;
-    const float pl = __builtin_fmaf(a, b, -ph);
-    const float t = s + ph;
-    const float bb = t - s;
-    c += ((s - (t - bb)) + (ph - bb)) + pl;
-    s = t;
-    sabs += __builtin_fabsf(ph);
-}
-
-__device__ __forceinline__ void group_dot(const float *__restrict__ vq,
-                                          const float *__restrict__ vj, int32_t m, int sub,
-                                          df &dot, float &sabs)
-{
-    float s = 0.0f, c = 0.0f;
-    sabs = 0.0f;
-    if ((m & 3) == 0) {
-#pragma unroll 4
-        for (int32_t i = 4 * sub; i < m; i += 32) {
-            const float4 a = *(const float4 *)(vq + i);
-            const float4 b = *(const float4 *)(vj + i);
-            dot2_step(a.x, b.x, s, c, sabs);
-            dot2_step(a.y, b.y, s, c, sabs);
-            dot2_step(a.z, b.z, s, c, sabs);
-            dot2_step(a.w, b.w, s, c, sabs);
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int32_t col = cb + 4 * sub + 32 * t;
+            const QT *a = qa + col, *b = qb + col;
+            p0 = __builtin_fma((double)a[0], (double)x[t].x, p0);
+            p1 = __builtin_fma((double)a[1], (double)x[t].y, p1);
+            p0 = __builtin_fma((double)a[2], (double)x[t].z, p0);
+            p1 = __builtin_fma((double)a[3], (double)x[t].w, p1);
+            r0 = __builtin_fma((double)b[0], (double)y[t].x, r0);
+            r1 = __builtin_fma((double)b[1], (double)y[t].y, r1);
+            r0 = __builtin_fma((double)b[2], (double)y[t].z, r0);
+            r1 = __builtin_fma((double)b[3], (double)y[t].w, r1);
         }
-    } else {
-#pragma unroll 4
-        for (int32_t i = sub; i < m; i += 8)
-            dot2_step(vq[i], vj[i], s, c, sabs);
     }
-    dot = two_sum(s, c);
+    t0 = p0 + p1;
+    t1 = r0 + r1;
 #pragma unroll
     for (int o = 1; o < 8; o <<= 1) {
-        const df other = {__shfl_xor(dot.hi, o, 64), __shfl_xor(dot.lo, o, 64)};
-        dot = df_add(dot, other);
-        sabs += __shfl_xor(sabs, o, 64);
+        t0 += __shfl_xor(t0, o, 64);
+        t1 += __shfl_xor(t1, o, 64);
     }
 }
 
-// Score one list slice into LDS slots [k, k + n_in) and merge it into the
-// running top-k at [0, k): eight lanes per candidate for the dot (results
-// parked in the slots), then one lane per candidate for the distance.
-__device__ void score_slice(const RescoreArgs &A, const int32_t *__restrict__ lst, int32_t n_in,
-                            int32_t qrow, int64_t qt, const float *vq, float *sh, float *sl,
-                            float *se, int32_t *sj)
+// orders the wavefront's own LDS accesses (no workgroup barrier)
+__device__ __forceinline__ void wave_sync()
 {
-    const int tid = threadIdx.x, grp = tid >> 3, sub = tid & 7;
-    const int32_t k = A.k, m = A.m;
-    __syncthreads();
-    for (int32_t e = grp; e < n_in; e += kRsThreads / 8) {
-        const int32_t j = lst[e];
-        df dot;
-        float sabs;
-        group_dot(vq, A.v + A.inv_order[j] * m, m, sub, dot, sabs);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// entries order by (distance, index); the sentinel (+inf, INT_MAX) is last
+__device__ __forceinline__ bool ent_lt(const Ent &a, const Ent &b)
+{
+    return a.d < b.d || (a.d == b.d && a.j < b.j);
+}
+
+__device__ __forceinline__ Ent ent_none()
+{
+    return Ent{__builtin_huge_val(), 0x7fffffff, 0.0f};
+}
+
+__device__ __forceinline__ Ent ent_shfl(const Ent &a, int src)
+{
+    return Ent{__shfl(a.d, src, 64), __shfl(a.j, src, 64), __shfl(a.e, src, 64)};
+}
+
+// compare-exchange with lane ^ mask: this lane keeps the smaller of the two
+// when take_min, else the larger
+__device__ __forceinline__ void ent_xchg(Ent &a, int mask, bool take_min)
+{
+    const Ent o = Ent{__shfl_xor(a.d, mask, 64), __shfl_xor(a.j, mask, 64),
+                      __shfl_xor(a.e, mask, 64)};
+    if (ent_lt(o, a) == take_min)
+        a = o;
+}
+
+__device__ __forceinline__ void ent_order(Ent &lo, Ent &hi)
+{
+    if (ent_lt(hi, lo)) {
+        const Ent t = lo;
+        lo = hi;
+        hi = t;
+    }
+}
+
+// ascending bitonic sort of one entry per lane across the wavefront
+__device__ __forceinline__ void sort64(Ent &a, int lane)
+{
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1)
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const bool up = size == 64 || (lane & size) == 0;
+            ent_xchg(a, stride, ((lane & stride) == 0) == up);
+        }
+}
+
+// The k smallest entries seen (k <= 64 KB), sorted, held by one wavefront:
+// rank r = lane + 64 b in p[b].  KB + 1 must be a power of two.
+template <int KB> struct WaveTopK {
+    Ent p[KB];
+
+    __device__ __forceinline__ void init()
+    {
+#pragma unroll
+        for (int b = 0; b < KB; b++)
+            p[b] = ent_none();
+    }
+
+    // offer one entry per lane (ent_none() for none): entries that do not
+    // rank ahead of the current last kept entry are dropped; if any remain,
+    // the batch is sorted and merged (bitonic merge of the kept entries with
+    // the reversed batch, the lower 64 KB kept)
+    __device__ __forceinline__ void offer(Ent c, int lane)
+    {
+        const Ent last = ent_shfl(p[KB - 1], 63);
+        const bool in = ent_lt(c, last);
+        if (!__any(in))
+            return;
+        if (!in)
+            c = ent_none();
+        sort64(c, lane);
+        Ent q[KB + 1];
+#pragma unroll
+        for (int b = 0; b < KB; b++)
+            q[b] = p[b];
+        q[KB] = ent_shfl(c, 63 - lane);
+#pragma unroll
+        for (int sb = (KB + 1) / 2; sb >= 1; sb >>= 1)
+#pragma unroll
+            for (int b = 0; b < KB + 1; b++)
+                if ((b & sb) == 0)
+                    ent_order(q[b], q[b + sb]);
+#pragma unroll
+        for (int stride = 32; stride >= 1; stride >>= 1)
+#pragma unroll
+            for (int b = 0; b < KB + 1; b++)
+                ent_xchg(q[b], stride, (lane & stride) == 0);
+#pragma unroll
+        for (int b = 0; b < KB; b++)
+            p[b] = q[b];
+    }
+
+    // the first k entries: rank r to dst[r] via `put(r, entry)`
+    template <typename F> __device__ __forceinline__ void emit(int k, int lane, F put) const
+    {
+#pragma unroll
+        for (int b = 0; b < KB; b++) {
+            const int r = lane + 64 * b;
+            if (r < k)
+                put(r, p[b]);
+        }
+    }
+};
+
+__device__ __forceinline__ void put_output(const RescoreArgs &A, int64_t pos_out, int r,
+                                           const Ent &e)
+{
+    const int64_t o = pos_out * A.k + r;
+    const bool ok = e.d < __builtin_huge_val();
+    A.out_idx[o] = ok ? (int64_t)e.j : -1;
+    A.out_dist[o] = ok ? (float)e.d : 3.40282347e38f;
+    if (A.out_err)
+        A.out_err[o] = ok ? e.e : 0.0f;
+}
+
+// query rows staged per wavefront: 256 doubles, or 512 floats
+constexpr int kStageM = 256;
+
+// 2: doubles (m <= 256), 1: floats (m <= 512), 0: not staged
+__device__ __forceinline__ int stage_mode(int32_t m)
+{
+    return (m & 3) != 0 ? 0 : m <= 256 ? 2 : m <= 512 ? 1 : 0;
+}
+
+__device__ __forceinline__ void stage_row(const float *__restrict__ src, int32_t m, int mode,
+                                          double *dst, int lane)
+{
+    if (mode == 2)
+        for (int c = lane; c < 256; c += 64)
+            dst[c] = c < m ? (double)src[c] : 0.0;
+    else if (mode == 1)
+        for (int c = lane; c < 512; c += 64)
+            ((float *)dst)[c] = c < m ? src[c] : 0.0f;
+}
+
+// Dots of one wavefront's batch of up to 64 candidates (tree rows jt, one
+// per lane) against the query row staged as doubles (rank <= 256, m % 4 == 0,
+// zero past m): lane group g (eight lanes) takes candidates g, g + 8, ...,
+// g + 56, each lane covering columns 4 sub + 32 t (t < 8) as float4 pieces.
+// The next candidate's row is loaded while the current one is multiplied, so
+// one batch waits on memory about once.  Dots go to s_dot[e]; error as
+// group_dot64's.
+__device__ __forceinline__ void batch_dots_staged(const float *__restrict__ v, int32_t m,
+                                                  const double *vqd, int32_t jt, int n_in,
+                                                  double *s_dot, int lane)
+{
+    const int grp = lane >> 3, sub = lane & 7;
+    int32_t jts[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+        jts[u] = __shfl(jt, grp + 8 * u, 64);
+    float4 cur[8], nxt[8];
+    if (grp < n_in) {
+        const float *row = v + (int64_t)jts[0] * m;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int32_t col = 4 * sub + 32 * t;
+            cur[t] = load_col4(row, col, m);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int e = grp + 8 * u;
+        if (u < 7 && e + 8 < n_in) {
+            const float *row = v + (int64_t)jts[u + 1] * m;
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const int32_t col = 4 * sub + 32 * t;
+                nxt[t] = load_col4(row, col, m);
+            }
+        }
+        if (e < n_in) {
+            double p0 = 0.0, p1 = 0.0;
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const int32_t col = 4 * sub + 32 * t;
+                const double2 a0 = *(const double2 *)(vqd + col);
+                const double2 a1 = *(const double2 *)(vqd + col + 2);
+                p0 = __builtin_fma(a0.x, (double)cur[t].x, p0);
+                p1 = __builtin_fma(a0.y, (double)cur[t].y, p1);
+                p0 = __builtin_fma(a1.x, (double)cur[t].z, p0);
+                p1 = __builtin_fma(a1.y, (double)cur[t].w, p1);
+            }
+            double t0 = p0 + p1;
+#pragma unroll
+            for (int o = 1; o < 8; o <<= 1)
+                t0 += __shfl_xor(t0, o, 64);
+            if (sub == 0)
+                s_dot[e] = t0;
+        }
+        if (u < 7) {
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                cur[t] = nxt[t];
+        }
+    }
+}
+
+// Dots of a batch for the other ranks: the float-staged query (rank <= 512,
+// m % 4 == 0) two candidates per lane group at a time, or the query row read
+// from memory.
+__device__ __forceinline__ void batch_dots_other(const float *__restrict__ v, int32_t m,
+                                                 int mode, const double *vqd, const float *vq_mem,
+                                                 int32_t jt, int n_in, double *s_dot, int lane)
+{
+    const int grp = lane >> 3, sub = lane & 7;
+#pragma unroll 1
+    for (int u = 0; u < 64; u += 16) {
+        const int ea = u + grp, eb = ea + 8;
+        const int32_t ja = __shfl(jt, ea, 64), jb = __shfl(jt, eb, 64);
+        if (ea >= n_in)
+            continue;
+        const float *va = v + (int64_t)ja * m;
+        const float *vb = v + (int64_t)(eb < n_in ? jb : ja) * m;
+        double ta, tb;
+        if (mode == 1) {
+            const float *vqs = (const float *)vqd;
+            dot_pair<float>(vqs, vqs, va, vb, m, sub, ta, tb);
+        } else {
+            ta = group_dot64(vq_mem, va, m, sub);
+            tb = group_dot64(vq_mem, vb, m, sub);
+        }
         if (sub == 0) {
-            sh[k + e] = dot.hi;
-            sl[k + e] = dot.lo;
-            se[k + e] = sabs;
-            sj[k + e] = j;
+            s_dot[ea] = ta;
+            if (eb < n_in)
+                s_dot[eb] = tb;
         }
     }
-    __syncthreads();
-    for (int32_t e = tid; e < n_in; e += kRsThreads) {
-        const int32_t j = sj[k + e];
-        float dh = 0.0f, dl = 0.0f, er = 0.0f;
-        if (j != qrow) {
-            const df dd = finish_distance(A, qt, A.inv_order[j], df{sh[k + e], sl[k + e]},
-                                          se[k + e], &er);
-            dh = dd.hi;
-            dl = dd.lo;
-        }
-        sh[k + e] = dh;
-        sl[k + e] = dl;
-        se[k + e] = er;
-    }
-    int p2 = 2;
-    while (p2 < k + n_in)
-        p2 <<= 1;
-    // entries past k left by an earlier round rank behind the current top-k,
-    // so they never re-enter it
-    sort_entries(sh, sl, se, sj, p2);
 }
 
-__device__ __forceinline__ void clear_entries(float *sh, float *sl, float *se, int32_t *sj)
+// One wavefront scores candidates for one query: `count` candidates, the
+// list `lst` (dataset indices) when given, else the range base + [0, count),
+// those at or past `lim` skipped; batches of 64 offered to `top`.
+template <int KB>
+__device__ __forceinline__ void score_query(const RescoreArgs &A, int mode, int64_t qt,
+                                            int32_t qrow, float qn, const int32_t *lst,
+                                            int64_t base, int64_t count, int64_t lim,
+                                            const double *vqd, double *s_dot, WaveTopK<KB> &top,
+                                            int lane)
 {
-    for (int t = threadIdx.x; t < kRsSort; t += kRsThreads) {
-        sh[t] = __builtin_huge_valf();
-        sl[t] = 0.0f;
-        se[t] = 0.0f;
-        sj[t] = 0x7fffffff;
-    }
-}
-
-__device__ __forceinline__ void write_row(const RescoreArgs &A, int64_t pos_out, const float *sh,
-                                          const float *sl, const float *se, const int32_t *sj)
-{
-    const int32_t k = A.k;
-    for (int t = threadIdx.x; t < k; t += kRsThreads) {
-        const bool ok = sh[t] < __builtin_huge_valf();
-        A.out_idx[pos_out * k + t] = ok ? (int64_t)sj[t] : -1;
-        A.out_dist[pos_out * k + t] = ok ? sh[t] + sl[t] : 3.40282347e38f;
-        if (A.out_err)
-            A.out_err[pos_out * k + t] = ok ? se[t] : 0.0f;
-    }
-}
-
-// One workgroup per query, its kCap-slot list scored in rounds of kRsChunk,
-// output row written.  Queries whose list overflowed are left to the hard
-// kernels.
-__global__ void __launch_bounds__(kRsThreads) rescore_kernel(RescoreArgs A)
-{
-    __shared__ float sh[kRsSort], sl[kRsSort], se[kRsSort];
-    __shared__ int32_t sj[kRsSort];
-    __shared__ int64_t s_item;
-    __shared__ __attribute__((aligned(16))) float s_vq[kHMaxM];
     const int32_t m = A.m;
+    for (int64_t c0 = 0; c0 < count; c0 += 64) {
+        const int n_in = count - c0 < 64 ? (int)(count - c0) : 64;
+        const bool valid = lane < n_in;
+        const int64_t j = valid ? (lst ? (int64_t)lst[c0 + lane] : base + c0 + lane) : 0;
+        const int32_t jt = valid ? (int32_t)A.inv_order[j] : 0;
+        if (mode == 2)
+            batch_dots_staged(A.v, m, vqd, jt, n_in, s_dot, lane);
+        else
+            batch_dots_other(A.v, m, mode, vqd, A.v + qt * m, jt, n_in, s_dot, lane);
+        wave_sync();
+        Ent c = ent_none();
+        if (valid && j < lim) {
+            c.j = (int32_t)j;
+            if (j == qrow) {
+                c.d = 0.0;
+                c.e = 0.0f;
+            } else {
+                c.d = finish64(A, qt, jt, s_dot[lane], dot64_error(m, qn, A.vnorm[j]), &c.e);
+            }
+        }
+        wave_sync();
+        top.offer(c, lane);
+    }
+}
+
+// ---- listed candidates ---------------------------------------------------------
+// One wavefront per query (from its XCD's chunk of sorted queries) scores its
+// list.  Queries whose list overflowed kCap are left to the hard kernels.
+
+template <int KB>
+__global__ void __launch_bounds__(kRsThreads, 3) rescore_kernel(RescoreArgs A)
+{
+    __shared__ __attribute__((aligned(16))) double s_vq[kWaves][kStageM];
+    __shared__ double s_dot[kWaves][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t m = A.m, k = A.k;
+    const int mode = stage_mode(m);
+    const int x = blockIdx.x % kXcds;
     for (;;) {
-        if (threadIdx.x == 0)
-            s_item = (int64_t)atomicAdd((unsigned long long *)A.counter, 1ull);
-        __syncthreads();
-        const int64_t s = s_item;
-        __syncthreads();
+        int64_t local = 0;
+        if (lane == 0)
+            local = (int64_t)atomicAdd((unsigned long long *)(A.counter + x), 1ull);
+        local = __shfl(local, 0, 64);
+        // this XCD's chunks of kXcdQueries consecutive sorted queries, so its
+        // L2 holds the candidate rows of neighbouring queries
+        const int64_t s = (x + kXcds * (local / kXcdQueries)) * kXcdQueries + local % kXcdQueries;
         if (s >= A.nq)
             break;
         const int32_t count = A.cnt[s];
@@ -790,70 +1061,36 @@ __global__ void __launch_bounds__(kRsThreads) rescore_kernel(RescoreArgs A)
             continue; // scored over all its candidates by the hard kernels
         const int32_t qrow = A.qrow[s];
         const int64_t qt = A.inv_order[qrow];
-        // the query row in LDS when it fits (score_slice syncs before use)
-        const float *vq = A.v + qt * m;
-        if (m <= kHMaxM) {
-            for (int c = threadIdx.x; c < m; c += kRsThreads)
-                s_vq[c] = vq[c];
-            vq = s_vq;
-        }
-        clear_entries(sh, sl, se, sj);
-        for (int32_t c0 = 0; c0 < count; c0 += kRsChunk) {
-            const int32_t n_in = count - c0 < kRsChunk ? count - c0 : kRsChunk;
-            score_slice(A, A.list + s * kCap + c0, n_in, qrow, qt, vq, sh, sl, se, sj);
-        }
-        write_row(A, A.qperm[s], sh, sl, se, sj);
-        __syncthreads();
+        wave_sync();
+        stage_row(A.v + qt * m, m, mode, s_vq[w], lane);
+        wave_sync();
+        WaveTopK<KB> top;
+        top.init();
+        score_query<KB>(A, mode, qt, qrow, A.vnorm[qrow], A.list + s * kCap, 0, count,
+                        A.n, s_vq[w], s_dot[w], top, lane);
+        const int64_t pos = A.qperm[s];
+        top.emit(k, lane, [&](int r, const Ent &e) { put_output(A, pos, r, e); });
     }
 }
 
 // ---- hard queries ------------------------------------------------------------
 // A query whose list overflowed kCap (a wide screen interval, e.g. tiny p) is
-// scored over all its valid candidates.  One workgroup = kHQ consecutive hard
-// queries x a kHSeg-candidate segment: every candidate row is read once for
-// all kHQ queries (eight lanes per candidate, kHQ compensated dots per lane
-// group), distances are finished one lane per pair, and each query keeps a
-// running top-k in LDS.  A merge kernel combines the segments.
+// scored over all its valid candidates.  One wavefront item = one hard query
+// x a kHSeg-candidate segment, scored as a list (the segment's rows stay in
+// the XCD's L2 for the other queries); a merge kernel combines the segments'
+// top-k lists.
 
-constexpr int kHQ = 4;        // hard queries per workgroup
-constexpr int kHSeg = 1024;   // candidates per workgroup
-constexpr int kHRound = 128;  // candidates per sort round
-constexpr int kHSort = 512;   // running top-k (k <= 256) + one round, padded
+constexpr int kHSeg = 1024;  // candidates per item
 
 struct HardArgs {
     RescoreArgs r;
     const int32_t *hard;  // sorted positions of the hard queries
     int32_t n_hard;
-    int32_t g0, gn;       // this batch: query groups [g0, g0 + gn)
-    int32_t nseg;         // segments per group (ceil(n / kHSeg))
-    HardEntry *part;      // gn x nseg x kHQ x k segment top-k lists
+    int32_t h0, hn;       // this batch: hard queries [h0, h0 + hn)
+    int32_t nseg;         // segments per query (ceil(n / kHSeg))
+    HardEntry *part;      // hn x nseg x k segment top-k lists
     int64_t *counter;
 };
-
-// ascending bitonic sort of the first `size` entries of each of `na` arrays
-// laid out [a][kHSort]
-__device__ void sort_arrays(float *h, float *l, float *e, int32_t *jv, int na, int size_total)
-{
-    const int half = size_total / 2;
-    for (int size = 2; size <= size_total; size <<= 1)
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            __syncthreads();
-            for (int t = threadIdx.x; t < na * half; t += blockDim.x) {
-                const int a = t / half, tt = t - a * half;
-                const int lo = a * kHSort + 2 * tt - (tt & (stride - 1)), hi = lo + stride;
-                const bool up = ((lo - a * kHSort) & size) == 0;
-                const bool sw = up ? entry_lt(h[hi], l[hi], jv[hi], h[lo], l[lo], jv[lo])
-                                   : entry_lt(h[lo], l[lo], jv[lo], h[hi], l[hi], jv[hi]);
-                if (sw) {
-                    float x = h[lo]; h[lo] = h[hi]; h[hi] = x;
-                    x = l[lo]; l[lo] = l[hi]; l[hi] = x;
-                    x = e[lo]; e[lo] = e[hi]; e[hi] = x;
-                    const int32_t y = jv[lo]; jv[lo] = jv[hi]; jv[hi] = y;
-                }
-            }
-        }
-    __syncthreads();
-}
 
 __device__ __forceinline__ int64_t hard_limit(const RescoreArgs &A, int64_t s)
 {
@@ -862,197 +1099,70 @@ __device__ __forceinline__ int64_t hard_limit(const RescoreArgs &A, int64_t s)
     return A.pred ? (qid < 0 ? A.n : (qid < A.n ? qid : A.n)) : A.n;
 }
 
-__global__ void __launch_bounds__(kRsThreads) hard_tile_kernel(HardArgs H)
+template <int KB>
+__global__ void __launch_bounds__(kRsThreads, 3) hard_tile_kernel(HardArgs H)
 {
-    __shared__ float sh[kHQ * kHSort], sl[kHQ * kHSort], se[kHQ * kHSort];
-    __shared__ int32_t sj[kHQ * kHSort];
-    __shared__ int64_t s_item;
-    __shared__ int64_t s_lim[kHQ], s_qt[kHQ];
-    __shared__ int32_t s_qrow[kHQ];
-    __shared__ __attribute__((aligned(16))) float s_vq[kHQ * kHMaxM];
+    __shared__ __attribute__((aligned(16))) double s_vq[kWaves][kStageM];
+    __shared__ double s_dot[kWaves][64];
     const RescoreArgs &A = H.r;
-    const int tid = threadIdx.x, grp = tid >> 3, sub = tid & 7;
-    const int32_t k = A.k, m = A.m;
-    // float4-aligned rows that fit LDS are staged
-    const bool staged = (m & 3) == 0 && m <= kHMaxM;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t m = A.m, k = A.k;
+    const int mode = stage_mode(m);
+    const int x = blockIdx.x % kXcds;
     for (;;) {
-        if (tid == 0)
-            s_item = (int64_t)atomicAdd((unsigned long long *)H.counter, 1ull);
-        __syncthreads();
-        const int64_t item = s_item;
-        __syncthreads();
-        if (item >= (int64_t)H.gn * H.nseg)
+        int64_t local = 0;
+        if (lane == 0)
+            local = (int64_t)atomicAdd((unsigned long long *)(H.counter + x), 1ull);
+        local = __shfl(local, 0, 64);
+        // this XCD takes the segments x, x + 8, ..., each for every query of
+        // the batch before the next, so its L2 holds the segment's rows
+        const int64_t seg_l = x + kXcds * (local / H.hn);
+        if (seg_l >= H.nseg)
             break;
-        const int32_t gl = (int32_t)(item / H.nseg), seg = (int32_t)(item % H.nseg);
-        const int32_t h0 = (H.g0 + gl) * kHQ;
-        const int nq_g = H.n_hard - h0 < kHQ ? H.n_hard - h0 : kHQ;
-        if (tid < kHQ) {
-            int64_t lim = 0, qtv = 0;
-            int32_t row = -1;
-            if (tid < nq_g) {
-                const int64_t s = H.hard[h0 + tid];
-                lim = hard_limit(A, s);
-                row = A.qrow[s];
-                qtv = A.inv_order[row];
-            }
-            s_lim[tid] = lim;
-            s_qt[tid] = qtv;
-            s_qrow[tid] = row;
-        }
-        __syncthreads();
-        int64_t qt[kHQ];
-        int64_t lim_max = 0;
-#pragma unroll
-        for (int i = 0; i < kHQ; i++) {
-            qt[i] = s_qt[i];
-            lim_max = s_lim[i] > lim_max ? s_lim[i] : lim_max;
-        }
-        if (staged)
-            for (int t = tid; t < kHQ * m; t += kRsThreads) {
-                const int i = t / m, col = t - i * m;
-                s_vq[i * kHMaxM + col] = A.v[s_qt[i] * m + col];
-            }
+        const int32_t hl = (int32_t)(local % H.hn), seg = (int32_t)seg_l;
+        const int64_t s = H.hard[H.h0 + hl];
+        const int64_t lim = hard_limit(A, s);
         const int64_t beg = (int64_t)seg * kHSeg;
-        if (beg >= lim_max)
-            continue; // no query of the group has candidates here
-        const int64_t end = beg + kHSeg < lim_max ? beg + kHSeg : lim_max;
-        for (int t = tid; t < kHQ * kHSort; t += kRsThreads) {
-            sh[t] = __builtin_huge_valf();
-            sl[t] = 0.0f;
-            se[t] = 0.0f;
-            sj[t] = 0x7fffffff;
-        }
-        for (int64_t c0 = beg; c0 < end; c0 += kHRound) {
-            const int n_in = end - c0 < kHRound ? (int)(end - c0) : kHRound;
-            __syncthreads();
-            // phase A: eight lanes per candidate, kHQ dots each
-            for (int e = grp; e < n_in; e += kRsThreads / 8) {
-                const int64_t j = c0 + e;
-                const float *vj = A.v + A.inv_order[j] * m;
-                float s_[kHQ], c_[kHQ], sa[kHQ];
-#pragma unroll
-                for (int i = 0; i < kHQ; i++) {
-                    s_[i] = 0.0f;
-                    c_[i] = 0.0f;
-                    sa[i] = 0.0f;
-                }
-                if (staged) {
-                    // query rows in LDS, candidate row in float4 pieces
-#pragma unroll 2
-                    for (int32_t col = 4 * sub; col < m; col += 32) {
-                        const float4 b = *(const float4 *)(vj + col);
-#pragma unroll
-                        for (int i = 0; i < kHQ; i++) {
-                            const float4 a = *(const float4 *)(s_vq + i * kHMaxM + col);
-                            dot2_step(a.x, b.x, s_[i], c_[i], sa[i]);
-                            dot2_step(a.y, b.y, s_[i], c_[i], sa[i]);
-                            dot2_step(a.z, b.z, s_[i], c_[i], sa[i]);
-                            dot2_step(a.w, b.w, s_[i], c_[i], sa[i]);
-                        }
-                    }
-                } else {
-                    for (int32_t col = sub; col < m; col += 8) {
-                        const float b = vj[col];
-#pragma unroll
-                        for (int i = 0; i < kHQ; i++)
-                            dot2_step(A.v[qt[i] * m + col], b, s_[i], c_[i], sa[i]);
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < kHQ; i++) {
-                    df dot = two_sum(s_[i], c_[i]);
-                    float sab = sa[i];
-#pragma unroll
-                    for (int o = 1; o < 8; o <<= 1) {
-                        const df other = {__shfl_xor(dot.hi, o, 64), __shfl_xor(dot.lo, o, 64)};
-                        dot = df_add(dot, other);
-                        sab += __shfl_xor(sab, o, 64);
-                    }
-                    if (sub == 0) {
-                        const int slot = i * kHSort + k + e;
-                        sh[slot] = dot.hi;
-                        sl[slot] = dot.lo;
-                        se[slot] = sab;
-                        sj[slot] = (int32_t)j;
-                    }
-                }
-            }
-            __syncthreads();
-            // phase B: one lane per (query, candidate) pair
-            for (int t = tid; t < kHQ * n_in; t += kRsThreads) {
-                const int i = t / n_in, e = t - i * n_in;
-                const int slot = i * kHSort + k + e;
-                const int32_t j = sj[slot];
-                float dh = __builtin_huge_valf(), dl = 0.0f, er = 0.0f;
-                int32_t jj = 0x7fffffff;
-                if (i < nq_g && (int64_t)j < s_lim[i]) {
-                    jj = j;
-                    if (j == s_qrow[i]) {
-                        dh = 0.0f;
-                    } else {
-                        const df dd = finish_distance(A, s_qt[i], A.inv_order[j],
-                                                      df{sh[slot], sl[slot]}, se[slot], &er);
-                        dh = dd.hi;
-                        dl = dd.lo;
-                    }
-                }
-                sh[slot] = dh;
-                sl[slot] = dl;
-                se[slot] = er;
-                sj[slot] = jj;
-            }
-            int p2 = 2;
-            while (p2 < k + n_in)
-                p2 <<= 1;
-            // entries past k left by an earlier round rank behind the current
-            // top-k, so they never re-enter it
-            sort_arrays(sh, sl, se, sj, kHQ, p2);
-        }
-        HardEntry *out = H.part + ((int64_t)gl * H.nseg + seg) * kHQ * k;
-        for (int t = tid; t < kHQ * k; t += kRsThreads) {
-            const int i = t / k, r = t - i * k;
-            const int slot = i * kHSort + r;
-            out[t] = HardEntry{sh[slot], sl[slot], se[slot], sj[slot]};
-        }
-        __syncthreads();
+        if (beg >= lim)
+            continue; // past the query's candidates (never read)
+        const int64_t end = beg + kHSeg < lim ? beg + kHSeg : lim;
+        const int32_t qrow = A.qrow[s];
+        const int64_t qt = A.inv_order[qrow];
+        wave_sync();
+        stage_row(A.v + qt * m, m, mode, s_vq[w], lane);
+        wave_sync();
+        WaveTopK<KB> top;
+        top.init();
+        score_query<KB>(A, mode, qt, qrow, A.vnorm[qrow], nullptr, beg, end - beg, lim,
+                        s_vq[w], s_dot[w], top, lane);
+        HardEntry *out = H.part + ((int64_t)hl * H.nseg + seg) * k;
+        top.emit(k, lane, [&](int r, const Ent &e) { out[r] = e; });
     }
 }
 
-// Merge the segment lists of each hard query of the batch; write its row.
+// Merge the segment lists of each hard query of the batch (one wavefront per
+// query); write its row.
+template <int KB>
 __global__ void __launch_bounds__(kRsThreads) hard_merge_kernel(HardArgs H)
 {
-    __shared__ float sh[kRsSort], sl[kRsSort], se[kRsSort];
-    __shared__ int32_t sj[kRsSort];
     const RescoreArgs &A = H.r;
+    const int lane = threadIdx.x & 63;
     const int32_t k = A.k;
-    const int32_t per = (kRsSort - k) / k; // lists merged per round
-    const int32_t h_begin = H.g0 * kHQ;
-    const int32_t h_end = (H.g0 + H.gn) * kHQ < H.n_hard ? (H.g0 + H.gn) * kHQ : H.n_hard;
-    for (int32_t h = h_begin + blockIdx.x; h < h_end; h += gridDim.x) {
-        const int64_t s = H.hard[h];
+    const int32_t stride = gridDim.x * kWaves;
+    for (int32_t hl = blockIdx.x * kWaves + (threadIdx.x >> 6); hl < H.hn; hl += stride) {
+        const int64_t s = H.hard[H.h0 + hl];
         const int64_t limit = hard_limit(A, s);
-        const int32_t nl_all = (int32_t)((limit + kHSeg - 1) / kHSeg);
-        const int32_t gl = h / kHQ - H.g0, i = h % kHQ;
-        __syncthreads();
-        clear_entries(sh, sl, se, sj);
-        for (int32_t g = 0; g < nl_all; g += per) {
-            const int32_t nl = nl_all - g < per ? nl_all - g : per;
-            __syncthreads();
-            for (int t = threadIdx.x; t < nl * k; t += kRsThreads) {
-                const int32_t seg = g + t / k;
-                const HardEntry e =
-                    H.part[(((int64_t)gl * H.nseg + seg) * kHQ + i) * k + t % k];
-                sh[k + t] = e.dh;
-                sl[k + t] = e.dl;
-                se[k + t] = e.err;
-                sj[k + t] = e.j;
-            }
-            int p2 = 2;
-            while (p2 < k + nl * k)
-                p2 <<= 1;
-            sort_entries(sh, sl, se, sj, p2);
+        const int32_t nl = (int32_t)((limit + kHSeg - 1) / kHSeg);
+        WaveTopK<KB> top;
+        top.init();
+        const int64_t total = (int64_t)nl * k;
+        const HardEntry *src = H.part + (int64_t)hl * H.nseg * k;
+        for (int64_t e0 = 0; e0 < total; e0 += 64) {
+            const int64_t e = e0 + lane;
+            top.offer(e < total ? src[e] : ent_none(), lane);
         }
-        write_row(A, A.qperm[s], sh, sl, se, sj);
+        const int64_t pos = A.qperm[s];
+        top.emit(k, lane, [&](int r, const Ent &e) { put_output(A, pos, r, e); });
     }
 }
 
@@ -1084,9 +1194,9 @@ __global__ void mixed_telemetry_kernel(const int64_t *__restrict__ qsorted,
                                        const int32_t *__restrict__ cnt, int64_t nq, int64_t n,
                                        int32_t pred, int64_t *telem)
 {
-    __shared__ unsigned long long s_acc[4];
+    __shared__ unsigned long long s_acc[5];
     __shared__ int s_max;
-    if (threadIdx.x < 4)
+    if (threadIdx.x < 5)
         s_acc[threadIdx.x] = 0ull;
     if (threadIdx.x == 0)
         s_max = 0;
@@ -1101,6 +1211,7 @@ __global__ void mixed_telemetry_kernel(const int64_t *__restrict__ qsorted,
         atomicAdd(&s_acc[1], hard ? 0ull : (unsigned long long)c);
         atomicAdd(&s_acc[2], hard ? 1ull : 0ull);
         atomicAdd(&s_acc[3], hard ? (unsigned long long)lim : 0ull);
+        atomicAdd(&s_acc[4], hard ? (unsigned long long)c : 0ull);
         atomicMax(&s_max, c);
     }
     __syncthreads();
@@ -1109,6 +1220,7 @@ __global__ void mixed_telemetry_kernel(const int64_t *__restrict__ qsorted,
         atomicAdd((unsigned long long *)&telem[CX_QT_LISTED], s_acc[1]);
         atomicAdd((unsigned long long *)&telem[CX_QT_HARD_QUERIES], s_acc[2]);
         atomicAdd((unsigned long long *)&telem[CX_QT_HARD_PAIRS], s_acc[3]);
+        atomicAdd((unsigned long long *)&telem[CX_QT_HARD_LISTED], s_acc[4]);
         atomicMax((unsigned long long *)&telem[CX_QT_MAX_LIST], (unsigned long long)s_max);
     }
 }
@@ -1220,10 +1332,10 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     int32_t *hist = (int32_t *)(ws + o_hist);
     int32_t *flag = (int32_t *)(ws + o_flag), *pos = (int32_t *)(ws + o_pos);
     int32_t *hard = (int32_t *)(ws + o_hard);
-    // counters: ctr[0..1] rescore items (int64), ctr[2..3] hard items (int64),
-    // ctr[4] hard count
-    int64_t *rs_items = (int64_t *)ctr, *hs_items = (int64_t *)(ctr + 2);
-    int32_t *n_hard_d = ctr + 4;
+    // counters: int64 [0, 8) rescore items per XCD, [8, 16) hard items per
+    // XCD; int32 at byte 128 the hard count
+    int64_t *rs_items = (int64_t *)ctr, *hs_items = (int64_t *)ctr + kXcds;
+    int32_t *n_hard_d = ctr + 32;
 
     const unsigned qb = (unsigned)((nq + 255) / 256);
     iota64_kernel<<<qb, 256, 0, st>>>(iota, nq);
@@ -1312,6 +1424,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
 
     RescoreArgs ra{};
     ra.v = t.v.ptr;
+    ra.vnorm = t.vnorm.ptr;
     ra.raw = t.raw.ptr;
     ra.p = t.p.ptr;
     ra.inv_order = t.inv_order.ptr;
@@ -1328,11 +1441,8 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     ra.pred = predecessor ? 1 : 0;
     ra.k = k;
     ra.var = t.var;
-    for (int c = 0; c < t.d; c++) {
-        const double il = 1.0 / (double)t.ls[c];
-        ra.inv_ls_hi[c] = (float)il;
-        ra.inv_ls_lo[c] = (float)(il - (double)(float)il);
-    }
+    for (int c = 0; c < t.d; c++)
+        ra.inv_ls[c] = 1.0 / (double)t.ls[c];
     ra.out_idx = d_idx;
     ra.out_dist = d_dist;
     ra.out_err = d_err;
@@ -1350,8 +1460,15 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     hipEvent_t ev = t.sync_event;
     CX_HIP(hipEventRecord(ev, st));
     // ... while the listed queries are rescored
-    const int64_t blocks = nq < 2048 ? nq : 2048;
-    rescore_kernel<<<(unsigned)blocks, kRsThreads, 0, st>>>(ra);
+    // top-k held as KB entries per lane: k <= 64 KB
+    const int kb = k <= 64 ? 1 : k <= 192 ? 3 : 7;
+    const unsigned rs_grid = xcd_grid((nq + kWaves - 1) / kWaves);
+    if (kb == 1)
+        rescore_kernel<1><<<rs_grid, kRsThreads, 0, st>>>(ra);
+    else if (kb == 3)
+        rescore_kernel<3><<<rs_grid, kRsThreads, 0, st>>>(ra);
+    else
+        rescore_kernel<7><<<rs_grid, kRsThreads, 0, st>>>(ra);
     CX_HIP(hipGetLastError());
     prof_mark(t, 6);
     CX_HIP(hipEventSynchronize(ev));
@@ -1364,24 +1481,33 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
         ha.n_hard = n_hard;
         ha.nseg = (int32_t)((n + kHSeg - 1) / kHSeg);
         ha.counter = hs_items;
-        const int32_t n_groups = (n_hard + kHQ - 1) / kHQ;
-        const int64_t group_bytes = (int64_t)ha.nseg * kHQ * k * sizeof(HardEntry);
+        const int64_t query_bytes = (int64_t)ha.nseg * k * sizeof(HardEntry);
         const int32_t batch = (int32_t)std::max<int64_t>(
-            1, std::min<int64_t>(n_groups, kPartBudget / group_bytes));
-        if (t.part.count < (size_t)(batch * group_bytes)) {
+            1, std::min<int64_t>(n_hard, kPartBudget / query_bytes));
+        if (t.part.count < (size_t)(batch * query_bytes)) {
             CX_HIP(hipStreamSynchronize(st));
-            t.part.alloc((size_t)(batch * group_bytes));
+            t.part.alloc((size_t)(batch * query_bytes));
         }
         ha.part = (HardEntry *)t.part.ptr;
-        for (int32_t g0 = 0; g0 < n_groups; g0 += batch) {
-            ha.g0 = g0;
-            ha.gn = std::min(batch, n_groups - g0);
-            CX_HIP(hipMemsetAsync(hs_items, 0, sizeof(int64_t), st));
-            const int64_t items = (int64_t)ha.gn * ha.nseg;
-            hard_tile_kernel<<<(unsigned)std::min<int64_t>(items, 2048), kRsThreads, 0, st>>>(ha);
-            CX_HIP(hipGetLastError());
-            const int64_t hq = std::min<int64_t>((int64_t)ha.gn * kHQ, 2048);
-            hard_merge_kernel<<<(unsigned)hq, kRsThreads, 0, st>>>(ha);
+        for (int32_t h0 = 0; h0 < n_hard; h0 += batch) {
+            ha.h0 = h0;
+            ha.hn = std::min(batch, n_hard - h0);
+            CX_HIP(hipMemsetAsync(hs_items, 0, kXcds * sizeof(int64_t), st));
+            const int64_t items = (int64_t)ha.hn * ha.nseg;
+            const unsigned tg = xcd_grid((items + kWaves - 1) / kWaves);
+            // one wavefront per hard query of the batch
+            const unsigned mg =
+                (unsigned)std::min<int64_t>(((int64_t)ha.hn + kWaves - 1) / kWaves, 2048);
+            if (kb == 1) {
+                hard_tile_kernel<1><<<tg, kRsThreads, 0, st>>>(ha);
+                hard_merge_kernel<1><<<mg, kRsThreads, 0, st>>>(ha);
+            } else if (kb == 3) {
+                hard_tile_kernel<3><<<tg, kRsThreads, 0, st>>>(ha);
+                hard_merge_kernel<3><<<mg, kRsThreads, 0, st>>>(ha);
+            } else {
+                hard_tile_kernel<7><<<tg, kRsThreads, 0, st>>>(ha);
+                hard_merge_kernel<7><<<mg, kRsThreads, 0, st>>>(ha);
+            }
             CX_HIP(hipGetLastError());
         }
     }

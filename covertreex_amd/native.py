@@ -152,6 +152,7 @@ class HipTree:
             # tiles per size instead of the reference's one entry per tile
             "block_sizes": [],
             "block_size_counts": {size: int(c) for size, c in zip((16, 32, 64), blocks)},
+            "hard_listed": int(out[_ffi.CX_QT_HARD_LISTED]),
         })
         return t
 

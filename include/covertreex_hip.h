@@ -248,6 +248,7 @@ cx_status cx_tree_build_telemetry(const cx_tree *t, int64_t *out, int64_t cap_ba
 #define CX_QT_FRONTIER 14       /* [CX_QT_LEVELS] frontier size per level, summed */
 #define CX_QT_EXPANDED 78       /* [CX_QT_LEVELS] children gathered per level */
 #define CX_QT_BLOCKS 142        /* [3] tiles of 16, 32, 64 candidates */
+#define CX_QT_HARD_LISTED 145   /* mixed: pairs the screen kept for the hard queries */
 cx_status cx_tree_set_query_telemetry(cx_tree *t, int32_t enable);
 /* *n_out = CX_QT_SIZE when the last call recorded telemetry, else 0. */
 cx_status cx_tree_last_query_telemetry(const cx_tree *t, int64_t *out, int32_t cap,

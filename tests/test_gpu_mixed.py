@@ -102,11 +102,16 @@ def test_c2_shape_sample(hip):
 
     rng = np.random.default_rng(0)
     x = rng.standard_normal((32768, 3)).astype(np.float32)
-    v, p, *_ = fp64.nystrom(x, inducing_count=256, seed=0, kernel_type=1)
+    v, p, _, _, _, ind = fp64.nystrom(x, inducing_count=256, seed=0, kernel_type=1)
     t = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
     q = np.arange(32768)
+    t.set_query_telemetry(True)
     idx, dst, err = t.knn_residual(q, 50, predecessor_mode=True, mode="mixed", return_error=True)
-    sample = np.random.default_rng(1).choice(32768, 256, replace=False)
+    # the inducing rows (p at the jitter floor) overflow the screen's lists and
+    # take the hard-query path: a quarter of them join the random sample
+    assert t.last_query_telemetry()["hard_queries"] > 100
+    sample = np.concatenate([np.random.default_rng(1).choice(32768, 256, replace=False),
+                             ind[1::4]])
     worst = _check(idx[sample], dst[sample], err[sample], x, v, p, sample, 50, True, 1)
     print(f"C2 sample max |d - d64| = {worst:.3e}", flush=True)
     # size-independent properties over all rows
