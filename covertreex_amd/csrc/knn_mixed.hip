@@ -17,14 +17,16 @@
 //                     the k-th exact squared distance;
 //             pass 2: every candidate with lb2 <= tau_q is appended to the
 //                     query's list (a superset of the exact k nearest).
-//   rescore   one workgroup per query: compensated FP32 (double-float)
-//             distance for each listed candidate, exact top-k, ascending
-//             output with a per-result error bound.  A query whose list
-//             overflowed its kCap slots is listed again, in full, into a pool
-//             sized by its pass-2 count, and rescored from there.
+//   rescore   one wavefront per query: compensated FP32 (Dot2 dot,
+//             double-float distance) for each listed candidate, exact top-k
+//             in registers, ascending output with a per-result error bound.
+//   hard      a query whose list overflowed its kCap slots is scored over all
+//             its valid candidates the same way, one wavefront per query and
+//             candidate segment, the segments' top-k lists merged after.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -68,27 +70,32 @@ __global__ void split_kernel(const float *__restrict__ v, const int64_t *__restr
     if (row >= n)
         return;
     const float *src = v + inv_order[row] * (int64_t)m;
-    double acc = 0.0;
+    float acc = 0.0f;
     for (int32_t c = threadIdx.x; c < mp; c += blockDim.x) {
         float x = c < m ? src[c] : 0.0f;
         uint16_t h = bf16_rne(x);
         uint16_t l = bf16_rne(x - bf16_to_f(h));
         vhi[row * (int64_t)mp + c] = h;
         vlo[row * (int64_t)mp + c] = l;
-        acc += (double)x * (double)x;
+        acc = __builtin_fmaf(x, x, acc);
     }
-    __shared__ double s_acc[64];
+    __shared__ float s_acc[64];
     for (int o = 32; o > 0; o >>= 1)
         acc += __shfl_xor(acc, o, 64);
     if ((threadIdx.x & 63) == 0)
         s_acc[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++)
+        float t = 0.0f;
+        const int nw = (int)(blockDim.x >> 6);
+        for (int w = 0; w < nw; w++)
             t += s_acc[w];
-        // round the norm up so products of norms bound |V_i . V_j| from above
-        vnorm[row] = (float)__builtin_sqrt(t) * (1.0f + 1.2e-7f) + 1e-30f;
+        // a sum of non-negative terms through `depth` roundings is within
+        // gamma_depth of the exact sum: scale it up by that, then round the
+        // norm up so products of norms bound |V_i . V_j| from above
+        const int depth = (mp + (int)blockDim.x - 1) / (int)blockDim.x + 6 + nw;
+        t *= 1.0f + (float)(depth + 2) * 1.2e-7f;
+        vnorm[row] = __builtin_sqrtf(t) * (1.0f + 1.2e-7f) + 1e-30f;
     }
 }
 
@@ -547,9 +554,11 @@ __global__ void tau_kernel(const int32_t *__restrict__ hist, int64_t s_begin, in
 
 // ---- exact scoring -------------------------------------------------------------
 // Listed candidates (rescore) and the candidates of hard queries are scored
-// exactly: FP64 dot, FP64 distance (finish64), and each query keeps its top-k
-// in the registers of one wavefront (WaveTopK).  Every kernel here runs its
-// wavefronts independently: no workgroup barriers.
+// exactly in FP32 with compensated arithmetic: a Dot2 dot (TwoProduct +
+// TwoSum per term) kept as a double-float, the kernel value, numerator,
+// division and square root in double-float (df.hpp), and each query keeps its
+// top-k in the registers of one wavefront (WaveTopK).  Every kernel here runs
+// its wavefronts independently: no workgroup barriers.
 
 constexpr int kWaves = 4;                       // wavefronts per workgroup
 constexpr int kRsThreads = 64 * kWaves;
@@ -566,9 +575,10 @@ static inline unsigned xcd_grid(int64_t items)
     return (unsigned)((b + kXcds - 1) / kXcds * kXcds);
 }
 
-// one scored pair: distance, candidate (dataset index), error bound
+// one scored pair: distance as a double-float (h, l), candidate (dataset
+// index), error bound
 struct Ent {
-    double d;
+    float h, l;
     int32_t j;
     float e;
 };
@@ -588,122 +598,119 @@ struct RescoreArgs {
     int64_t n, nq;
     int32_t d, m, kernel, pred, k;
     float var;
-    double inv_ls[kMaxD]; // 1 / lengthscale
+    float c_dot;          // dot error bound per |V_q||V_j| (dot_error_coeff)
+    float e_kern;         // double-float kernel value error bound (kernel_error)
+    float inv_ls_hi[kMaxD], inv_ls_lo[kMaxD];
     int64_t *out_idx;
     float *out_dist, *out_err;
     int64_t *counter;
 };
 
-// e^x for x <= 0 in FP64, from the mathematics: x = k ln2 + r with k the
-// integer nearest x / ln2 (r by two FMAs against ln 2 split into its nearest
-// double and the remainder, so |r| <= 0.3466 and r is accurate to about
-// 2^-53 |r|), e^r by its Taylor series through r^13/13! in Horner form
-// (truncation below 6e-18 relative; the Horner sums add at most
-// 26 * 2^-53 * e^|r| / e^r < 5.8e-15 relative), then the exact scaling by
-// 2^k.  Relative error below 7.1e-15; results under e^-690 are returned as 0.
-__device__ __forceinline__ double exp_neg64(double x)
+// Error of the compensated dot per |V_q||V_j| (Cauchy-Schwarz bounds
+// sum|V_q[c] V_j[c]|): each lane runs Dot2 (Ogita, Rump and Oishi 2005) over
+// at most n = ceil(m/8) terms in each of its chains, whose unrounded result
+// s + c is within gamma_n^2 sum|products| (gamma_n = n u / (1 - n u),
+// u = 2^-24); the chains and the eight lanes are then added as double-floats
+// (four additions, each within 2^-44 of the magnitudes added).
+static inline float dot_error_coeff(int32_t m)
 {
-    if (x < -690.0)
-        return 0.0;
-    constexpr double kLn2Hi = 0x1.62e42fefa39efp-1;  // ln 2 rounded to double
-    constexpr double kLn2Lo = 0x1.abc9e3b39803fp-56; // ln 2 - kLn2Hi
-    constexpr double kInvLn2 = 0x1.71547652b82fep+0;
-    const double kd = __builtin_rint(x * kInvLn2);
-    double r = __builtin_fma(-kd, kLn2Hi, x);
-    r = __builtin_fma(-kd, kLn2Lo, r);
-    double e = 1.0 / 6227020800.0; // 1/13!
-    e = __builtin_fma(e, r, 1.0 / 479001600.0);
-    e = __builtin_fma(e, r, 1.0 / 39916800.0);
-    e = __builtin_fma(e, r, 1.0 / 3628800.0);
-    e = __builtin_fma(e, r, 1.0 / 362880.0);
-    e = __builtin_fma(e, r, 1.0 / 40320.0);
-    e = __builtin_fma(e, r, 1.0 / 5040.0);
-    e = __builtin_fma(e, r, 1.0 / 720.0);
-    e = __builtin_fma(e, r, 1.0 / 120.0);
-    e = __builtin_fma(e, r, 1.0 / 24.0);
-    e = __builtin_fma(e, r, 1.0 / 6.0);
-    e = __builtin_fma(e, r, 0.5);
-    e = __builtin_fma(e, r, 1.0);
-    e = __builtin_fma(e, r, 1.0);
-    return __builtin_ldexp(e, (int)kd);
+    const double u = 1.0 / 16777216.0;
+    const double n = (double)((m + 7) / 8);
+    const double g = n * u / (1.0 - n * u);
+    return (float)((g * g + 4.0 * std::ldexp(1.0, -44)) * 1.01);
 }
 
-// The residual distance in FP64 from the dot V_q.V_j and a bound on that
-// dot's error; returns the distance and a bound on its error (err, which also
-// covers rounding the distance to FP32).  With u = 2^-53 and IEEE FP64
-// division, the square roots taken within 2u:
-// - d^2 = sum ((x_q - x_j) / l)^2: relative (d + 7) u;
-// - K: RBF var e^(-d^2/2), absolute var (0.37 (d + 7) u + u + 7.1e-15)
-//   (max x e^-x = 1/e); Matern-5/2 var (1 + a + a^2/3) e^-a, a = sqrt(5 d^2)
-//   with relative error ((d + 8)/2 + 2) u and max (1 + a + a^2/3)(2 + a) e^-a
-//   < 2.6: absolute var ((1.3 d + 21) u + 7.1e-15); both within
-//   eK = |var| ((1.3 d + 24) u + 7.2e-15);
-// - num = K - dot: eK + dot_err + u |num|;
-// - rho = num / sqrt(max(p_q p_j, 1e-18)) (the product exact): e_num / den
-//   plus 4u |rho|;
-// - s = 1 - |rho| (clamped to [0, 1]): plus u s;
-// - dist = sqrt(s): |sqrt(s) - sqrt(s')| <= min(sqrt(e_s), e_s / (sqrt(s) +
-//   sqrt(max(s - e_s, 0)))), plus 2u dist for the root itself.
-__device__ __forceinline__ double finish64(const RescoreArgs &A, int64_t qt, int64_t jt,
-                                          double dot, float dot_err, float *err)
+// Error of the double-float kernel value: d^2 from TwoSum differences times
+// 1/l as a double-float (relative (d + 2) 2^-45), df_exp within 2^-43
+// relative plus the argument's error |x| (d + 2) 2^-45 (RBF: max x e^-x =
+// 1/e; Matern-5/2: a = sqrt(5 d^2) within (d + 3) 2^-46, the polynomial and
+// product within 4 2^-44, and max (1 + a + a^2/3)(2 + a) e^-a < 2.6), the
+// product by var 2^-46: both kernels within |var| ((1.3 d + 4) 2^-45 + 8 2^-44).
+static inline float kernel_error(float var, int32_t d)
 {
-    constexpr double u = 0x1p-53;
+    return (float)(std::fabs((double)var) *
+                   ((1.3 * d + 4.0) * std::ldexp(1.0, -45) + 8.0 * std::ldexp(1.0, -44)) * 1.01);
+}
+
+// The residual distance from the double-float dot V_q.V_j and a bound on that
+// dot's error: kernel value, numerator, division and square root in
+// double-float.  Returns the distance and a bound on its error (err, which
+// also covers rounding the distance to FP32).
+// Error: e_num = dot_err + e_kern + 2^-44 (|K| + |dot|) (the subtraction);
+// the square root of max(p_q p_j, 1e-18) and the division add 2^-42 |rho|;
+// s = 1 - |rho| adds 2^-43; then |sqrt(s) - sqrt(s')| <= min(sqrt(e_s),
+// e_s / (sqrt(s) + sqrt(max(s - e_s, 0)))), plus 2^-43 sqrt(s) for df_sqrt.
+__device__ __forceinline__ df finish_distance(const RescoreArgs &A, int64_t qt, int64_t jt, df dot,
+                                              float dot_err, float *err)
+{
     const int32_t d = A.d;
     const float *xq = A.raw + qt * d, *xj = A.raw + jt * d;
-    double d2 = 0.0;
+    df d2 = {0.0f, 0.0f};
     for (int c = 0; c < d; c++) {
-        const double t = ((double)xq[c] - (double)xj[c]) * A.inv_ls[c];
-        d2 = __builtin_fma(t, t, d2);
+        const df diff = two_sum(xq[c], -xj[c]);
+        const df s = df_mul(diff, df{A.inv_ls_hi[c], A.inv_ls_lo[c]});
+        d2 = df_add(d2, df_mul(s, s));
     }
-    double kv;
+    df kv;
     if (A.kernel == 0) {
-        kv = exp_neg64(-0.5 * d2);
+        kv = df_exp(df_mul_f(d2, -0.5f));
     } else {
-        const double a = __builtin_sqrt(5.0 * d2);
-        kv = __builtin_fma(a, __builtin_fma(a, 1.0 / 3.0, 1.0), 1.0) * exp_neg64(-a);
+        const df a = df_sqrt(df_mul_f(d2, 5.0f));
+        constexpr DfConst third = split_const(1.0 / 3.0);
+        const df poly =
+            df_add(df_add(df{1.0f, 0.0f}, a), df_mul(df_mul(a, a), df{third.hi, third.lo}));
+        kv = df_mul(poly, df_exp(df_neg(a)));
     }
-    kv *= (double)A.var;
-    const double num = kv - dot;
-    double pp = (double)A.p[qt] * (double)A.p[jt];
-    pp = pp < 1e-18 ? 1e-18 : pp;
-    const double den = __builtin_sqrt(pp);
-    double rho = __builtin_fabs(num / den);
-    rho = rho > 1.0 ? 1.0 : rho;
-    const double sv = 1.0 - rho;
-    const double dist = __builtin_sqrt(sv);
-    const double e_k =
-        __builtin_fabs((double)A.var) * ((1.3 * d + 24.0) * u + 7.2e-15);
-    const double e_num = e_k + (double)dot_err + u * __builtin_fabs(num);
-    const double e_s = (e_num / den) * (1.0 + 8.0 * u) + 4.0 * u * rho + u * sv + 1e-300;
-    const double lo = sv - e_s > 0.0 ? __builtin_sqrt(sv - e_s) : 0.0;
-    double e_d = dist > 0.0 ? e_s / (dist + lo) : __builtin_sqrt(e_s);
-    const double e_sq = __builtin_sqrt(e_s);
+    kv = df_mul_f(kv, A.var);
+    const df num = df_sub(kv, dot);
+    df pp = two_prod(A.p[qt], A.p[jt]);
+    if (pp.hi < 1e-18f)
+        pp = {1e-18f, 0.0f};
+    const df den = df_sqrt(pp);
+    df rho = df_div(num, den);
+    if (rho.hi < 0.0f)
+        rho = df_neg(rho);
+    if (rho.hi > 1.0f || (rho.hi == 1.0f && rho.lo > 0.0f)) // |rho| clamped to 1
+        rho = {1.0f, 0.0f};
+    const df sv = df_sub(df{1.0f, 0.0f}, rho);
+    const df dist = df_sqrt(sv);
+    // the bound itself in FP32 with hardware reciprocal and root (about
+    // 1 ulp each), covered by the 1.01 and 1.001 factors
+    const float e_num = dot_err + A.e_kern +
+                        5.7e-14f * (__builtin_fabsf(kv.hi) + __builtin_fabsf(dot.hi));
+    const float e_s = e_num * __builtin_amdgcn_rcpf(den.hi) * 1.001f + rho.hi * 2.3e-13f +
+                      1.2e-13f + 1e-37f;
+    const float sd = dist.hi, s_hi = sv.hi;
+    const float lo = s_hi - e_s > 0.0f ? __builtin_amdgcn_sqrtf(s_hi - e_s) * 0.999f : 0.0f;
+    const float e_sq = __builtin_amdgcn_sqrtf(e_s) * 1.001f;
+    float e_d = sd > 0.0f ? e_s * __builtin_amdgcn_rcpf(sd + lo) * 1.001f : e_sq;
     e_d = e_d < e_sq ? e_d : e_sq;
-    *err = (float)((e_d + 2.0 * u * dist) * 1.01 + dist * 6e-8);
+    *err = (e_d + sd * 1.2e-13f) * 1.01f + sd * 6e-8f;
     return dist;
 }
 
-__device__ __forceinline__ float dot64_error(int32_t m, float norm_q, float norm_j)
+// One Dot2 step: TwoProduct a*b, TwoSum into s, the two errors into c.
+__device__ __forceinline__ void dot2_step(float a, float b, float &s, float &c)
 {
-    return ((float)(m / 8 + 6) * 1.12e-16f + 3.56e-15f) * norm_q * norm_j;
+    const float ph = a * b;
+    const float pl = __builtin_fmaf(a, b, -ph);
+    const float t = s + ph;
+    const float bb = t - s;
+    c += ((s - (t - bb)) + (ph - bb)) + pl;
+    s = t;
 }
 
-// Dot V_q . V_j in FP64, eight lanes per row pair.  A product of two f32
-// values is exact in FP64, so the only rounding is in the sums: each lane
-// runs FMA chains over its columns and the group adds its lanes in three
-// steps.  With u = 2^-53 the error is at most (m/8 + 5) u sum|V_q[c] V_j[c]|
-// <= (m/8 + 5) u |V_q| |V_j|, which dot64_error bounds.  This form reads the
-// query row from memory (ranks that do not fit the staged forms).
-__device__ __forceinline__ double group_dot64(const float *__restrict__ vq,
-                                              const float *__restrict__ vj, int32_t m, int sub)
+// two Dot2 chains of a lane, then the eight lanes of the group, as one
+// double-float
+__device__ __forceinline__ df dot2_reduce(float s0, float c0, float s1, float c1)
 {
-    double s0 = 0.0;
-    for (int32_t i = sub; i < m; i += 8)
-        s0 = __builtin_fma((double)vq[i], (double)vj[i], s0);
+    df x = df_add(two_sum(s0, c0), two_sum(s1, c1));
 #pragma unroll
-    for (int o = 1; o < 8; o <<= 1)
-        s0 += __shfl_xor(s0, o, 64);
-    return s0;
+    for (int o = 1; o < 8; o <<= 1) {
+        const df y = {__shfl_xor(x.hi, o, 64), __shfl_xor(x.lo, o, 64)};
+        x = df_add(x, y);
+    }
+    return x;
 }
 
 // row[col .. col + 3] as a float4, zero when col >= m (m % 4 == 0): the load
@@ -716,48 +723,6 @@ __device__ __forceinline__ float4 load_col4(const float *__restrict__ row, int32
     return in ? v : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// Two FP64 dots per eight-lane group at once: rows vj0 and vj1 against the
-// staged query rows qa and qb (QT = double, or float; zero past m up to a
-// multiple of 64 columns; rows float4-aligned), every load of both rows
-// issued before the first FMA of a 64-column step.  Error as group_dot64's.
-template <typename QT>
-__device__ __forceinline__ void dot_pair(const QT *qa, const QT *qb, const float *vj0,
-                                         const float *vj1, int32_t m, int sub, double &t0,
-                                         double &t1)
-{
-    double p0 = 0.0, p1 = 0.0, r0 = 0.0, r1 = 0.0;
-#pragma unroll 1
-    for (int32_t cb = 0; cb < m; cb += 64) {
-        float4 x[2], y[2];
-#pragma unroll
-        for (int t = 0; t < 2; t++) {
-            const int32_t col = cb + 4 * sub + 32 * t;
-            x[t] = load_col4(vj0, col, m);
-            y[t] = load_col4(vj1, col, m);
-        }
-#pragma unroll
-        for (int t = 0; t < 2; t++) {
-            const int32_t col = cb + 4 * sub + 32 * t;
-            const QT *a = qa + col, *b = qb + col;
-            p0 = __builtin_fma((double)a[0], (double)x[t].x, p0);
-            p1 = __builtin_fma((double)a[1], (double)x[t].y, p1);
-            p0 = __builtin_fma((double)a[2], (double)x[t].z, p0);
-            p1 = __builtin_fma((double)a[3], (double)x[t].w, p1);
-            r0 = __builtin_fma((double)b[0], (double)y[t].x, r0);
-            r1 = __builtin_fma((double)b[1], (double)y[t].y, r1);
-            r0 = __builtin_fma((double)b[2], (double)y[t].z, r0);
-            r1 = __builtin_fma((double)b[3], (double)y[t].w, r1);
-        }
-    }
-    t0 = p0 + p1;
-    t1 = r0 + r1;
-#pragma unroll
-    for (int o = 1; o < 8; o <<= 1) {
-        t0 += __shfl_xor(t0, o, 64);
-        t1 += __shfl_xor(t1, o, 64);
-    }
-}
-
 // orders the wavefront's own LDS accesses (no workgroup barrier)
 __device__ __forceinline__ void wave_sync()
 {
@@ -766,28 +731,29 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// entries order by (distance, index); the sentinel (+inf, INT_MAX) is last
+// entries order by (distance hi, lo, index); the sentinel is last
 __device__ __forceinline__ bool ent_lt(const Ent &a, const Ent &b)
 {
-    return a.d < b.d || (a.d == b.d && a.j < b.j);
+    return a.h < b.h || (a.h == b.h && (a.l < b.l || (a.l == b.l && a.j < b.j)));
 }
 
 __device__ __forceinline__ Ent ent_none()
 {
-    return Ent{__builtin_huge_val(), 0x7fffffff, 0.0f};
+    return Ent{__builtin_huge_valf(), 0.0f, 0x7fffffff, 0.0f};
 }
 
 __device__ __forceinline__ Ent ent_shfl(const Ent &a, int src)
 {
-    return Ent{__shfl(a.d, src, 64), __shfl(a.j, src, 64), __shfl(a.e, src, 64)};
+    return Ent{__shfl(a.h, src, 64), __shfl(a.l, src, 64), __shfl(a.j, src, 64),
+               __shfl(a.e, src, 64)};
 }
 
 // compare-exchange with lane ^ mask: this lane keeps the smaller of the two
 // when take_min, else the larger
 __device__ __forceinline__ void ent_xchg(Ent &a, int mask, bool take_min)
 {
-    const Ent o = Ent{__shfl_xor(a.d, mask, 64), __shfl_xor(a.j, mask, 64),
-                      __shfl_xor(a.e, mask, 64)};
+    const Ent o = Ent{__shfl_xor(a.h, mask, 64), __shfl_xor(a.l, mask, 64),
+                      __shfl_xor(a.j, mask, 64), __shfl_xor(a.e, mask, 64)};
     if (ent_lt(o, a) == take_min)
         a = o;
 }
@@ -859,7 +825,7 @@ template <int KB> struct WaveTopK {
             p[b] = q[b];
     }
 
-    // the first k entries: rank r to dst[r] via `put(r, entry)`
+    // the first k entries: rank r to `put(r, entry)`
     template <typename F> __device__ __forceinline__ void emit(int k, int lane, F put) const
     {
 #pragma unroll
@@ -875,43 +841,36 @@ __device__ __forceinline__ void put_output(const RescoreArgs &A, int64_t pos_out
                                            const Ent &e)
 {
     const int64_t o = pos_out * A.k + r;
-    const bool ok = e.d < __builtin_huge_val();
+    const bool ok = e.h < __builtin_huge_valf();
     A.out_idx[o] = ok ? (int64_t)e.j : -1;
-    A.out_dist[o] = ok ? (float)e.d : 3.40282347e38f;
+    A.out_dist[o] = ok ? e.h + e.l : 3.40282347e38f;
     if (A.out_err)
         A.out_err[o] = ok ? e.e : 0.0f;
 }
 
-// query rows staged per wavefront: 256 doubles, or 512 floats
-constexpr int kStageM = 256;
+// query rows staged per wavefront (floats, zero past m)
+constexpr int kStageM = 512;
 
-// 2: doubles (m <= 256), 1: floats (m <= 512), 0: not staged
-__device__ __forceinline__ int stage_mode(int32_t m)
+__device__ __forceinline__ bool staged_rank(int32_t m) { return (m & 3) == 0 && m <= kStageM; }
+
+__device__ __forceinline__ void stage_row(const float *__restrict__ src, int32_t m, float *dst,
+                                          int lane)
 {
-    return (m & 3) != 0 ? 0 : m <= 256 ? 2 : m <= 512 ? 1 : 0;
+    if (staged_rank(m))
+        for (int c = lane; c < kStageM; c += 64)
+            dst[c] = c < m ? src[c] : 0.0f;
 }
 
-__device__ __forceinline__ void stage_row(const float *__restrict__ src, int32_t m, int mode,
-                                          double *dst, int lane)
-{
-    if (mode == 2)
-        for (int c = lane; c < 256; c += 64)
-            dst[c] = c < m ? (double)src[c] : 0.0;
-    else if (mode == 1)
-        for (int c = lane; c < 512; c += 64)
-            ((float *)dst)[c] = c < m ? src[c] : 0.0f;
-}
-
-// Dots of one wavefront's batch of up to 64 candidates (tree rows jt, one
-// per lane) against the query row staged as doubles (rank <= 256, m % 4 == 0,
-// zero past m): lane group g (eight lanes) takes candidates g, g + 8, ...,
-// g + 56, each lane covering columns 4 sub + 32 t (t < 8) as float4 pieces.
-// The next candidate's row is loaded while the current one is multiplied, so
-// one batch waits on memory about once.  Dots go to s_dot[e]; error as
-// group_dot64's.
-__device__ __forceinline__ void batch_dots_staged(const float *__restrict__ v, int32_t m,
-                                                  const double *vqd, int32_t jt, int n_in,
-                                                  double *s_dot, int lane)
+// Dots of one wavefront's batch of up to 64 candidates (tree rows jt, one per
+// lane) against the staged query row (rank <= 256, m % 4 == 0): lane group g
+// (eight lanes) takes candidates g, g + 8, ..., g + 56, each lane covering
+// columns 4 sub + 32 t (t < 8) as float4 pieces in two Dot2 chains.  The next
+// candidate's row is loaded while the current one is multiplied, so a batch
+// waits on memory about once.  Dots go to (s_dh, s_dl)[e]; error bound
+// c_dot |V_q||V_j|.
+__device__ __forceinline__ void batch_dots_256(const float *__restrict__ v, int32_t m,
+                                               const float *vq, int32_t jt, int n_in, float *s_dh,
+                                               float *s_dl, int lane)
 {
     const int grp = lane >> 3, sub = lane & 7;
     int32_t jts[8];
@@ -922,10 +881,8 @@ __device__ __forceinline__ void batch_dots_staged(const float *__restrict__ v, i
     if (grp < n_in) {
         const float *row = v + (int64_t)jts[0] * m;
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
-            const int32_t col = 4 * sub + 32 * t;
-            cur[t] = load_col4(row, col, m);
-        }
+        for (int t = 0; t < 8; t++)
+            cur[t] = load_col4(row, 4 * sub + 32 * t, m);
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
@@ -933,29 +890,24 @@ __device__ __forceinline__ void batch_dots_staged(const float *__restrict__ v, i
         if (u < 7 && e + 8 < n_in) {
             const float *row = v + (int64_t)jts[u + 1] * m;
 #pragma unroll
-            for (int t = 0; t < 8; t++) {
-                const int32_t col = 4 * sub + 32 * t;
-                nxt[t] = load_col4(row, col, m);
-            }
+            for (int t = 0; t < 8; t++)
+                nxt[t] = load_col4(row, 4 * sub + 32 * t, m);
         }
         if (e < n_in) {
-            double p0 = 0.0, p1 = 0.0;
+            float s0 = 0.0f, c0 = 0.0f, s1 = 0.0f, c1 = 0.0f;
 #pragma unroll
             for (int t = 0; t < 8; t++) {
-                const int32_t col = 4 * sub + 32 * t;
-                const double2 a0 = *(const double2 *)(vqd + col);
-                const double2 a1 = *(const double2 *)(vqd + col + 2);
-                p0 = __builtin_fma(a0.x, (double)cur[t].x, p0);
-                p1 = __builtin_fma(a0.y, (double)cur[t].y, p1);
-                p0 = __builtin_fma(a1.x, (double)cur[t].z, p0);
-                p1 = __builtin_fma(a1.y, (double)cur[t].w, p1);
+                const float4 a = *(const float4 *)(vq + 4 * sub + 32 * t);
+                dot2_step(a.x, cur[t].x, s0, c0);
+                dot2_step(a.y, cur[t].y, s1, c1);
+                dot2_step(a.z, cur[t].z, s0, c0);
+                dot2_step(a.w, cur[t].w, s1, c1);
             }
-            double t0 = p0 + p1;
-#pragma unroll
-            for (int o = 1; o < 8; o <<= 1)
-                t0 += __shfl_xor(t0, o, 64);
-            if (sub == 0)
-                s_dot[e] = t0;
+            const df x = dot2_reduce(s0, c0, s1, c1);
+            if (sub == 0) {
+                s_dh[e] = x.hi;
+                s_dl[e] = x.lo;
+            }
         }
         if (u < 7) {
 #pragma unroll
@@ -965,34 +917,48 @@ __device__ __forceinline__ void batch_dots_staged(const float *__restrict__ v, i
     }
 }
 
-// Dots of a batch for the other ranks: the float-staged query (rank <= 512,
-// m % 4 == 0) two candidates per lane group at a time, or the query row read
-// from memory.
+// Dots of a batch for the other ranks: the staged query row (rank <= 512,
+// m % 4 == 0) one candidate per lane group at a time in 128-column steps, or
+// the query row read from memory (any rank, columns sub + 8 t).
 __device__ __forceinline__ void batch_dots_other(const float *__restrict__ v, int32_t m,
-                                                 int mode, const double *vqd, const float *vq_mem,
-                                                 int32_t jt, int n_in, double *s_dot, int lane)
+                                                 const float *vq, const float *vq_mem,
+                                                 int32_t jt, int n_in, float *s_dh, float *s_dl,
+                                                 int lane)
 {
     const int grp = lane >> 3, sub = lane & 7;
+    const bool staged = staged_rank(m);
 #pragma unroll 1
-    for (int u = 0; u < 64; u += 16) {
-        const int ea = u + grp, eb = ea + 8;
-        const int32_t ja = __shfl(jt, ea, 64), jb = __shfl(jt, eb, 64);
-        if (ea >= n_in)
+    for (int u = 0; u < 64; u += 8) {
+        const int e = u + grp;
+        const int32_t je = __shfl(jt, e, 64);
+        if (e >= n_in)
             continue;
-        const float *va = v + (int64_t)ja * m;
-        const float *vb = v + (int64_t)(eb < n_in ? jb : ja) * m;
-        double ta, tb;
-        if (mode == 1) {
-            const float *vqs = (const float *)vqd;
-            dot_pair<float>(vqs, vqs, va, vb, m, sub, ta, tb);
+        const float *row = v + (int64_t)je * m;
+        float s0 = 0.0f, c0 = 0.0f, s1 = 0.0f, c1 = 0.0f;
+        if (staged) {
+#pragma unroll 1
+            for (int32_t cb = 0; cb < m; cb += 128) {
+                float4 x[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    x[t] = load_col4(row, cb + 4 * sub + 32 * t, m);
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const float4 a = *(const float4 *)(vq + cb + 4 * sub + 32 * t);
+                    dot2_step(a.x, x[t].x, s0, c0);
+                    dot2_step(a.y, x[t].y, s1, c1);
+                    dot2_step(a.z, x[t].z, s0, c0);
+                    dot2_step(a.w, x[t].w, s1, c1);
+                }
+            }
         } else {
-            ta = group_dot64(vq_mem, va, m, sub);
-            tb = group_dot64(vq_mem, vb, m, sub);
+            for (int32_t col = sub; col < m; col += 8)
+                dot2_step(vq_mem[col], row[col], s0, c0);
         }
+        const df x = dot2_reduce(s0, c0, s1, c1);
         if (sub == 0) {
-            s_dot[ea] = ta;
-            if (eb < n_in)
-                s_dot[eb] = tb;
+            s_dh[e] = x.hi;
+            s_dl[e] = x.lo;
         }
     }
 }
@@ -1001,31 +967,35 @@ __device__ __forceinline__ void batch_dots_other(const float *__restrict__ v, in
 // list `lst` (dataset indices) when given, else the range base + [0, count),
 // those at or past `lim` skipped; batches of 64 offered to `top`.
 template <int KB>
-__device__ __forceinline__ void score_query(const RescoreArgs &A, int mode, int64_t qt,
-                                            int32_t qrow, float qn, const int32_t *lst,
-                                            int64_t base, int64_t count, int64_t lim,
-                                            const double *vqd, double *s_dot, WaveTopK<KB> &top,
-                                            int lane)
+__device__ __forceinline__ void score_query(const RescoreArgs &A, int64_t qt, int32_t qrow,
+                                            float qn, const int32_t *lst, int64_t base,
+                                            int64_t count, int64_t lim, const float *vq,
+                                            float *s_dh, float *s_dl, WaveTopK<KB> &top, int lane)
 {
     const int32_t m = A.m;
+    const bool fast = (m & 3) == 0 && m <= 256;
     for (int64_t c0 = 0; c0 < count; c0 += 64) {
         const int n_in = count - c0 < 64 ? (int)(count - c0) : 64;
         const bool valid = lane < n_in;
         const int64_t j = valid ? (lst ? (int64_t)lst[c0 + lane] : base + c0 + lane) : 0;
         const int32_t jt = valid ? (int32_t)A.inv_order[j] : 0;
-        if (mode == 2)
-            batch_dots_staged(A.v, m, vqd, jt, n_in, s_dot, lane);
+        if (fast)
+            batch_dots_256(A.v, m, vq, jt, n_in, s_dh, s_dl, lane);
         else
-            batch_dots_other(A.v, m, mode, vqd, A.v + qt * m, jt, n_in, s_dot, lane);
+            batch_dots_other(A.v, m, vq, A.v + qt * m, jt, n_in, s_dh, s_dl, lane);
         wave_sync();
         Ent c = ent_none();
         if (valid && j < lim) {
             c.j = (int32_t)j;
             if (j == qrow) {
-                c.d = 0.0;
+                c.h = 0.0f;
+                c.l = 0.0f;
                 c.e = 0.0f;
             } else {
-                c.d = finish64(A, qt, jt, s_dot[lane], dot64_error(m, qn, A.vnorm[j]), &c.e);
+                const df dd = finish_distance(A, qt, jt, df{s_dh[lane], s_dl[lane]},
+                                              A.c_dot * qn * A.vnorm[j], &c.e);
+                c.h = dd.hi;
+                c.l = dd.lo;
             }
         }
         wave_sync();
@@ -1040,11 +1010,10 @@ __device__ __forceinline__ void score_query(const RescoreArgs &A, int mode, int6
 template <int KB>
 __global__ void __launch_bounds__(kRsThreads, 3) rescore_kernel(RescoreArgs A)
 {
-    __shared__ __attribute__((aligned(16))) double s_vq[kWaves][kStageM];
-    __shared__ double s_dot[kWaves][64];
+    __shared__ __attribute__((aligned(16))) float s_vq[kWaves][kStageM];
+    __shared__ float s_dh[kWaves][64], s_dl[kWaves][64];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int32_t m = A.m, k = A.k;
-    const int mode = stage_mode(m);
     const int x = blockIdx.x % kXcds;
     for (;;) {
         int64_t local = 0;
@@ -1062,12 +1031,12 @@ __global__ void __launch_bounds__(kRsThreads, 3) rescore_kernel(RescoreArgs A)
         const int32_t qrow = A.qrow[s];
         const int64_t qt = A.inv_order[qrow];
         wave_sync();
-        stage_row(A.v + qt * m, m, mode, s_vq[w], lane);
+        stage_row(A.v + qt * m, m, s_vq[w], lane);
         wave_sync();
         WaveTopK<KB> top;
         top.init();
-        score_query<KB>(A, mode, qt, qrow, A.vnorm[qrow], A.list + s * kCap, 0, count,
-                        A.n, s_vq[w], s_dot[w], top, lane);
+        score_query<KB>(A, qt, qrow, A.vnorm[qrow], A.list + s * kCap, 0, count, A.n, s_vq[w],
+                        s_dh[w], s_dl[w], top, lane);
         const int64_t pos = A.qperm[s];
         top.emit(k, lane, [&](int r, const Ent &e) { put_output(A, pos, r, e); });
     }
@@ -1102,12 +1071,11 @@ __device__ __forceinline__ int64_t hard_limit(const RescoreArgs &A, int64_t s)
 template <int KB>
 __global__ void __launch_bounds__(kRsThreads, 3) hard_tile_kernel(HardArgs H)
 {
-    __shared__ __attribute__((aligned(16))) double s_vq[kWaves][kStageM];
-    __shared__ double s_dot[kWaves][64];
+    __shared__ __attribute__((aligned(16))) float s_vq[kWaves][kStageM];
+    __shared__ float s_dh[kWaves][64], s_dl[kWaves][64];
     const RescoreArgs &A = H.r;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int32_t m = A.m, k = A.k;
-    const int mode = stage_mode(m);
     const int x = blockIdx.x % kXcds;
     for (;;) {
         int64_t local = 0;
@@ -1129,12 +1097,12 @@ __global__ void __launch_bounds__(kRsThreads, 3) hard_tile_kernel(HardArgs H)
         const int32_t qrow = A.qrow[s];
         const int64_t qt = A.inv_order[qrow];
         wave_sync();
-        stage_row(A.v + qt * m, m, mode, s_vq[w], lane);
+        stage_row(A.v + qt * m, m, s_vq[w], lane);
         wave_sync();
         WaveTopK<KB> top;
         top.init();
-        score_query<KB>(A, mode, qt, qrow, A.vnorm[qrow], nullptr, beg, end - beg, lim,
-                        s_vq[w], s_dot[w], top, lane);
+        score_query<KB>(A, qt, qrow, A.vnorm[qrow], nullptr, beg, end - beg, lim, s_vq[w],
+                        s_dh[w], s_dl[w], top, lane);
         HardEntry *out = H.part + ((int64_t)hl * H.nseg + seg) * k;
         top.emit(k, lane, [&](int r, const Ent &e) { out[r] = e; });
     }
@@ -1441,8 +1409,13 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     ra.pred = predecessor ? 1 : 0;
     ra.k = k;
     ra.var = t.var;
-    for (int c = 0; c < t.d; c++)
-        ra.inv_ls[c] = 1.0 / (double)t.ls[c];
+    ra.c_dot = dot_error_coeff(t.m);
+    ra.e_kern = kernel_error(t.var, t.d);
+    for (int c = 0; c < t.d; c++) {
+        const double il = 1.0 / (double)t.ls[c];
+        ra.inv_ls_hi[c] = (float)il;
+        ra.inv_ls_lo[c] = (float)(il - (double)(float)il);
+    }
     ra.out_idx = d_idx;
     ra.out_dist = d_dist;
     ra.out_err = d_err;
