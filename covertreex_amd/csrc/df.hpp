@@ -88,7 +88,7 @@ __device__ __forceinline__ df df_div(df a, df b)
 }
 
 // sqrt(a): the hardware root of a.hi (about 1 ulp, v_sqrt_f32) corrected by
-// one Newton step on the double-float remainder, within 2^-44 relative.
+// one Newton step on the double-float remainder, within 2^-43 relative.
 __device__ __forceinline__ df df_sqrt(df a)
 {
     if (!(a.hi > 0.0f))
