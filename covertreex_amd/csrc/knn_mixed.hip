@@ -7,7 +7,7 @@
 //
 //   prep      V -> dataset-order BF16 hi/lo split (V = hi + lo + r, |r| <=
 //             2^-18 |V|), row norms |V_i| rounded up.
-//   screen    one workgroup per 128-query tile x 2048-candidate segment, eight
+//   screen    one workgroup per 64-query tile x 2048-candidate segment, four
 //             wavefronts of 32 candidates x 64 queries; V.V on MFMA
 //             (v_mfma_f32_32x32x16_bf16) as hi.hi + (hi.lo + lo.hi) in two
 //             FP32 accumulators, kernel value in FP32, and a certified
@@ -38,7 +38,7 @@ namespace cx {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-constexpr int kQT = 128;     // queries per workgroup tile
+constexpr int kQT = 64;      // queries per workgroup tile
 constexpr int kCT = 128;     // candidates per candidate tile
 constexpr int kKC = 32;      // K chunk staged in LDS
 constexpr int kKP = kKC + 8; // padded LDS row (bf16)
@@ -48,7 +48,8 @@ constexpr int kCap = 2048;   // listed candidates per query before overflow
 constexpr int kMaxD = 16;
 constexpr int kMixMaxK = 256;
 constexpr int64_t kSeg = 16 * kCT; // candidates per workgroup (grid y)
-constexpr int kScreenThreads = 512; // eight wavefronts
+constexpr int kScreenWaves = 4;
+constexpr int kScreenThreads = 64 * kScreenWaves;
 
 // ---- prep ------------------------------------------------------------------
 
@@ -163,10 +164,10 @@ template <int KT> __device__ __forceinline__ float fast_kernel(float d2, const S
 // wavefront stores and reloads 2 KiB contiguous per candidate tile.
 __device__ __forceinline__ int64_t code_word(int64_t blk, int ct, int wid, int lane)
 {
-    return (((blk * (kSeg / kCT) + ct) * 8 + wid) * 64 + lane) * 8;
+    return (((blk * (kSeg / kCT) + ct) * kScreenWaves + wid) * 64 + lane) * 8;
 }
 
-// Per-lane query columns of a 128-query tile (shared by the screen and the
+// Per-lane query columns of a query tile (shared by the screen and the
 // filter): column, Vecchia limit on the candidate index, own dataset row.
 __device__ __forceinline__ void tile_queries(const ScreenArgs &A, int64_t s0, int qh, int lane,
                                              int32_t *qcol, int32_t *qlim, int32_t *qself,
@@ -201,10 +202,10 @@ __device__ __forceinline__ void tile_range(const ScreenArgs &A, int64_t s0, int3
     *c_end = e < b + (int32_t)kSeg ? e : b + (int32_t)kSeg;
 }
 
-// Pass 1.  One workgroup = 128 candidates (MFMA rows) x 128 queries (MFMA
-// columns) per candidate tile, looping over a 2048-candidate segment.  Eight
-// wavefronts: wavefront w owns candidate rows 32*(w&3).. and query columns
-// 64*(w>>2).., i.e. two 32x32 output blocks, each with a hi.hi and a cross
+// Pass 1.  One workgroup = 128 candidates (MFMA rows) x 64 queries (MFMA
+// columns) per candidate tile, looping over a 2048-candidate segment.  Four
+// wavefronts: wavefront w owns candidate rows 32*w.. and all 64 query
+// columns, i.e. two 32x32 output blocks, each with a hi.hi and a cross
 // accumulator.  Per pair: the certified interval of |rho|; its upper end
 // feeds the query's histogram, its lower end, as a squared-distance lower
 // bound, is stored as a one-byte code for the filter.
@@ -212,7 +213,7 @@ __device__ __forceinline__ void tile_range(const ScreenArgs &A, int64_t s0, int3
 // registers); DD == 0 reads them from LDS for any d <= kMaxD.  KT is the
 // kernel (0 RBF, 1 Matern-5/2).  Coordinates arrive scaled by 1/lengthscale.
 template <int DD, int KT>
-__global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
+__global__ void __launch_bounds__(kScreenThreads, 3) screen_kernel(ScreenArgs A)
 {
     __shared__ __attribute__((aligned(16))) uint16_t sAh[kCT * kKP];
     __shared__ __attribute__((aligned(16))) uint16_t sAl[kCT * kKP];
@@ -227,7 +228,7 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
     extern __shared__ __attribute__((aligned(16))) float sDyn[];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int cb = wid & 3, qh = wid >> 2;
+    const int cb = wid, qh = 0;
     // blockIdx.x = query tile within the launch, heavy tiles first
     const int32_t tl = A.n_tiles - 1 - (int32_t)blockIdx.x;
     const int64_t s0 = (int64_t)(A.tile0 + tl) * kQT;
@@ -286,30 +287,37 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
             acc_h[b] = floatx16{};
             acc_x[b] = floatx16{};
         }
+        // stage 128 candidate + 64 query rows x 32 bf16, hi and lo: 1536
+        // segments of 16 B, six per thread (candidate rows tid/4 and 64 +
+        // tid/4, query row tid/4; byte offsets fit 32 bits, which the driver
+        // checks); the next K chunk's loads fly while this one is multiplied
+        const uint32_t seg8 = (uint32_t)(tid & 3) * 8u;
+        const uint32_t rc0 = (uint32_t)(c0 + (tid >> 2)) * (uint32_t)mp + seg8;
+        const uint32_t rc1 = rc0 + 64u * (uint32_t)mp;
+        const uint32_t rq = (uint32_t)sQrow[tid >> 2] * (uint32_t)mp + seg8;
+        const char *vh = (const char *)A.vhi, *vl = (const char *)A.vlo;
+        uint4 v0 = *(const uint4 *)(vh + rc0 * 2u), v1 = *(const uint4 *)(vh + rc1 * 2u);
+        uint4 v2 = *(const uint4 *)(vl + rc0 * 2u), v3 = *(const uint4 *)(vl + rc1 * 2u);
+        uint4 v4 = *(const uint4 *)(vh + rq * 2u), v5 = *(const uint4 *)(vl + rq * 2u);
+        const int srow = tid >> 2, sseg = (tid & 3) * 8;
         for (int kc = 0; kc < mp; kc += kKC) {
             __syncthreads();
-            // stage 128 candidate + 128 query rows x 32 bf16, hi and lo:
-            // 2048 segments of 16 B, four per thread
-            uint4 val[4];
-#pragma unroll
-            for (int it = 0; it < 4; it++) {
-                const int idx = tid + kScreenThreads * it;
-                const int side = idx >> 10, part = (idx >> 9) & 1;
-                const int row = (idx >> 2) & 127, seg = idx & 3;
-                const int64_t vrow = side ? (int64_t)sQrow[row] : (int64_t)(c0 + row);
-                const uint16_t *src = (part ? A.vlo : A.vhi) + vrow * mp + kc + seg * 8;
-                val[it] = *(const uint4 *)src;
-            }
-#pragma unroll
-            for (int it = 0; it < 4; it++) {
-                const int idx = tid + kScreenThreads * it;
-                const int side = idx >> 10, part = (idx >> 9) & 1;
-                const int row = (idx >> 2) & 127, seg = idx & 3;
-                uint16_t *dst = (side ? (part ? sBl : sBh) : (part ? sAl : sAh)) + row * kKP +
-                                seg * 8;
-                *(uint4 *)dst = val[it];
-            }
+            *(uint4 *)(sAh + srow * kKP + sseg) = v0;
+            *(uint4 *)(sAh + (64 + srow) * kKP + sseg) = v1;
+            *(uint4 *)(sAl + srow * kKP + sseg) = v2;
+            *(uint4 *)(sAl + (64 + srow) * kKP + sseg) = v3;
+            *(uint4 *)(sBh + srow * kKP + sseg) = v4;
+            *(uint4 *)(sBl + srow * kKP + sseg) = v5;
             __syncthreads();
+            if (kc + kKC < mp) {
+                const uint32_t o = (uint32_t)(kc + kKC) * 2u;
+                v0 = *(const uint4 *)(vh + rc0 * 2u + o);
+                v1 = *(const uint4 *)(vh + rc1 * 2u + o);
+                v2 = *(const uint4 *)(vl + rc0 * 2u + o);
+                v3 = *(const uint4 *)(vl + rc1 * 2u + o);
+                v4 = *(const uint4 *)(vh + rq * 2u + o);
+                v5 = *(const uint4 *)(vl + rq * 2u + o);
+            }
 #pragma unroll
             for (int ks = 0; ks < kKC; ks += 16) {
                 const int koff = ks + 8 * (lane >> 5);
@@ -440,7 +448,7 @@ __global__ void __launch_bounds__(kScreenThreads, 4) screen_kernel(ScreenArgs A)
 __global__ void __launch_bounds__(kScreenThreads) filter_kernel(ScreenArgs A)
 {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int cb = wid & 3, qh = wid >> 2;
+    const int cb = wid, qh = 0;
     const int32_t tl = A.n_tiles - 1 - (int32_t)blockIdx.x;
     const int64_t s0 = (int64_t)(A.tile0 + tl) * kQT;
     const int64_t blk = (int64_t)tl * A.n_seg + blockIdx.y;
@@ -1221,6 +1229,9 @@ static void prepare_mixed(TreeState &t)
     const int64_t n = t.n;
     t.mp = ((t.m + kKC - 1) / kKC) * kKC;
     const int64_t npad = ((n + kCT - 1) / kCT) * kCT + kCT;
+    // the screen addresses the BF16 rows with 32-bit byte offsets
+    CX_REQUIRE(npad * t.mp * 2 < ((int64_t)1 << 32),
+               "mixed-precision kNN supports N * rank below 2^31 (BF16 row offsets)");
     t.vhi.alloc((size_t)npad * t.mp);
     t.vlo.alloc((size_t)npad * t.mp);
     CX_HIP(hipMemsetAsync(t.vhi.ptr, 0, t.vhi.bytes(), st));
@@ -1357,7 +1368,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     sa.list = list;
     const int32_t n_tiles = (int32_t)((nq + kQT - 1) / kQT);
     sa.n_seg = (int32_t)((n + kSeg - 1) / kSeg);
-    const size_t dyn_lds = sizeof(float) * 2 * (size_t)t.d * kCT;
+    const size_t dyn_lds = sizeof(float) * (size_t)t.d * (kCT + kQT);
     // launch groups of query tiles whose pair codes fit the code budget
     const int64_t tile_bytes = (int64_t)sa.n_seg * kSeg * kQT;
     const int32_t group =
