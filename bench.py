@@ -28,10 +28,10 @@ sys.path.insert(0, ROOT)
 N, D, M, K = 32768, 3, 256, 50
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per screen_kernel<1> launch on this workload: rocprofv3 FETCH_SIZE
-# (x2.000, calibrated on a 1 GiB copy) + WRITE_SIZE (x1.000), recorded in
-# profiles/mixed_knn_c2_n32768_m256_k50.md
-SCREEN_PASS1_HBM_BYTES = 1.341e9
+# HBM bytes per screen_kernel<3, 1> launch on this workload: rocprofv3
+# FETCH_SIZE (x2.000, calibrated on a 1 GiB copy) + WRITE_SIZE (x1.000),
+# recorded in profiles/mixed_knn_c2_n32768_m256_k50.md
+SCREEN_PASS1_HBM_BYTES = 3.085e9
 
 
 def make_inputs():
@@ -207,7 +207,7 @@ def main() -> int:
                      "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4),
                      "traffic": SCREEN_PASS1_HBM_BYTES,
                      "traffic_source": "profiles/mixed_knn_c2_n32768_m256_k50.md",
-                     "kernel": "screen_kernel<1>",
+                     "kernel": "screen_kernel<3, 1>",
                      "algorithmic_flops_per_launch": N * flops_per_q},
         "cpu_baseline": cpu,
         "serve": serve,
