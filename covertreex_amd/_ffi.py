@@ -45,6 +45,9 @@ CX_QT_FRONTIER = 14
 CX_QT_EXPANDED = 78
 CX_QT_BLOCKS = 142
 CX_QT_HARD_LISTED = 145
+# cover-tree invariant counts (CX_INV_*)
+CX_INV_FIELDS = 8
+INV_FIELDS = ("covering", "nesting", "separation", "forest", "nodes", "pairs", "batches")
 
 _i64 = C.c_int64
 _i32 = C.c_int32
@@ -81,6 +84,7 @@ SIGNATURES = {
     "cx_tree_insert_euclidean": (_i32, [_p, _p, _i64, _i32]),
     "cx_tree_delete": (_i32, [_p, _p, _i64]),
     "cx_tree_build_telemetry": (_i32, [_p, _p, _i64, _p]),
+    "cx_tree_check_invariants": (_i32, [_p, _p]),
     "cx_tree_set_query_telemetry": (_i32, [_p, _i32]),
     "cx_tree_last_query_telemetry": (_i32, [_p, _p, _i32, _p]),
     "cx_tree_clear_last_query_telemetry": (_i32, [_p]),

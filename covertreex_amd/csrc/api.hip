@@ -509,6 +509,16 @@ cx_status cx_tree_delete(cx_tree *t, const int64_t *indices, int64_t n_remove)
     });
 }
 
+cx_status cx_tree_check_invariants(cx_tree *t, int64_t *out)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr && out != nullptr, "NULL pointer");
+        std::lock_guard<std::mutex> lock(t->s.mu);
+        CX_HIP(hipSetDevice(t->s.device));
+        check_invariants(t->s, out);
+    });
+}
+
 cx_status cx_tree_build_telemetry(const cx_tree *t, int64_t *out, int64_t cap_batches,
                                   int64_t *n_batches)
 {

@@ -674,6 +674,11 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
     batch_insert_kernel<KIND><<<1, kBiThreads, 0, st>>>(mv, links, t.raw.ptr, t.d, grid_scale,
                                                         use_grid, s_begin, n, chunk, bst.ptr);
     CX_HIP(hipGetLastError());
+    for (int64_t b = 0; b < nb; b++) {
+        const int64_t b0 = s_begin + b * chunk;
+        t.live_batches.push_back(b0);
+        t.live_batches.push_back(n - b0 < chunk ? n - b0 : chunk);
+    }
     const size_t old = t.batch_stats.size();
     t.batch_stats.resize(old + (size_t)nb * kBatchFields);
     CX_HIP(hipMemcpyAsync(t.batch_stats.data() + old, bst.ptr, bst.bytes(), hipMemcpyDeviceToHost,
@@ -1026,7 +1031,9 @@ void delete_rows(TreeState &t, const int64_t *remove, int64_t n_remove, bool by_
     t.n = n1;
     t.mixed_ready = false;
 
-    // re-insert the detached groups, then the derived arrays
+    // re-insert the detached groups, then the derived arrays; the kept nodes
+    // moved, so only the re-inserted batches stay separation groups
+    t.live_batches.clear();
     if (t.metric == 0) {
         const MetricView mv = residual_view(t);
         int64_t g0 = kept;
@@ -1068,6 +1075,162 @@ void subtree_bounds_device(const int64_t *parents_host, const int64_t *n2d_host,
     subtree_bounds_on_stream(par.ptr, n2d.ptr, depth.ptr, md, n, mn.ptr, mx.ptr, st);
     CX_HIP(hipMemcpy(min_host, mn.ptr, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
     CX_HIP(hipMemcpy(max_host, mx.ptr, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+}
+
+
+// ---- invariant checker ---------------------------------------------------------
+// The cover-tree invariants batch insert establishes (the properties of the
+// reference's construction, src/algo/batch/mod.rs:447-800, checked the same
+// way by the oracle's tests):
+// - covering: a node inserted at level L lies within 2^(L+1) of its parent
+//   (d^2 <= 4^(L+1), the build's own FP32 distance);
+// - nesting: the cover sets only grow — a node's parent was in the tree when
+//   the node was inserted (the parent's batch is the node's or an earlier
+//   one) and the parent chain reaches a root;
+// - separation: nodes one batch inserted at the same level L, strictly between
+//   the bottom and top levels, are pairwise farther than 2^L (d^2 > 4^L) and,
+//   when the grid conflicts are on, not grid neighbours;
+// - forest: the child lists and the parent pointers describe the same forest,
+//   and a dropped node (no level) has neither parent nor children.
+// A batch's links, levels and distances are final once it is inserted (later
+// batches only prepend to child lists), so checking every live batch after an
+// insert call is the check after each batch.
+
+// index of the live batch holding position i (batches sorted by start), -1
+// when none does
+__device__ __forceinline__ int64_t batch_of(const int64_t *__restrict__ batches, int64_t nb,
+                                            int64_t i)
+{
+    int64_t lo = 0, hi = nb;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (batches[2 * mid] <= i)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    const int64_t b = lo - 1;
+    return b >= 0 && i < batches[2 * b] + batches[2 * b + 1] ? b : -1;
+}
+
+template <int KIND>
+__global__ void check_links_kernel(MetricView mv, const int64_t *__restrict__ parents,
+                                   const int64_t *__restrict__ children,
+                                   const int64_t *__restrict__ next,
+                                   const int32_t *__restrict__ levels, int64_t n,
+                                   const int64_t *__restrict__ batches, int64_t nb,
+                                   int32_t max_depth, int64_t *cnt)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int64_t p = parents[i];
+    const int32_t L = levels[i];
+    unsigned long long cov = 0, nest = 0, forest = 0, kids = 0;
+    if (p >= 0) {
+        if (L == kLevelUnset) {
+            forest++;
+        } else {
+            const float d2 = build_dist_sq<KIND>(mv, i, p);
+            if (!(d2 <= __builtin_ldexpf(1.0f, 2 * (L + 1))))
+                cov++;
+            const int64_t bi = batch_of(batches, nb, i), bp = batch_of(batches, nb, p);
+            if (bi >= 0 && bp >= 0 && bp > bi)
+                nest++;
+        }
+        // the parent chain reaches a root within the tree's depth
+        int64_t a = p;
+        for (int32_t steps = 0; a >= 0 && steps <= max_depth; steps++)
+            a = parents[a];
+        if (a >= 0)
+            nest++;
+    }
+    // the child list: each entry's parent is i (bounded walk)
+    int64_t c = children[i];
+    for (int64_t steps = 0; c != -1 && steps <= n; steps++) {
+        kids++;
+        if (c < 0 || c >= n || parents[c] != i) {
+            forest++;
+            break;
+        }
+        c = next[c];
+    }
+    if (c != -1)
+        forest++; // a list longer than n entries is a cycle
+    if (L == kLevelUnset && kids > 0)
+        forest++;
+    atomicAdd((unsigned long long *)&cnt[CX_INV_COVERING], cov);
+    atomicAdd((unsigned long long *)&cnt[CX_INV_NESTING], nest);
+    atomicAdd((unsigned long long *)&cnt[CX_INV_FOREST], forest);
+    atomicAdd((unsigned long long *)&cnt[CX_INV_NODES], p >= 0 ? 1ull : 0ull);
+    // list entries minus linked nodes: zero when the two describe one forest
+    atomicAdd((unsigned long long *)&cnt[CX_INV_FIELDS], kids);
+}
+
+// one workgroup per live batch: every pair of its nodes at one level
+template <int KIND>
+__global__ void check_separation_kernel(MetricView mv, const int32_t *__restrict__ levels,
+                                        const float *__restrict__ raw, int32_t gd, float gs,
+                                        int32_t use_grid, const int64_t *__restrict__ batches,
+                                        int32_t lo, int32_t hi, int64_t *cnt)
+{
+    const int64_t start = batches[2 * blockIdx.x], size = batches[2 * blockIdx.x + 1];
+    unsigned long long viol = 0, pairs = 0;
+    for (int64_t a = start + threadIdx.x; a < start + size; a += blockDim.x) {
+        const int32_t La = levels[a];
+        if (La <= lo || La >= hi)
+            continue;
+        const float lim = __builtin_ldexpf(1.0f, 2 * La);
+        for (int64_t b = a + 1; b < start + size; b++) {
+            if (levels[b] != La)
+                continue;
+            pairs++;
+            if (!(build_dist_sq<KIND>(mv, a, b) > lim))
+                viol++;
+            else if (use_grid && grid_adjacent(raw, gd, gs, a, b))
+                viol++;
+        }
+    }
+    atomicAdd((unsigned long long *)&cnt[CX_INV_SEPARATION], viol);
+    atomicAdd((unsigned long long *)&cnt[CX_INV_PAIRS], pairs);
+}
+
+void check_invariants(TreeState &t, int64_t *out)
+{
+    hipStream_t st = t.stream;
+    const int64_t n = t.n;
+    DevBuf<int64_t> cnt(CX_INV_FIELDS + 1);
+    CX_HIP(hipMemsetAsync(cnt.ptr, 0, sizeof(int64_t) * (CX_INV_FIELDS + 1), st));
+    const int64_t nb = (int64_t)t.live_batches.size() / 2;
+    DevBuf<int64_t> bat(nb > 0 ? 2 * nb : 2);
+    if (nb > 0)
+        CX_HIP(hipMemcpyAsync(bat.ptr, t.live_batches.data(), sizeof(int64_t) * 2 * nb,
+                              hipMemcpyHostToDevice, st));
+    auto run = [&](const MetricView &mv, auto kind) {
+        constexpr int K = decltype(kind)::value;
+        if (n > 0)
+            check_links_kernel<K><<<nblocks(n, 256), 256, 0, st>>>(
+                mv, t.parents.ptr, t.children.ptr, t.next.ptr, t.levels.ptr, n, bat.ptr, nb,
+                t.max_depth, cnt.ptr);
+        if (nb > 0)
+            check_separation_kernel<K><<<(unsigned)nb, 256, 0, st>>>(
+                mv, t.levels.ptr, t.raw.ptr, t.d, t.grid_scale, t.metric == 0 ? 1 : 0, bat.ptr,
+                t.min_level, t.max_level, cnt.ptr);
+    };
+    if (t.metric == 0)
+        run(residual_view(t), std::integral_constant<int, kMetricResidual>{});
+    else
+        run(MetricView{t.raw.ptr, nullptr, nullptr, nullptr, t.d, 0, 1.0f, 0},
+            std::integral_constant<int, kMetricEuclidean>{});
+    CX_HIP(hipGetLastError());
+    int64_t h[CX_INV_FIELDS + 1];
+    CX_HIP(hipMemcpyAsync(h, cnt.ptr, sizeof(h), hipMemcpyDeviceToHost, st));
+    CX_HIP(hipStreamSynchronize(st));
+    const int64_t list_minus_linked = h[CX_INV_FIELDS] - h[CX_INV_NODES];
+    h[CX_INV_FOREST] += list_minus_linked < 0 ? -list_minus_linked : list_minus_linked;
+    h[CX_INV_BATCHES] = nb;
+    for (int f = 0; f < CX_INV_FIELDS; f++)
+        out[f] = h[f];
 }
 
 } // namespace cx

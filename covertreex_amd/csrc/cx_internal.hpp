@@ -167,6 +167,9 @@ struct TreeState {
     int32_t chunk = 512;
     float grid_scale = 1.0f;
     std::vector<int64_t> batch_stats;
+    // (start, size) of every batch inserted since the build or the last delete
+    // (tree positions): the invariant checker's separation groups
+    std::vector<int64_t> live_batches;
 
     // reusable query workspace
     DevBuf<uint8_t> work;
@@ -196,6 +199,8 @@ void insert_residual_rows(TreeState &t, const float *V, const float *p, const fl
                           int64_t n_new, int32_t chunk);
 void insert_euclidean_rows(TreeState &t, const float *points, int64_t n_new, int32_t chunk);
 void delete_rows(TreeState &t, const int64_t *remove, int64_t n_remove, bool by_node);
+// Cover-tree invariants of the tree (CX_INV_* counts into out).
+void check_invariants(TreeState &t, int64_t *out);
 void morton_order_device(const float *coords_host, int64_t n, int32_t d, int64_t *order_host);
 void subtree_bounds_device(const int64_t *parents_host, const int64_t *n2d_host, int64_t n,
                            int64_t *min_host, int64_t *max_host);

@@ -30,6 +30,7 @@ class EuclideanHipTree(HipTree):
                                                   int(chunk_size), C.byref(out)))
         t = cls(out.value, x.shape[0])
         t.dim = x.shape[1]
+        t._after_insert()
         return t
 
     def insert(self, points, *, chunk_size: int | None = None) -> None:
@@ -44,6 +45,7 @@ class EuclideanHipTree(HipTree):
         with self._lock:
             check(_ffi.load().cx_tree_insert_euclidean(self._h, ptr(x), x.shape[0], chunk))
             self.n += x.shape[0]
+        self._after_insert()
 
     def knn_points(self, queries, k: int, *, predecessor_mode: bool = False,
                    mode: str = "exact"):

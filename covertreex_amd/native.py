@@ -9,6 +9,7 @@ query via :meth:`HipTree.knn_residual` (``knn_query_residual``) and the
 from __future__ import annotations
 
 import ctypes as C
+import os
 import threading
 
 import numpy as np
@@ -31,6 +32,10 @@ def _f32c(a, ndim):
     return arr
 
 
+class CoverTreeInvariantError(RuntimeError):
+    """A device tree failed its cover-tree invariant check."""
+
+
 class HipTree:
     """A cover tree resident in HBM on the current HIP device."""
 
@@ -38,6 +43,10 @@ class HipTree:
         self._h = C.c_void_p(handle)
         self.n = int(n)
         self._lock = threading.Lock()
+        # check the cover-tree invariants on the device after every insert
+        # call (and after the build, by the builders), raising on a violation
+        self.check_after_insert = os.environ.get("COVERTREEX_HIP_CHECK_INVARIANTS", "0") not in (
+            "", "0", "false", "False")
 
     # -- construction ---------------------------------------------------------
     @classmethod
@@ -63,7 +72,9 @@ class HipTree:
                                          float(rbf_var), ptr(ls), ls.size, int(kernel_type),
                                          int(chunk_size or n or 1), mode,
                                          float(grid_whiten_scale), C.byref(out)))
-        return cls(out.value, n)
+        t = cls(out.value, n)
+        t._after_insert()
+        return t
 
     def close(self) -> None:
         if self._h is not None and self._h.value:
@@ -187,6 +198,7 @@ class HipTree:
             check(_ffi.load().cx_tree_insert_residual(self._h, ptr(v), ptr(p), ptr(x), p.size,
                                                       chunk))
             self.n += p.size
+        self._after_insert()
 
     # -- delete (covertreex/algo/batch_delete.py:628-689) --------------------------
     def delete(self, indices) -> None:
@@ -197,6 +209,32 @@ class HipTree:
         with self._lock:
             check(_ffi.load().cx_tree_delete(self._h, ptr(idx), idx.size))
             self.n -= idx.size
+        self._after_insert()
+
+    # -- cover-tree invariants ----------------------------------------------------
+    def check_invariants(self, *, raise_on_violation: bool = True) -> dict:
+        """Covering (d(node, parent) <= 2^(level+1)), nesting (a node's parent
+        was in the tree when it was inserted, and the parent chain reaches a
+        root), separation (nodes one batch inserted at one level
+        pairwise farther than 2^level and, with grid conflicts, not grid
+        neighbours) and forest consistency, checked on the device: violation
+        counts plus what was checked (nodes, pairs, batches).  These are the
+        properties the reference's batch insert guarantees
+        (src/algo/batch/mod.rs:447-800); separation covers the batches
+        inserted since the build or the last delete.  Raises
+        CoverTreeInvariantError on a violation when raise_on_violation."""
+        out = np.zeros(_ffi.CX_INV_FIELDS, np.int64)
+        with self._lock:
+            check(_ffi.load().cx_tree_check_invariants(self._h, ptr(out)))
+        res = {k: int(out[i]) for i, k in enumerate(_ffi.INV_FIELDS)}
+        bad = {k: res[k] for k in ("covering", "nesting", "separation", "forest") if res[k]}
+        if bad and raise_on_violation:
+            raise CoverTreeInvariantError(f"cover-tree invariants violated: {bad} ({res})")
+        return res
+
+    def _after_insert(self) -> None:
+        if self.check_after_insert:
+            self.check_invariants()
 
     # -- getters (src/lib.rs:825-904) -------------------------------------------
     def info(self):

@@ -250,6 +250,28 @@ cx_status cx_tree_build_telemetry(const cx_tree *t, int64_t *out, int64_t cap_ba
 #define CX_QT_BLOCKS 142        /* [3] tiles of 16, 32, 64 candidates */
 #define CX_QT_HARD_LISTED 145   /* mixed: pairs the screen kept for the hard queries */
 cx_status cx_tree_set_query_telemetry(cx_tree *t, int32_t enable);
+/* Cover-tree invariants (the properties batch insert establishes; the
+ * oracle's tests check the same on the CPU, tests/test_oracle_tree.py):
+ * writes CX_INV_FIELDS int64 counts.  Covering: d(node, parent)^2 <=
+ * 4^(level+1); nesting: a node's parent was in the tree when the node was
+ * inserted (same or earlier batch) and the parent chain reaches a root;
+ * separation: nodes one
+ * batch inserted at one level (strictly inside the level range) pairwise
+ * d^2 > 4^level and, with grid conflicts, not grid neighbours; forest: child
+ * lists and parent pointers agree, dropped nodes have no links.  Separation
+ * covers the batches inserted since the build or the last delete.  No
+ * counterpart entry point in the reference (its construction guarantees
+ * these, src/algo/batch/mod.rs:447-800). */
+#define CX_INV_FIELDS 8
+#define CX_INV_COVERING 0
+#define CX_INV_NESTING 1
+#define CX_INV_SEPARATION 2
+#define CX_INV_FOREST 3
+#define CX_INV_NODES 4      /* linked nodes checked */
+#define CX_INV_PAIRS 5      /* same-level pairs of one batch checked */
+#define CX_INV_BATCHES 6    /* separation groups */
+cx_status cx_tree_check_invariants(cx_tree *t, int64_t *out);
+
 /* *n_out = CX_QT_SIZE when the last call recorded telemetry, else 0. */
 cx_status cx_tree_last_query_telemetry(const cx_tree *t, int64_t *out, int32_t cap,
                                        int32_t *n_out);
