@@ -42,15 +42,103 @@ TreeState::~TreeState()
     codes.release();
     part.release();
     work.release();
+    telem_buf.release();
     if (host_word)
-        (void)hipHostFree(host_word);
+        release_host_word(host_word);
     if (sync_event)
-        (void)hipEventDestroy(sync_event);
+        release_event(sync_event);
     for (int i = 0; i < kMaxMarks; i++)
         if (marks[i])
             (void)hipEventDestroy(marks[i]);
     if (stream)
-        (void)hipStreamDestroy(stream);
+        release_stream(device, stream);
+}
+
+namespace {
+std::mutex g_pool_mu;
+std::vector<std::pair<int, hipStream_t>> g_streams;
+std::vector<int32_t *> g_words;
+std::vector<hipEvent_t> g_events;
+} // namespace
+
+hipStream_t acquire_stream(int device)
+{
+    {
+        std::lock_guard<std::mutex> lock(g_pool_mu);
+        for (size_t i = 0; i < g_streams.size(); i++)
+            if (g_streams[i].first == device) {
+                hipStream_t s = g_streams[i].second;
+                g_streams.erase(g_streams.begin() + (ptrdiff_t)i);
+                return s;
+            }
+    }
+    hipStream_t s = nullptr;
+    CX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+void release_stream(int device, hipStream_t s)
+{
+    (void)hipStreamSynchronize(s);
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    g_streams.emplace_back(device, s);
+}
+
+int32_t *acquire_host_word()
+{
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    if (g_words.empty()) {
+        // one pinned page, cut into 64-byte words (kept for the process)
+        char *page = nullptr;
+        CX_HIP(hipHostMalloc((void **)&page, 4096, hipHostMallocDefault));
+        for (int i = 0; i < 4096 / 64; i++)
+            g_words.push_back((int32_t *)(page + 64 * i));
+    }
+    int32_t *w = g_words.back();
+    g_words.pop_back();
+    return w;
+}
+
+void release_host_word(int32_t *w)
+{
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    g_words.push_back(w);
+}
+
+hipEvent_t acquire_event()
+{
+    {
+        std::lock_guard<std::mutex> lock(g_pool_mu);
+        if (!g_events.empty()) {
+            hipEvent_t e = g_events.back();
+            g_events.pop_back();
+            return e;
+        }
+    }
+    hipEvent_t e = nullptr;
+    CX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return e;
+}
+
+void release_event(hipEvent_t e)
+{
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    g_events.push_back(e);
+}
+
+void ensure_stream_pool(int device)
+{
+    static std::mutex mu;
+    static std::vector<int> done;
+    std::lock_guard<std::mutex> lock(mu);
+    for (int d : done)
+        if (d == device)
+            return;
+    hipMemPool_t pool = nullptr;
+    CX_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+    uint64_t thr = UINT64_MAX;
+    CX_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+    done.push_back(device);
 }
 
 void prof_mark(TreeState &t, int i)
@@ -178,7 +266,9 @@ cx_status cx_tree_build_residual(const float *V, const float *p, const float *co
         try {
             TreeState &t = h->s;
             CX_HIP(hipGetDevice(&t.device));
-            CX_HIP(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
+            t.stream = acquire_stream(t.device);
+            ensure_stream_pool(t.device);
+            StreamScope ss_(t.stream);
             t.n = n;
             t.m = m;
             t.d = d;
@@ -229,6 +319,7 @@ cx_status cx_tree_knn_residual_device(cx_tree *t, const int64_t *d_queries, int6
         CX_REQUIRE(nq >= 0, "nq must be >= 0");
         CX_REQUIRE(nq == 0 || (d_queries && d_out_idx && d_out_dist), "NULL query/output pointer");
         std::lock_guard<std::mutex> lock(t->s.mu);
+        StreamScope ss_(t->s.stream);
         CX_REQUIRE(t->s.metric == 0, "residual kNN on a Euclidean tree");
         CX_HIP(hipSetDevice(t->s.device));
         if (nq == 0)
@@ -260,6 +351,7 @@ cx_status cx_tree_knn_residual(cx_tree *t, const int64_t *queries, int64_t nq, i
         CX_REQUIRE(nq == 0 || (queries && out_idx && out_dist), "NULL query/output pointer");
         if (nq == 0)
             return;
+        StreamScope ss_(t->s.stream);
         DevBuf<int64_t> q(nq), oi((size_t)nq * k);
         DevBuf<float> od((size_t)nq * k), oe(out_err ? (size_t)nq * k : 0);
         hipStream_t st = t->s.stream;
@@ -293,7 +385,9 @@ cx_status cx_tree_build_euclidean(const float *points, int64_t n, int32_t d, int
         try {
             TreeState &t = h->s;
             CX_HIP(hipGetDevice(&t.device));
-            CX_HIP(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
+            t.stream = acquire_stream(t.device);
+            ensure_stream_pool(t.device);
+            StreamScope ss_(t.stream);
             t.n = n;
             t.d = d;
             t.metric = 1;
@@ -320,6 +414,7 @@ cx_status cx_tree_knn_euclidean(cx_tree *t, const float *queries, int64_t nq, in
         if (nq == 0)
             return;
         std::lock_guard<std::mutex> lock(t->s.mu);
+        StreamScope ss_(t->s.stream);
         CX_HIP(hipSetDevice(t->s.device));
         TreeState &s = t->s;
         s.n_marks = 0;
@@ -431,6 +526,7 @@ cx_status cx_tree_set_profiling(cx_tree *t, int32_t enable)
     return guarded([&] {
         CX_REQUIRE(t != nullptr, "tree is NULL");
         std::lock_guard<std::mutex> lock(t->s.mu);
+        StreamScope ss_(t->s.stream);
         t->s.profiling = enable != 0;
         t->s.n_marks = 0;
     });
@@ -441,6 +537,7 @@ cx_status cx_tree_last_profile(cx_tree *t, float *ms_out, int32_t cap, int32_t *
     return guarded([&] {
         CX_REQUIRE(t != nullptr && ms_out != nullptr && n_out != nullptr, "NULL pointer");
         std::lock_guard<std::mutex> lock(t->s.mu);
+        StreamScope ss_(t->s.stream);
         TreeState &s = t->s;
         const int phases = s.n_marks > 1 ? s.n_marks - 1 : 0;
         *n_out = phases;
@@ -460,6 +557,7 @@ cx_status cx_tree_insert_residual(cx_tree *t, const float *V, const float *p,
         CX_REQUIRE(n_new >= 0, "n_new must be >= 0");
         CX_REQUIRE(n_new == 0 || (V && p && coords), "V, p and coords must be non-NULL");
         std::lock_guard<std::mutex> lock(t->s.mu);
+        StreamScope ss_(t->s.stream);
         TreeState &s = t->s;
         CX_REQUIRE(s.metric == 0, "insert_residual on a Euclidean tree");
         CX_REQUIRE(s.n > 0, "insert into an empty tree: build it with the rows instead");
@@ -484,6 +582,7 @@ cx_status cx_tree_insert_euclidean(cx_tree *t, const float *points, int64_t n_ne
         CX_REQUIRE(n_new >= 0, "n_new must be >= 0");
         CX_REQUIRE(n_new == 0 || points, "points must be non-NULL");
         std::lock_guard<std::mutex> lock(t->s.mu);
+        StreamScope ss_(t->s.stream);
         TreeState &s = t->s;
         CX_REQUIRE(s.metric == 1, "insert_euclidean on a residual tree");
         CX_REQUIRE(s.n > 0, "insert into an empty tree: build it with the points instead");
@@ -500,6 +599,7 @@ cx_status cx_tree_delete(cx_tree *t, const int64_t *indices, int64_t n_remove)
         CX_REQUIRE(t != nullptr, "tree is NULL");
         CX_REQUIRE(n_remove >= 0 && (n_remove == 0 || indices), "NULL indices");
         std::lock_guard<std::mutex> lock(t->s.mu);
+        StreamScope ss_(t->s.stream);
         TreeState &s = t->s;
         CX_REQUIRE(n_remove < s.n, "cannot delete every point of a tree");
         CX_HIP(hipSetDevice(s.device));
@@ -514,6 +614,7 @@ cx_status cx_tree_check_invariants(cx_tree *t, int64_t *out)
     return guarded([&] {
         CX_REQUIRE(t != nullptr && out != nullptr, "NULL pointer");
         std::lock_guard<std::mutex> lock(t->s.mu);
+        StreamScope ss_(t->s.stream);
         CX_HIP(hipSetDevice(t->s.device));
         check_invariants(t->s, out);
     });
@@ -547,6 +648,7 @@ cx_status cx_tree_set_query_telemetry(cx_tree *t, int32_t enable)
     return guarded([&] {
         CX_REQUIRE(t != nullptr, "tree is NULL");
         std::lock_guard<std::mutex> lock(t->s.mu);
+        StreamScope ss_(t->s.stream);
         t->s.query_telemetry = enable != 0;
     });
 }
@@ -568,6 +670,7 @@ cx_status cx_tree_clear_last_query_telemetry(cx_tree *t)
     return guarded([&] {
         CX_REQUIRE(t != nullptr, "tree is NULL");
         std::lock_guard<std::mutex> lock(t->s.mu);
+        StreamScope ss_(t->s.stream);
         t->s.last_telem.clear();
     });
 }

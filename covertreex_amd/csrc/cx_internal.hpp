@@ -35,15 +35,45 @@ void set_last_error(const std::string &msg);
             throw ::cx::Error(CX_ERR_INVALID, std::string(msg)); \
     } while (0)
 
+// The stream that orders this thread's DevBuf allocations while a call works
+// on a tree (StreamScope): buffers come from the device's stream-ordered pool
+// (hipMallocAsync / hipFreeAsync on that stream), so freeing one never waits
+// for other trees' work on the device; with no scope, plain hipMalloc/hipFree.
+inline thread_local hipStream_t tls_alloc_stream = nullptr;
+
+struct StreamScope {
+    hipStream_t prev;
+    explicit StreamScope(hipStream_t s) : prev(tls_alloc_stream) { tls_alloc_stream = s; }
+    ~StreamScope() { tls_alloc_stream = prev; }
+    StreamScope(const StreamScope &) = delete;
+    StreamScope &operator=(const StreamScope &) = delete;
+};
+
+// Keep freed pool memory cached in the process (no release threshold), once
+// per device.
+void ensure_stream_pool(int device);
+
+// Process-wide pools of the per-tree runtime objects whose creation or
+// destruction can wait on other work on the device (pinned host memory, and
+// for symmetry streams and events): a tree takes them at first use and gives
+// them back when it is destroyed.
+hipStream_t acquire_stream(int device);
+void release_stream(int device, hipStream_t s);
+int32_t *acquire_host_word();
+void release_host_word(int32_t *w);
+hipEvent_t acquire_event();
+void release_event(hipEvent_t e);
+
 // Owning device buffer.
 template <typename T> struct DevBuf {
     T *ptr = nullptr;
     size_t count = 0;
+    hipStream_t st = nullptr; // the stream ordering its allocation and free
     DevBuf() = default;
     explicit DevBuf(size_t n) { alloc(n); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), count(o.count)
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), count(o.count), st(o.st)
     {
         o.ptr = nullptr;
         o.count = 0;
@@ -54,41 +84,56 @@ template <typename T> struct DevBuf {
             release();
             ptr = o.ptr;
             count = o.count;
+            st = o.st;
             o.ptr = nullptr;
             o.count = 0;
         }
         return *this;
     }
     ~DevBuf() { release(); }
+    static T *raw_alloc(size_t n, hipStream_t s)
+    {
+        T *q = nullptr;
+        if (s)
+            CX_HIP(hipMallocAsync((void **)&q, n * sizeof(T) + 256, s));
+        else
+            CX_HIP(hipMalloc((void **)&q, n * sizeof(T) + 256));
+        return q;
+    }
     void alloc(size_t n)
     {
         release();
         count = n;
+        st = tls_alloc_stream;
         if (n)
-            CX_HIP(hipMalloc((void **)&ptr, n * sizeof(T) + 256));
+            ptr = raw_alloc(n, st);
     }
     void release()
     {
-        if (ptr)
-            (void)hipFree(ptr);
+        if (ptr) {
+            if (st)
+                (void)hipFreeAsync(ptr, st);
+            else
+                (void)hipFree(ptr);
+        }
         ptr = nullptr;
         count = 0;
     }
     size_t bytes() const { return count * sizeof(T); }
-    // reallocate to n elements keeping the first min(count, n) (stream-ordered
-    // copy, then the old buffer is freed after the stream drains)
-    void grow_keep(size_t n, hipStream_t st)
+    // reallocate to n elements keeping the first min(count, n), ordered on
+    // stream s (the old buffer's free follows the copy)
+    void grow_keep(size_t n, hipStream_t s)
     {
-        T *np = nullptr;
-        if (n)
-            CX_HIP(hipMalloc((void **)&np, n * sizeof(T) + 256));
+        T *np = n ? raw_alloc(n, s) : nullptr;
         const size_t keep = count < n ? count : n;
         if (keep)
-            CX_HIP(hipMemcpyAsync(np, ptr, keep * sizeof(T), hipMemcpyDeviceToDevice, st));
-        CX_HIP(hipStreamSynchronize(st));
+            CX_HIP(hipMemcpyAsync(np, ptr, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
+        if (!st)
+            CX_HIP(hipStreamSynchronize(s));
         release();
         ptr = np;
         count = n;
+        st = s;
     }
 };
 

@@ -1437,10 +1437,10 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan_bytes, flag, pos, (int)nq, st));
     hard_scatter_kernel<<<qb, 256, 0, st>>>(flag, pos, nq, hard, n_hard_d);
     if (!t.host_word)
-        CX_HIP(hipHostMalloc((void **)&t.host_word, 64, hipHostMallocDefault));
+        t.host_word = acquire_host_word();
     CX_HIP(hipMemcpyAsync(t.host_word, n_hard_d, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     if (!t.sync_event)
-        CX_HIP(hipEventCreateWithFlags(&t.sync_event, hipEventDisableTiming));
+        t.sync_event = acquire_event();
     hipEvent_t ev = t.sync_event;
     CX_HIP(hipEventRecord(ev, st));
     // ... while the listed queries are rescored

@@ -107,7 +107,22 @@ void nystrom_device(const double *pts_host, int64_t n, int32_t d, const int64_t 
                     int32_t m, double var, const double *ls_host, int32_t n_ls, int32_t kernel,
                     double jitter, float *v_host, float *p_host)
 {
-    hipStream_t st = nullptr;
+    // a stream of its own (concurrent callers overlap on the device) that also
+    // orders the work buffers' allocations
+    struct OwnStream {
+        int dev = 0;
+        hipStream_t s = nullptr;
+        ~OwnStream()
+        {
+            if (s)
+                release_stream(dev, s);
+        }
+    } own;
+    CX_HIP(hipGetDevice(&own.dev));
+    own.s = acquire_stream(own.dev);
+    ensure_stream_pool(own.dev);
+    hipStream_t st = own.s;
+    StreamScope scope(st);
     std::vector<double> inv_ls(d);
     for (int32_t c = 0; c < d; c++) {
         double l = (ls_host && n_ls > 0) ? ls_host[c < n_ls ? c : 0] : 1.0;
@@ -117,10 +132,10 @@ void nystrom_device(const double *pts_host, int64_t n, int32_t d, const int64_t 
     DevBuf<int64_t> ind(m);
     DevBuf<float> v((size_t)n * m), p(n);
     DevBuf<int32_t> status(1);
-    CX_HIP(hipMemcpy(pts.ptr, pts_host, sizeof(double) * n * d, hipMemcpyHostToDevice));
-    CX_HIP(hipMemcpy(il.ptr, inv_ls.data(), sizeof(double) * d, hipMemcpyHostToDevice));
-    CX_HIP(hipMemcpy(ind.ptr, ind_host, sizeof(int64_t) * m, hipMemcpyHostToDevice));
-    CX_HIP(hipMemset(status.ptr, 0, sizeof(int32_t)));
+    CX_HIP(hipMemcpyAsync(pts.ptr, pts_host, sizeof(double) * n * d, hipMemcpyHostToDevice, st));
+    CX_HIP(hipMemcpyAsync(il.ptr, inv_ls.data(), sizeof(double) * d, hipMemcpyHostToDevice, st));
+    CX_HIP(hipMemcpyAsync(ind.ptr, ind_host, sizeof(int64_t) * m, hipMemcpyHostToDevice, st));
+    CX_HIP(hipMemsetAsync(status.ptr, 0, sizeof(int32_t), st));
     const int64_t mm = (int64_t)m * m;
     kmm_kernel<<<(unsigned)((mm + 255) / 256), 256, 0, st>>>(pts.ptr, ind.ptr, m, d, il.ptr, var,
                                                             kernel, jitter * var, kmm.ptr);
@@ -129,11 +144,13 @@ void nystrom_device(const double *pts_host, int64_t n, int32_t d, const int64_t 
         pts.ptr, n, d, ind.ptr, m, il.ptr, var, kernel, kmm.ptr, vt.ptr, v.ptr, p.ptr);
     CX_HIP(hipGetLastError());
     int32_t bad = 0;
-    CX_HIP(hipMemcpy(&bad, status.ptr, sizeof(int32_t), hipMemcpyDeviceToHost));
+    CX_HIP(hipMemcpyAsync(&bad, status.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CX_HIP(hipStreamSynchronize(st));
     if (bad)
         throw Error(CX_ERR_INVALID, "inducing kernel matrix is not positive definite");
-    CX_HIP(hipMemcpy(v_host, v.ptr, sizeof(float) * n * m, hipMemcpyDeviceToHost));
-    CX_HIP(hipMemcpy(p_host, p.ptr, sizeof(float) * n, hipMemcpyDeviceToHost));
+    CX_HIP(hipMemcpyAsync(v_host, v.ptr, sizeof(float) * n * m, hipMemcpyDeviceToHost, st));
+    CX_HIP(hipMemcpyAsync(p_host, p.ptr, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+    CX_HIP(hipStreamSynchronize(st));
 }
 
 } // namespace cx
