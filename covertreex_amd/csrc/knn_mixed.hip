@@ -572,16 +572,10 @@ constexpr int kWaves = 4;                       // wavefronts per workgroup
 constexpr int kRsThreads = 64 * kWaves;
 
 // Workgroup b of a launch runs on XCD b % kXcds (round-robin dispatch); the
-// persistent kernels give each XCD its own work counter and work range.
+// scoring kernels give each XCD its own range of work.
 constexpr int kXcds = 8;
 constexpr int64_t kXcdQueries = 256; // consecutive sorted queries per XCD chunk
 
-// persistent grid: up to 2048 workgroups, a multiple of kXcds
-static inline unsigned xcd_grid(int64_t items)
-{
-    const int64_t b = items < 2048 ? items : 2048;
-    return (unsigned)((b + kXcds - 1) / kXcds * kXcds);
-}
 
 // one scored pair: distance as a double-float (h, l), candidate (dataset
 // index), error bound
@@ -611,7 +605,6 @@ struct RescoreArgs {
     float inv_ls_hi[kMaxD], inv_ls_lo[kMaxD];
     int64_t *out_idx;
     float *out_dist, *out_err;
-    int64_t *counter;
 };
 
 // Error of the compensated dot per |V_q||V_j| (Cauchy-Schwarz bounds
@@ -1022,20 +1015,19 @@ __global__ void __launch_bounds__(kRsThreads, 3) rescore_kernel(RescoreArgs A)
     __shared__ float s_dh[kWaves][64], s_dl[kWaves][64];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int32_t m = A.m, k = A.k;
+    // one query per wavefront: workgroup b runs on XCD x = b % kXcds, and the
+    // XCD's workgroups take its chunks x, x + 8, ... of kXcdQueries
+    // consecutive sorted queries in order, so its L2 holds the candidate rows
+    // of neighbouring queries (a plain grid: other streams' kernels interleave)
     const int x = blockIdx.x % kXcds;
-    for (;;) {
-        int64_t local = 0;
-        if (lane == 0)
-            local = (int64_t)atomicAdd((unsigned long long *)(A.counter + x), 1ull);
-        local = __shfl(local, 0, 64);
-        // this XCD's chunks of kXcdQueries consecutive sorted queries, so its
-        // L2 holds the candidate rows of neighbouring queries
+    {
+        const int64_t local = (int64_t)(blockIdx.x / kXcds) * kWaves + w;
         const int64_t s = (x + kXcds * (local / kXcdQueries)) * kXcdQueries + local % kXcdQueries;
         if (s >= A.nq)
-            break;
+            return;
         const int32_t count = A.cnt[s];
         if (count > kCap)
-            continue; // scored over all its candidates by the hard kernels
+            return; // scored over all its candidates by the hard kernels
         const int32_t qrow = A.qrow[s];
         const int64_t qt = A.inv_order[qrow];
         wave_sync();
@@ -1066,7 +1058,6 @@ struct HardArgs {
     int32_t h0, hn;       // this batch: hard queries [h0, h0 + hn)
     int32_t nseg;         // segments per query (ceil(n / kHSeg))
     HardEntry *part;      // hn x nseg x k segment top-k lists
-    int64_t *counter;
 };
 
 __device__ __forceinline__ int64_t hard_limit(const RescoreArgs &A, int64_t s)
@@ -1084,23 +1075,21 @@ __global__ void __launch_bounds__(kRsThreads, 3) hard_tile_kernel(HardArgs H)
     const RescoreArgs &A = H.r;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int32_t m = A.m, k = A.k;
+    // one item per wavefront: workgroup b runs on XCD x = b % kXcds, and the
+    // XCD takes the segments x, x + 8, ..., each for every query of the batch
+    // before the next, so its L2 holds the segment's rows
     const int x = blockIdx.x % kXcds;
-    for (;;) {
-        int64_t local = 0;
-        if (lane == 0)
-            local = (int64_t)atomicAdd((unsigned long long *)(H.counter + x), 1ull);
-        local = __shfl(local, 0, 64);
-        // this XCD takes the segments x, x + 8, ..., each for every query of
-        // the batch before the next, so its L2 holds the segment's rows
+    {
+        const int64_t local = (int64_t)(blockIdx.x / kXcds) * kWaves + w;
         const int64_t seg_l = x + kXcds * (local / H.hn);
         if (seg_l >= H.nseg)
-            break;
+            return;
         const int32_t hl = (int32_t)(local % H.hn), seg = (int32_t)seg_l;
         const int64_t s = H.hard[H.h0 + hl];
         const int64_t lim = hard_limit(A, s);
         const int64_t beg = (int64_t)seg * kHSeg;
         if (beg >= lim)
-            continue; // past the query's candidates (never read)
+            return; // past the query's candidates (never read)
         const int64_t end = beg + kHSeg < lim ? beg + kHSeg : lim;
         const int32_t qrow = A.qrow[s];
         const int64_t qt = A.inv_order[qrow];
@@ -1311,10 +1300,8 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     int32_t *hist = (int32_t *)(ws + o_hist);
     int32_t *flag = (int32_t *)(ws + o_flag), *pos = (int32_t *)(ws + o_pos);
     int32_t *hard = (int32_t *)(ws + o_hard);
-    // counters: int64 [0, 8) rescore items per XCD, [8, 16) hard items per
-    // XCD; int32 at byte 128 the hard count
-    int64_t *rs_items = (int64_t *)ctr, *hs_items = (int64_t *)ctr + kXcds;
-    int32_t *n_hard_d = ctr + 32;
+    // ctr[0]: the hard-query count
+    int32_t *n_hard_d = ctr;
 
     const unsigned qb = (unsigned)((nq + 255) / 256);
     iota64_kernel<<<qb, 256, 0, st>>>(iota, nq);
@@ -1430,7 +1417,6 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     ra.out_idx = d_idx;
     ra.out_dist = d_dist;
     ra.out_err = d_err;
-    ra.counter = rs_items;
 
     // hard queries: ordered list (read back once) ...
     hard_flag_kernel<<<qb, 256, 0, st>>>(cnt, nq, flag);
@@ -1446,7 +1432,10 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     // ... while the listed queries are rescored
     // top-k held as KB entries per lane: k <= 64 KB
     const int kb = k <= 64 ? 1 : k <= 192 ? 3 : 7;
-    const unsigned rs_grid = xcd_grid((nq + kWaves - 1) / kWaves);
+    // workgroups: each XCD gets ceil(chunks / 8) chunks of kXcdQueries queries
+    const int64_t n_chunks = (nq + kXcdQueries - 1) / kXcdQueries;
+    const unsigned rs_grid =
+        (unsigned)(kXcds * ((n_chunks + kXcds - 1) / kXcds) * (kXcdQueries / kWaves));
     if (kb == 1)
         rescore_kernel<1><<<rs_grid, kRsThreads, 0, st>>>(ra);
     else if (kb == 3)
@@ -1464,7 +1453,6 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
         ha.hard = hard;
         ha.n_hard = n_hard;
         ha.nseg = (int32_t)((n + kHSeg - 1) / kHSeg);
-        ha.counter = hs_items;
         const int64_t query_bytes = (int64_t)ha.nseg * k * sizeof(HardEntry);
         const int32_t batch = (int32_t)std::max<int64_t>(
             1, std::min<int64_t>(n_hard, kPartBudget / query_bytes));
@@ -1476,9 +1464,9 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
         for (int32_t h0 = 0; h0 < n_hard; h0 += batch) {
             ha.h0 = h0;
             ha.hn = std::min(batch, n_hard - h0);
-            CX_HIP(hipMemsetAsync(hs_items, 0, kXcds * sizeof(int64_t), st));
-            const int64_t items = (int64_t)ha.hn * ha.nseg;
-            const unsigned tg = xcd_grid((items + kWaves - 1) / kWaves);
+            // workgroups: each XCD gets ceil(nseg / 8) segments x hn queries
+            const int64_t per_xcd = ((ha.nseg + kXcds - 1) / kXcds) * (int64_t)ha.hn;
+            const unsigned tg = (unsigned)(kXcds * ((per_xcd + kWaves - 1) / kWaves));
             // one wavefront per hard query of the batch
             const unsigned mg =
                 (unsigned)std::min<int64_t>(((int64_t)ha.hn + kWaves - 1) / kWaves, 2048);
