@@ -7,13 +7,11 @@
 //   si_cache      -> compute_si_cache_residual, src/algo.rs:52-106
 //   subtree bounds-> compute_subtree_index_bounds, src/tree.rs:144-185
 //
-// Insertion is sequential over batches and levels, so one persistent
-// workgroup (16 wavefronts) walks the whole build while the tree stays in HBM
-// and each batch's candidate state stays in LDS.  Within a level the work is
-// spread over the workgroup: one candidate per thread for the near/far scan,
-// one far candidate per thread for each greedy-MIS step and for the
-// non-leader assignment.  Distances use the reference f32 arithmetic
-// (device_common.hpp) so every comparison matches the reference bit for bit.
+// Insertion is sequential over batches and levels: each level of each batch
+// is a scan over all CUs, a conflict pass over all CUs and one resolving
+// workgroup, while the tree stays in HBM.  Distances use the reference f32
+// arithmetic (device_common.hpp) so every comparison matches the reference
+// bit for bit.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -22,10 +20,6 @@
 #include "device_common.hpp"
 
 namespace cx {
-
-constexpr int kBiThreads = 1024;
-constexpr int kBiMaxChunk = 1024;
-constexpr int kBiWaves = kBiThreads / 64;
 
 enum { kMetricResidual = 0, kMetricEuclidean = 1 };
 
@@ -95,20 +89,21 @@ __device__ __forceinline__ bool grid_adjacent(const float *__restrict__ raw, int
     return true;
 }
 
-// Ordered block compaction: returns the position of this thread's flagged
-// item among flagged items of lower thread id; *total receives the count.
-__device__ __forceinline__ int block_prefix(bool flag, int *s_cnt, int *total)
+// Ordered compaction over a workgroup of WAVES wavefronts: returns the
+// position of this thread's flagged item among flagged items of lower thread
+// id; *total receives the count.
+template <int WAVES> __device__ __forceinline__ int wg_prefix(bool flag, int *s_cnt, int *total)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint64_t mask = __ballot(flag);
-    int pre = __popcll(mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    const uint64_t mask = __ballot(flag);
+    const int pre = __popcll(mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
     if (lane == 0)
         s_cnt[wid] = __popcll(mask);
     __syncthreads();
     int off = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kBiWaves; w++) {
-        int c = s_cnt[w];
+    for (int w = 0; w < WAVES; w++) {
+        const int c = s_cnt[w];
         off += (w < wid) ? c : 0;
         tot += c;
     }
@@ -117,251 +112,560 @@ __device__ __forceinline__ int block_prefix(bool flag, int *s_cnt, int *total)
     return off + pre;
 }
 
-__device__ __forceinline__ int block_min_int(int v, int *s_red)
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t u = __shfl_xor(v, o, 64);
+        v = u < v ? u : v;
+    }
+    return v;
+}
+
+// ---- batch insert ------------------------------------------------------------
+//
+// batch_insert_collect (src/algo/batch/mod.rs:447-800) is sequential over
+// batches and, inside a batch, over levels 9..-10.  Each level is three
+// launches on the tree's stream:
+//
+//   bi_scan_kernel     (all CUs)  near/far scan: for every candidate, the
+//                                 nearest of its cover and the cover's
+//                                 children within 2^level (580-627); the
+//                                 root's children are kept as an array, so
+//                                 the long root list is scored by many
+//                                 wavefronts at once
+//   bi_pairs_kernel    (all CUs)  d^2 of every pair of far candidates and the
+//                                 conflict bits: grid-adjacent or d^2 <= r^2
+//                                 (370-400, 662-674)
+//   bi_resolve_kernel  (one WG)   greedy MIS in far order over the conflict
+//                                 bits (compute_mis_greedy, mis.rs:9-59),
+//                                 leader links (694-714), non-leader
+//                                 attachment from the stored distances
+//                                 (716-769), the next level's candidates,
+//                                 and at level -10 the bottom survivors
+//                                 (791-800) and the batch's counters
+//
+// Every distance is the reference's f32 arithmetic (device_common.hpp) and
+// every choice keeps the reference's order (strict <, first in list / far /
+// leader order wins, links prepended in the reference's link order), so the
+// tree equals the sequential build's bit for bit.  The batch state between
+// launches lives in a BiState in device memory.
+
+constexpr int kBiMaxChunk = 1024;
+constexpr int kBiWords = kBiMaxChunk / 64;   // conflict words per far row
+constexpr int kBiResThreads = 1024;          // resolve workgroup
+constexpr int kBiResWaves = kBiResThreads / 64;
+constexpr int kBiParThreads = 256;           // scan / pair workgroups
+constexpr int kBiParWaves = kBiParThreads / 64;
+constexpr int kBiParBlocks = 512;
+// upper-triangle conflict words of kBiMaxChunk far rows: row i keeps words
+// (i / 64) .. W-1, i.e. 64 * W * (W + 1) / 2 words for W = 16
+constexpr int kBiAdjWords = 64 * kBiWords * (kBiWords + 1) / 2;
+constexpr uint64_t kBiFar = ~0ull; // scan key of a candidate with no cover within r
+
+struct BiState {
+    int64_t s, nb;  // first row and size of the batch
+    int32_t nc;     // candidates at the current level
+    int32_t nrk;    // entries of the root's child array (oldest link first)
+    int64_t ev_scan, ev_mis, ev_nl, orphans, e_grid, e_dist, leaders, far_levels;
+    uint64_t tk_begin, tk_mark, tk_pairs, tk_scan, tk_mis, tk_nl;
+    int32_t cand[kBiMaxChunk], cover[kBiMaxChunk];
+    // scan result: (d^2 bits << 32) | position in [cover, children...], or kBiFar
+    uint64_t key[kBiMaxChunk];
+    // winner node and children scanned, for candidates whose cover is not the root
+    int32_t node[kBiMaxChunk], len[kBiMaxChunk];
+};
+
+// Offset of far row i's first stored word (word i / 64) in the packed upper
+// triangle of W words per row.
+__device__ __forceinline__ int adj_off(int i, int W)
+{
+    const int b = i >> 6;
+    return 64 * (b * W - (b * (b - 1)) / 2) + (i & 63) * (W - b);
+}
+
+// Node 0's child list into rk, oldest link first (list order is newest
+// first: link_child prepends, src/tree.rs:98-103).  A list holds at most
+// n_nodes entries; the walk stops there.
+__global__ void bi_root_kids_kernel(TreeLinks t, BiState *S, int32_t *__restrict__ rk,
+                                    int64_t n_nodes)
+{
+    const int64_t first = t.children[0];
+    int32_t cnt = 0;
+    for (int64_t c = first; c != -1 && cnt < n_nodes;) {
+        cnt++;
+        const int64_t nx = t.next[c];
+        if (nx == c || nx == first)
+            break;
+        c = nx;
+    }
+    int64_t c = first;
+    for (int32_t i = 0; i < cnt; i++) {
+        rk[cnt - 1 - i] = (int32_t)c;
+        c = t.next[c];
+    }
+    S->nrk = cnt;
+}
+
+// add_point(row, i32::MIN, -1) for the batch's rows (511-513); the root
+// bootstrap at level 10 when the batch starts at row 0 (521-531); every
+// candidate starts at cover 0 (539-540).
+__global__ void __launch_bounds__(kBiResThreads)
+    bi_begin_kernel(TreeLinks t, BiState *S, int64_t s, int64_t nb)
+{
+    const int tid = threadIdx.x;
+    for (int i = tid; i < nb; i += kBiResThreads) {
+        st_rx(&t.parents[s + i], (int64_t)-1);
+        st_rx(&t.children[s + i], (int64_t)-1);
+        st_rx(&t.next[s + i], (int64_t)-1);
+        st_rx(&t.levels[s + i], kLevelUnset);
+    }
+    const int first = s == 0 ? 1 : 0;
+    const int nc = (int)nb - first;
+    for (int i = tid; i < nc; i += kBiResThreads) {
+        S->cand[i] = (int32_t)(s + first + i);
+        S->cover[i] = 0;
+        S->key[i] = kBiFar;
+    }
+    if (tid == 0) {
+        if (s == 0) {
+            st_rx(&t.levels[0], (int32_t)10);
+            S->nrk = 0;
+        }
+        S->s = s;
+        S->nb = nb;
+        S->nc = nc;
+        S->ev_scan = S->ev_mis = S->ev_nl = S->orphans = 0;
+        S->e_grid = S->e_dist = S->leaders = S->far_levels = 0;
+        const uint64_t now = wall_clock64();
+        S->tk_begin = S->tk_mark = S->tk_pairs = now;
+        S->tk_scan = S->tk_mis = S->tk_nl = 0;
+    }
+}
+
+// Near/far scan (580-627).  Work items are (candidate i, group g): when the
+// cover is the root, group g scores positions 64g .. 64g+63 of [root,
+// children...] (one per lane) and folds its best into key[i] with a 64-bit
+// atomic min, which picks the smallest d^2 and, among equal ones, the first
+// position, as the sequential strict-< scan does; otherwise group 0 walks
+// the cover's child list, 64 positions per pass, and stores the result.
+template <int KIND>
+__global__ void __launch_bounds__(kBiParThreads)
+    bi_scan_kernel(MetricView mv, TreeLinks t, BiState *S, const int32_t *__restrict__ rk,
+                   float r2, int64_t n_nodes)
+{
+    const int nc = S->nc;
+    if (nc == 0)
+        return;
+    const int lane = threadIdx.x & 63;
+    const int nrk = S->nrk;
+    const int R = (nrk + 64) / 64;
+    const int64_t items = (int64_t)nc * R;
+    const int64_t nw = (int64_t)gridDim.x * kBiParWaves;
+    for (int64_t it = (int64_t)blockIdx.x * kBiParWaves + (threadIdx.x >> 6); it < items;
+         it += nw) {
+        const int i = (int)(it / R), g = (int)(it % R);
+        const int64_t q = S->cand[i], p = S->cover[i];
+        if (p == 0) {
+            const int pos = g * 64 + lane;
+            uint64_t key = kBiFar;
+            if (pos <= nrk) {
+                const int64_t c = pos == 0 ? 0 : rk[nrk - pos];
+                const float dc = build_dist_sq<KIND>(mv, q, c);
+                if (dc <= r2)
+                    key = ((uint64_t)__float_as_uint(dc) << 32) | (uint32_t)pos;
+            }
+            key = wave_min_u64(key);
+            if (lane == 0 && key != kBiFar)
+                atomicMin((unsigned long long *)&S->key[i], (unsigned long long)key);
+            continue;
+        }
+        if (g != 0)
+            continue;
+        const int64_t first = t.children[p];
+        int64_t c = first;
+        bool more = c != -1;
+        uint64_t best = kBiFar;
+        int64_t bnode = -1;
+        int32_t count = 0;
+        for (int base = 0;; base += 64) {
+            int64_t mine = (base == 0 && lane == 0) ? p : -1;
+            for (int l = base == 0 ? 1 : 0; l < 64 && more; l++) {
+                if (lane == l)
+                    mine = c;
+                count++;
+                const int64_t nx = t.next[c];
+                if (nx == c || nx == first || nx == -1 || count >= n_nodes)
+                    more = false;
+                c = nx;
+            }
+            uint64_t key = kBiFar;
+            if (mine >= 0) {
+                const float dc = build_dist_sq<KIND>(mv, q, mine);
+                if (dc <= r2)
+                    key = ((uint64_t)__float_as_uint(dc) << 32) | (uint32_t)(base + lane);
+            }
+            const uint64_t wmin = wave_min_u64(key);
+            if (wmin < best) {
+                const int src = __ffsll((unsigned long long)__ballot(key == wmin)) - 1;
+                best = wmin;
+                bnode = __shfl(mine, src, 64);
+            }
+            if (!more)
+                break;
+        }
+        if (lane == 0) {
+            S->key[i] = best;
+            S->node[i] = (int32_t)bnode;
+            S->len[i] = count;
+        }
+    }
+}
+
+// The far candidates, in candidate order, into s_far; returns their count.
+__device__ __forceinline__ int far_list_256(const BiState *S, int nc, int32_t *s_far, int *s_cnt)
+{
+    int nf = 0;
+    for (int base = 0; base < nc; base += kBiParThreads) {
+        const int i = base + (int)threadIdx.x;
+        const bool f = i < nc && S->key[i] == kBiFar;
+        int tot;
+        const int pos = wg_prefix<kBiParWaves>(f, s_cnt, &tot);
+        if (f)
+            s_far[nf + pos] = S->cand[i];
+        nf += tot;
+    }
+    __syncthreads();
+    return nf;
+}
+
+// Conflicts among far candidates: for far indices i < j, dist[i][j] = d^2
+// and bit j of row i's conflict word = grid-adjacent || d^2 <= r^2, with the
+// grid bits alone in gadj.  Work items are (i, word w >= i / 64).
+template <int KIND>
+__global__ void __launch_bounds__(kBiParThreads)
+    bi_pairs_kernel(MetricView mv, BiState *S, const float *__restrict__ raw, int32_t gd,
+                    float gs, int32_t use_grid, float r2, float *__restrict__ dist,
+                    uint64_t *__restrict__ adj, uint64_t *__restrict__ gadj)
+{
+    __shared__ int32_t s_far[kBiMaxChunk];
+    __shared__ int s_cnt[kBiParWaves];
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        S->tk_pairs = wall_clock64();
+    const int nc = S->nc;
+    if (nc < 2)
+        return;
+    const int nf = far_list_256(S, nc, s_far, s_cnt);
+    if (nf < 2)
+        return;
+    const int lane = threadIdx.x & 63;
+    const int W = (nf + 63) >> 6;
+    const int items = nf * W;
+    const int nw = (int)gridDim.x * kBiParWaves;
+    for (int it = (int)blockIdx.x * kBiParWaves + (int)(threadIdx.x >> 6); it < items; it += nw) {
+        const int i = it / W, w = it - i * W;
+        if (w < (i >> 6))
+            continue;
+        const int j = w * 64 + lane;
+        bool a = false, g = false;
+        if (j > i && j < nf) {
+            const int64_t x = s_far[i], y = s_far[j];
+            if (use_grid)
+                g = grid_adjacent(raw, gd, gs, x, y);
+            const float d = build_dist_sq<KIND>(mv, x, y);
+            a = g || d <= r2;
+            dist[(size_t)i * kBiMaxChunk + j] = d;
+        }
+        const uint64_t ab = __ballot(a), gb = __ballot(g);
+        if (lane == 0) {
+            adj[i * kBiWords + w] = ab;
+            gadj[i * kBiWords + w] = gb;
+        }
+    }
+}
+
+// *acc (LDS) += the sum of v over the workgroup.
+__device__ __forceinline__ void wg_add_1024(int64_t v, int64_t *s_red, int64_t *acc)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        int u = __shfl_xor(v, o, 64);
-        v = u < v ? u : v;
-    }
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
     if (lane == 0)
         s_red[wid] = v;
     __syncthreads();
-    int r = s_red[0];
-#pragma unroll
-    for (int w = 1; w < kBiWaves; w++)
-        r = s_red[w] < r ? s_red[w] : r;
+    if (threadIdx.x == 0) {
+        int64_t r = 0;
+        for (int w = 0; w < kBiResWaves; w++)
+            r += s_red[w];
+        *acc += r;
+    }
     __syncthreads();
-    return r;
 }
 
-// Inserts rows [s_begin, n) in batches of `chunk`.  Per batch b (counted from
-// s_begin) it records bstats[b * kBatchFields + f] (see cx_internal.hpp):
-// start row, size, leaders, bottom-level survivors, dropped orphans, levels
-// with a far set, distance evaluations of the near/far scan, the MIS and the
-// non-leader step, conflict edges found through the grid and through
-// distance, and wall-clock ticks of the three steps and of the whole batch.
-template <int KIND>
-__global__ void __launch_bounds__(kBiThreads)
-    batch_insert_kernel(MetricView mv, TreeLinks t, const float *__restrict__ raw, int32_t gd,
-                        float gs, int32_t use_grid, int64_t s_begin, int64_t n, int64_t chunk,
-                        int64_t *bstats)
+struct LinkScratch {
+    int32_t prev[kBiMaxChunk];
+    uint8_t last[kBiMaxChunk];
+    int cnt[kBiResWaves];
+    int nrk;
+};
+
+// set_level + set_parent for nodes[0..cnt) in this order (src/tree.rs:
+// 105-120): each link prepends to the parent's list, so a parent's run of
+// nodes ends up newest first ahead of its old head.  In parallel: each node
+// finds the previous node of its run (a wavefront compares 64 earlier
+// parents per step), old heads are read before any write, and node 0's new
+// children are appended to rk in order.
+__device__ void link_in_order(TreeLinks t, int32_t *__restrict__ rk, const int32_t *nodes,
+                              const int32_t *pars, int cnt, int32_t level, LinkScratch &L)
 {
-    __shared__ int32_t s_ca[kBiMaxChunk], s_cb[kBiMaxChunk];
-    __shared__ int32_t s_va[kBiMaxChunk], s_vb[kBiMaxChunk];
-    __shared__ int32_t s_best[kBiMaxChunk], s_far[kBiMaxChunk], s_farorig[kBiMaxChunk];
-    __shared__ int32_t s_leader[kBiMaxChunk], s_nlbest[kBiMaxChunk];
-    __shared__ int8_t s_state[kBiMaxChunk];
-    __shared__ int s_cnt[kBiWaves], s_red[kBiWaves];
-    __shared__ int s_nl;
-
-    const int tid = threadIdx.x;
-
-    for (int64_t s = s_begin; s < n; s += chunk) {
-        const int64_t e = s + chunk < n ? s + chunk : n;
-        const int nb = (int)(e - s);
-        int64_t ev_scan = 0, ev_mis = 0, ev_nl = 0, n_orphans = 0, e_grid = 0, e_dist = 0;
-        int64_t n_leaders = 0, n_far_levels = 0;
-        uint64_t tk_scan = 0, tk_mis = 0, tk_nl = 0;
-        const uint64_t tk_begin = wall_clock64();
-        // add_point(row, i32::MIN, -1), src/algo/batch/mod.rs:511-513
-        for (int i = tid; i < nb; i += kBiThreads) {
-            st_rx(&t.parents[s + i], (int64_t)-1);
-            st_rx(&t.children[s + i], (int64_t)-1);
-            st_rx(&t.next[s + i], (int64_t)-1);
-            st_rx(&t.levels[s + i], kLevelUnset);
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid < cnt)
+        L.last[tid] = 1;
+    __syncthreads();
+    for (int k = wid; k < cnt; k += kBiResWaves) {
+        const int32_t P = pars[k];
+        int prev = -1;
+        for (int base = k - 1; base >= 0; base -= 64) {
+            const int idx = base - lane;
+            const uint64_t b = __ballot(idx >= 0 && pars[idx] == P);
+            if (b) {
+                prev = base - (__ffsll((unsigned long long)b) - 1);
+                break;
+            }
         }
-        int32_t *cur_c = s_ca, *cur_v = s_va, *nxt_c = s_cb, *nxt_v = s_vb;
-        int nc;
-        if (s == 0) { // root bootstrap at level 10, levels -10..10 (521-531)
-            if (tid == 0)
-                st_rx(&t.levels[0], (int32_t)10);
-            nc = nb - 1;
-            for (int i = tid; i < nc; i += kBiThreads) {
-                cur_c[i] = (int32_t)(s + 1 + i);
-                cur_v[i] = 0;
+        if (lane == 0) {
+            L.prev[k] = prev;
+            if (prev >= 0)
+                L.last[prev] = 0;
+        }
+    }
+    __syncthreads();
+    int64_t old = -1;
+    if (tid < cnt && L.prev[tid] < 0)
+        old = ld_rx(&t.children[pars[tid]]);
+    __syncthreads();
+    if (tid < cnt) {
+        const int32_t v = nodes[tid], P = pars[tid];
+        st_rx(&t.parents[v], (int64_t)P);
+        st_rx(&t.levels[v], level);
+        st_rx(&t.next[v], L.prev[tid] >= 0 ? (int64_t)nodes[L.prev[tid]] : old);
+        if (L.last[tid])
+            st_rx(&t.children[P], (int64_t)v);
+    }
+    const bool r0 = tid < cnt && pars[tid] == 0;
+    int nr;
+    const int pr = wg_prefix<kBiResWaves>(r0, L.cnt, &nr);
+    const int base = L.nrk;
+    if (r0)
+        rk[base + pr] = nodes[tid];
+    __syncthreads();
+    if (tid == 0)
+        L.nrk = base + nr;
+    __syncthreads();
+}
+
+// The rest of a level after the scan and the conflict pass: near/far split,
+// greedy MIS, leaders, non-leaders, the next level's candidates; at level
+// -10 also the bottom survivors and the batch's counters (bs: its row of
+// kBatchFields, or null).
+template <int KIND>
+__global__ void __launch_bounds__(kBiResThreads)
+    bi_resolve_kernel(TreeLinks t, BiState *S, int32_t *__restrict__ rk,
+                      const float *__restrict__ dist, const uint64_t *__restrict__ adj,
+                      const uint64_t *__restrict__ gadj, int32_t level, float r2,
+                      int64_t *__restrict__ bs)
+{
+    __shared__ int32_t s_nxc[kBiMaxChunk], s_nxv[kBiMaxChunk]; // next level
+    __shared__ int32_t s_far[kBiMaxChunk], s_fcov[kBiMaxChunk];
+    __shared__ int32_t s_lead[kBiMaxChunk], s_lfar[kBiMaxChunk], s_lpar[kBiMaxChunk];
+    __shared__ int32_t s_nlf[kBiMaxChunk];
+    __shared__ unsigned long long s_nlkey[kBiMaxChunk];
+    __shared__ uint64_t s_adj[kBiAdjWords];
+    __shared__ uint64_t s_sel[kBiWords];
+    __shared__ int64_t s_red[kBiResWaves];
+    __shared__ int64_t s_acc[3]; // scan evaluations, grid edges, distance-only edges
+    __shared__ LinkScratch L;
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint64_t tk0 = wall_clock64();
+    const int nc = S->nc;
+    const int nrk0 = S->nrk;
+    if (tid == 0) {
+        L.nrk = nrk0;
+        s_acc[0] = s_acc[1] = s_acc[2] = 0;
+    }
+    int64_t ev = 0;
+    bool near = false;
+    int32_t cand = 0, cover = 0, node = -1;
+    if (tid < nc) {
+        cand = S->cand[tid];
+        cover = S->cover[tid];
+        const uint64_t key = S->key[tid];
+        near = key != kBiFar;
+        if (cover == 0) {
+            ev = 1 + nrk0;
+            if (near) {
+                const int pos = (int)(uint32_t)key;
+                node = pos == 0 ? 0 : rk[nrk0 - pos];
             }
         } else {
-            nc = nb;
-            for (int i = tid; i < nc; i += kBiThreads) {
-                cur_c[i] = (int32_t)(s + i);
-                cur_v[i] = 0; // active sets start as [root_idx = 0] (539-540)
+            ev = 1 + S->len[tid];
+            node = S->node[tid];
+        }
+    }
+    wg_add_1024(ev, s_red, &s_acc[0]);
+    int nn;
+    const int pn = wg_prefix<kBiResWaves>(near, L.cnt, &nn);
+    if (near) {
+        s_nxc[pn] = cand;
+        s_nxv[pn] = node;
+    } else if (tid < nc) {
+        s_far[tid - pn] = cand;
+        s_fcov[tid - pn] = cover;
+    }
+    const int nf = nc - nn;
+    __syncthreads();
+    int64_t leaders = 0, orphans = 0, ev_mis = 0, ev_nl = 0;
+    uint64_t tk_mis = 0, tk_nl = 0;
+    if (nf > 0) {
+        const int W = (nf + 63) >> 6;
+        int64_t eg = 0, ed = 0;
+        for (int idx = tid; idx < nf * W; idx += kBiResThreads) {
+            const int i = idx / W, w = idx - i * W;
+            if (w < (i >> 6))
+                continue;
+            uint64_t a = 0, g = 0;
+            if (nf > 1) {
+                a = adj[i * kBiWords + w];
+                g = gadj[i * kBiWords + w];
             }
+            s_adj[adj_off(i, W) + w - (i >> 6)] = a;
+            eg += __popcll(g);
+            ed += __popcll(a & ~g);
+        }
+        wg_add_1024(eg, s_red, &s_acc[1]);
+        wg_add_1024(ed, s_red, &s_acc[2]);
+        ev_mis = (int64_t)nf * (nf - 1) / 2;
+        // greedy MIS in far order: take the first undecided candidate, drop
+        // every later candidate in conflict with it
+        if (wid == 0) {
+            uint64_t U = 0, sel = 0;
+            if (lane < W)
+                U = (lane < W - 1 || (nf & 63) == 0) ? ~0ull : ((1ull << (nf & 63)) - 1);
+            for (;;) {
+                const uint64_t nz = __ballot(U != 0);
+                if (nz == 0)
+                    break;
+                const int f = __ffsll((unsigned long long)nz) - 1;
+                const uint64_t wf = __shfl(U, f, 64);
+                const int bit = __ffsll((unsigned long long)wf) - 1;
+                const int istar = f * 64 + bit;
+                if (lane == f) {
+                    U &= ~(1ull << bit);
+                    sel |= 1ull << bit;
+                }
+                if (lane >= f && lane < W)
+                    U &= ~s_adj[adj_off(istar, W) + (lane - f)];
+            }
+            if (lane < W)
+                s_sel[lane] = sel;
         }
         __syncthreads();
-        for (int32_t level = 9; level >= -10 && nc > 0; --level) {
-            const float r = ldexpf(1.0f, level);
-            const float r2 = r * r;
-            const uint64_t tk0 = wall_clock64();
-            // near/far over each candidate's cover and the cover's children (580-627)
-            if (tid < nc) {
-                const int64_t q = cur_c[tid], p = cur_v[tid];
-                float bd = 3.40282347e38f;
-                int64_t bn = -1;
-                float dp = build_dist_sq<KIND>(mv, q, p);
-                ev_scan++;
-                if (dp <= r2 && dp < bd) {
-                    bd = dp;
-                    bn = p;
-                }
-                const int64_t first = ld_rx(&t.children[p]);
-                int64_t c = first;
-                while (c != -1) {
-                    float dc = build_dist_sq<KIND>(mv, q, c);
-                    ev_scan++;
-                    if (dc <= r2 && dc < bd) {
-                        bd = dc;
-                        bn = c;
-                    }
-                    int64_t nx = ld_rx(&t.next[c]);
-                    if (nx == c)
-                        break;
-                    c = nx;
-                    if (c == first)
-                        break;
-                }
-                s_best[tid] = (int32_t)bn;
-            }
-            __syncthreads();
-            const uint64_t tk1 = wall_clock64();
-            tk_scan += tk1 - tk0;
-            bool active = tid < nc;
-            bool near = active && s_best[tid] >= 0;
-            int nn = 0;
-            int pos = block_prefix(near, s_cnt, &nn);
-            if (near) {
-                nxt_c[pos] = cur_c[tid];
-                nxt_v[pos] = s_best[tid];
-            } else if (active) {
-                int fpos = tid - pos;
-                s_far[fpos] = cur_c[tid];
-                s_farorig[fpos] = tid;
-            }
-            const int nf = nc - nn;
-            __syncthreads();
-            if (nf > 0) {
-                n_far_levels++;
-                // greedy MIS in far-list order (compute_mis_greedy, mis.rs:9-59)
-                // over pairwise d_sq <= r^2 conflicts (370-400) and grid edges
-                // (662-674): only rows of selected candidates are ever needed.
-                if (tid < nf)
-                    s_state[tid] = 0;
-                __syncthreads();
-                int start = 0;
-                for (;;) {
-                    int cand = (tid < nf && tid >= start && s_state[tid] == 0) ? tid : INT32_MAX;
-                    int istar = block_min_int(cand, s_red);
-                    if (istar == INT32_MAX)
-                        break;
-                    if (tid == istar)
-                        s_state[tid] = 1;
-                    if (tid > istar && tid < nf && s_state[tid] == 0) {
-                        const int64_t a = s_far[istar], b = s_far[tid];
-                        bool adj = false;
-                        if (use_grid)
-                            adj = grid_adjacent(raw, gd, gs, a, b);
-                        if (adj) {
-                            e_grid++;
-                        } else {
-                            adj = build_dist_sq<KIND>(mv, a, b) <= r2;
-                            ev_mis++;
-                            e_dist += adj ? 1 : 0;
-                        }
-                        if (adj)
-                            s_state[tid] = -1;
-                    }
-                    __syncthreads();
-                    start = istar + 1;
-                }
-                // leaders: level and parent = previous cover, in far order (694-714)
-                if (tid == 0) {
-                    int nl = 0;
-                    for (int i = 0; i < nf; i++) {
-                        if (s_state[i] != 1)
-                            continue;
-                        const int64_t qn = s_far[i];
-                        s_leader[nl++] = (int32_t)qn;
-                        st_rx(&t.levels[qn], level);
-                        const int64_t par = cur_v[s_farorig[i]];
-                        st_rx(&t.parents[qn], par);
-                        link_child(t, par, qn);
-                    }
-                    s_nl = nl;
-                    n_leaders += nl;
-                }
-                __syncthreads();
-                const uint64_t tk2 = wall_clock64();
-                tk_mis += tk2 - tk1;
-                const int nl = s_nl;
-                // non-leaders: nearest leader within r, strict <, first wins (716-769)
-                if (tid < nf) {
-                    int32_t bl = -1;
-                    if (s_state[tid] != 1) {
-                        float bd = 3.40282347e38f;
-                        const int64_t qn = s_far[tid];
-                        for (int l = 0; l < nl; l++) {
-                            float dl = build_dist_sq<KIND>(mv, qn, s_leader[l]);
-                            if (dl <= r2 && dl < bd) {
-                                bd = dl;
-                                bl = s_leader[l];
-                            }
-                        }
-                        ev_nl += nl;
-                        if (bl < 0)
-                            n_orphans++;
-                    }
-                    s_nlbest[tid] = bl;
-                }
-                __syncthreads();
-                bool keep = tid < nf && s_state[tid] != 1 && s_nlbest[tid] >= 0;
-                int nk = 0;
-                int kp = block_prefix(keep, s_cnt, &nk);
-                if (keep) {
-                    nxt_c[nn + kp] = s_far[tid];
-                    nxt_v[nn + kp] = s_nlbest[tid];
-                }
-                nn += nk;
-                __syncthreads();
-                tk_nl += wall_clock64() - tk2;
-            }
-            int32_t *tc = cur_c, *tv = cur_v;
-            cur_c = nxt_c;
-            cur_v = nxt_v;
-            nxt_c = tc;
-            nxt_v = tv;
-            nc = nn;
-            __syncthreads();
+        const bool isl = tid < nf && ((s_sel[tid >> 6] >> (tid & 63)) & 1);
+        int nl;
+        const int pl = wg_prefix<kBiResWaves>(isl, L.cnt, &nl);
+        if (isl) {
+            s_lead[pl] = s_far[tid];
+            s_lfar[pl] = tid;
+            s_lpar[pl] = s_fcov[tid];
         }
-        // survivors at the bottom level (791-800)
-        if (tid == 0) {
-            for (int i = 0; i < nc; i++) {
-                const int64_t qn = cur_c[i];
-                st_rx(&t.levels[qn], (int32_t)-10);
-                st_rx(&t.parents[qn], (int64_t)cur_v[i]);
-                link_child(t, cur_v[i], qn);
+        const bool isn = tid < nf && !isl;
+        int nnl;
+        const int pu = wg_prefix<kBiResWaves>(isn, L.cnt, &nnl);
+        if (isn)
+            s_nlf[pu] = tid;
+        if (tid < nnl)
+            s_nlkey[tid] = kBiFar;
+        __syncthreads();
+        link_in_order(t, rk, s_lead, s_lpar, nl, level, L);
+        leaders = nl;
+        const uint64_t tk1 = wall_clock64();
+        tk_mis = tk1 - tk0;
+        // non-leaders: nearest leader within r, strict <, first leader wins;
+        // items are (non-leader, run of 32 leaders)
+        const int nch = (nl + 31) >> 5;
+        for (int it = tid; it < nnl * nch; it += kBiResThreads) {
+            const int u = it / nch, k0 = (it - u * nch) * 32;
+            const int k1 = k0 + 32 < nl ? k0 + 32 : nl;
+            const int i = s_nlf[u];
+            uint64_t best = kBiFar;
+#pragma unroll 4
+            for (int k = k0; k < k1; k++) {
+                const int lf = s_lfar[k];
+                const float d = i < lf ? dist[(size_t)i * kBiMaxChunk + lf]
+                                       : dist[(size_t)lf * kBiMaxChunk + i];
+                if (d <= r2) {
+                    const uint64_t key = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)k;
+                    best = key < best ? key : best;
+                }
             }
+            if (best != kBiFar)
+                atomicMin(&s_nlkey[u], (unsigned long long)best);
         }
         __syncthreads();
-        if (bstats) {
-            int64_t *bs = bstats + ((s - s_begin) / chunk) * kBatchFields;
-            atomicAdd((unsigned long long *)&bs[6], (unsigned long long)ev_scan);
-            atomicAdd((unsigned long long *)&bs[7], (unsigned long long)ev_mis);
-            atomicAdd((unsigned long long *)&bs[8], (unsigned long long)ev_nl);
-            atomicAdd((unsigned long long *)&bs[4], (unsigned long long)n_orphans);
-            atomicAdd((unsigned long long *)&bs[9], (unsigned long long)e_grid);
-            atomicAdd((unsigned long long *)&bs[10], (unsigned long long)e_dist);
-            if (tid == 0) {
-                bs[0] = s;
-                bs[1] = nb;
-                bs[2] = n_leaders;
-                bs[3] = nc;
-                bs[5] = n_far_levels;
-                bs[11] = (int64_t)tk_scan;
-                bs[12] = (int64_t)tk_mis;
-                bs[13] = (int64_t)tk_nl;
-                bs[14] = (int64_t)(wall_clock64() - tk_begin);
-            }
+        const bool keep = tid < nnl && s_nlkey[tid] != kBiFar;
+        int nk;
+        const int pk = wg_prefix<kBiResWaves>(keep, L.cnt, &nk);
+        if (keep) {
+            s_nxc[nn + pk] = s_far[s_nlf[tid]];
+            s_nxv[nn + pk] = s_lead[(int)(uint32_t)s_nlkey[tid]];
+        }
+        orphans = nnl - nk;
+        ev_nl = (int64_t)nnl * nl;
+        nn += nk;
+        __syncthreads();
+        tk_nl = wall_clock64() - tk1;
+    }
+    for (int i = tid; i < nn; i += kBiResThreads) {
+        S->cand[i] = s_nxc[i];
+        S->cover[i] = s_nxv[i];
+        S->key[i] = kBiFar;
+    }
+    if (level == -10)
+        link_in_order(t, rk, s_nxc, s_nxv, nn, level, L);
+    if (tid == 0) {
+        S->nc = nn;
+        S->nrk = L.nrk;
+        S->ev_scan += s_acc[0];
+        S->ev_mis += ev_mis;
+        S->ev_nl += ev_nl;
+        S->orphans += orphans;
+        S->e_grid += s_acc[1];
+        S->e_dist += s_acc[2];
+        S->leaders += leaders;
+        S->far_levels += nf > 0 ? 1 : 0;
+        S->tk_scan += S->tk_pairs > S->tk_mark ? S->tk_pairs - S->tk_mark : 0;
+        S->tk_mis += (tk0 > S->tk_pairs ? tk0 - S->tk_pairs : 0) + tk_mis;
+        S->tk_nl += tk_nl;
+        const uint64_t now = wall_clock64();
+        S->tk_mark = now;
+        if (level == -10 && bs) {
+            bs[0] = S->s;
+            bs[1] = S->nb;
+            bs[2] = S->leaders;
+            bs[3] = nn;
+            bs[4] = S->orphans;
+            bs[5] = S->far_levels;
+            bs[6] = S->ev_scan;
+            bs[7] = S->ev_mis;
+            bs[8] = S->ev_nl;
+            bs[9] = S->e_grid;
+            bs[10] = S->e_dist;
+            bs[11] = (int64_t)S->tk_scan;
+            bs[12] = (int64_t)S->tk_mis;
+            bs[13] = (int64_t)S->tk_nl;
+            bs[14] = (int64_t)(now - S->tk_begin);
         }
     }
 }
@@ -670,14 +974,32 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
     const int64_t nb = (n - s_begin + chunk - 1) / chunk;
     DevBuf<int64_t> bst((size_t)nb * kBatchFields);
     CX_HIP(hipMemsetAsync(bst.ptr, 0, bst.bytes(), st));
+    CX_REQUIRE(chunk > 0 && chunk <= kBiMaxChunk, "chunk must be in [1, 1024]");
     TreeLinks links{t.parents.ptr, t.children.ptr, t.next.ptr, t.levels.ptr};
-    batch_insert_kernel<KIND><<<1, kBiThreads, 0, st>>>(mv, links, t.raw.ptr, t.d, grid_scale,
-                                                        use_grid, s_begin, n, chunk, bst.ptr);
-    CX_HIP(hipGetLastError());
+    DevBuf<BiState> state(1);
+    DevBuf<int32_t> rk(t.n);
+    DevBuf<float> dist((size_t)kBiMaxChunk * kBiMaxChunk);
+    DevBuf<uint64_t> adj((size_t)kBiMaxChunk * kBiWords), gadj((size_t)kBiMaxChunk * kBiWords);
+    if (s_begin > 0)
+        bi_root_kids_kernel<<<1, 1, 0, st>>>(links, state.ptr, rk.ptr, t.n);
     for (int64_t b = 0; b < nb; b++) {
         const int64_t b0 = s_begin + b * chunk;
+        const int64_t bn = n - b0 < chunk ? n - b0 : chunk;
+        bi_begin_kernel<<<1, kBiResThreads, 0, st>>>(links, state.ptr, b0, bn);
+        for (int32_t level = 9; level >= -10; level--) {
+            const float r = ldexpf(1.0f, level), r2 = r * r;
+            bi_scan_kernel<KIND>
+                <<<kBiParBlocks, kBiParThreads, 0, st>>>(mv, links, state.ptr, rk.ptr, r2, t.n);
+            bi_pairs_kernel<KIND><<<kBiParBlocks, kBiParThreads, 0, st>>>(
+                mv, state.ptr, t.raw.ptr, t.d, grid_scale, use_grid, r2, dist.ptr, adj.ptr,
+                gadj.ptr);
+            bi_resolve_kernel<KIND><<<1, kBiResThreads, 0, st>>>(
+                links, state.ptr, rk.ptr, dist.ptr, adj.ptr, gadj.ptr, level, r2,
+                bst.ptr + b * kBatchFields);
+        }
+        CX_HIP(hipGetLastError());
         t.live_batches.push_back(b0);
-        t.live_batches.push_back(n - b0 < chunk ? n - b0 : chunk);
+        t.live_batches.push_back(bn);
     }
     const size_t old = t.batch_stats.size();
     t.batch_stats.resize(old + (size_t)nb * kBatchFields);

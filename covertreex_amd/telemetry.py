@@ -12,11 +12,12 @@ here writes records with the same identity fields (``schema_id``,
 quantity; :func:`record_build_batches` emits one record per device batch from
 ``HipTree.build_telemetry()``.
 
-Device-build specifics, recorded under ``hip_*`` keys: the conflict graph is
-never materialised (the lazy greedy MIS evaluates only the rows of selected
-candidates), so conflict-graph and MIS time are one figure
-(``conflict_graph_ms``; ``mis_ms`` is 0 and ``hip_conflict_mis_fused`` is 1),
-and ``conflict_adj_pairs`` counts the conflicts the MIS found.
+Device-build specifics, recorded under ``hip_*`` keys: the conflict pass
+(every far pair's distance and conflict bit) and the greedy MIS run back to
+back on the device and are timed as one figure (``conflict_graph_ms``;
+``mis_ms`` is 0 and ``hip_conflict_mis_fused`` is 1); ``conflict_adj_pairs``
+counts the conflict edges (grid and distance) and ``hip_mis_evals`` the
+far pairs scored, n_far * (n_far - 1) / 2 per level as the reference counts.
 """
 
 from __future__ import annotations

@@ -36,6 +36,7 @@ def _same_batches(ht, ot):
     for h, o in zip(hb, ot.batch_stats):
         assert (h["start"], h["size"]) == (o["start"], o["size"])
         assert h["scan_evals"] == o["filter_evals"]
+        assert h["mis_evals"] == o["conflict_pairs"]
         assert h["nonleader_evals"] == o["nonleader_evals"]
         assert h["leaders"] == o["leaders"]
         assert h["orphans"] == o["orphans_dropped"]
