@@ -174,6 +174,8 @@ struct BiState {
     uint64_t key[kBiMaxChunk];
     // winner node and children scanned, for candidates whose cover is not the root
     int32_t node[kBiMaxChunk], len[kBiMaxChunk];
+    // distance evaluations of levels 9..0 per candidate when one launch ran them
+    int64_t ev_pre[kBiMaxChunk];
 };
 
 // Offset of far row i's first stored word (word i / 64) in the packed upper
@@ -226,6 +228,7 @@ __global__ void __launch_bounds__(kBiResThreads)
         S->cand[i] = (int32_t)(s + first + i);
         S->cover[i] = 0;
         S->key[i] = kBiFar;
+        S->ev_pre[i] = 0;
     }
     if (tid == 0) {
         if (s == 0) {
@@ -319,6 +322,104 @@ __global__ void __launch_bounds__(kBiParThreads)
             S->node[i] = (int32_t)bnode;
             S->len[i] = count;
         }
+    }
+}
+
+// Levels 9..0 under the residual metric.  d^2 = 1 - |clamp(rho)| lies in
+// [0, 1] <= 4^level for level >= 0, so at these levels every candidate is
+// near, no leader is linked and each candidate's descent depends only on the
+// tree as the batch found it: one workgroup per candidate runs the ten scans
+// back to back (the same positions, keys and first-minimum choice as
+// bi_scan_kernel), then leaves the candidate at its level-0 cover with its
+// distance evaluations in ev_pre.
+__global__ void __launch_bounds__(kBiParThreads)
+    bi_descend_kernel(MetricView mv, TreeLinks t, BiState *S, const int32_t *__restrict__ rk,
+                      int64_t n_nodes)
+{
+    __shared__ unsigned long long s_wmin[kBiParWaves];
+    __shared__ int64_t s_wnode[kBiParWaves];
+    __shared__ int64_t s_cover;
+    const int i = (int)blockIdx.x;
+    if (i >= S->nc)
+        return;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int64_t q = S->cand[i];
+    const int nrk = S->nrk;
+    int64_t p = 0, ev = 0;
+    for (int32_t level = 9; level >= 0; --level) {
+        const float r = ldexpf(1.0f, level), r2 = r * r;
+        if (p == 0) {
+            uint64_t best = kBiFar;
+            for (int pos = tid; pos <= nrk; pos += kBiParThreads) {
+                const int64_t c = pos == 0 ? 0 : rk[nrk - pos];
+                const float dc = dist_sq_ref(mv, q, c);
+                if (dc <= r2) {
+                    const uint64_t key = ((uint64_t)__float_as_uint(dc) << 32) | (uint32_t)pos;
+                    best = key < best ? key : best;
+                }
+            }
+            best = wave_min_u64(best);
+            if (lane == 0)
+                s_wmin[wid] = best;
+            __syncthreads();
+            if (tid == 0) {
+                uint64_t b = s_wmin[0];
+                for (int w = 1; w < kBiParWaves; w++)
+                    b = s_wmin[w] < b ? s_wmin[w] : b;
+                const int pos = (int)(uint32_t)b;
+                s_cover = pos == 0 ? 0 : rk[nrk - pos];
+            }
+            ev += 1 + nrk;
+        } else {
+            if (wid == 0) {
+                const int64_t first = t.children[p];
+                int64_t c = first;
+                bool more = c != -1;
+                uint64_t best = kBiFar;
+                int64_t bnode = -1;
+                int32_t count = 0;
+                for (int base = 0;; base += 64) {
+                    int64_t mine = (base == 0 && lane == 0) ? p : -1;
+                    for (int l = base == 0 ? 1 : 0; l < 64 && more; l++) {
+                        if (lane == l)
+                            mine = c;
+                        count++;
+                        const int64_t nx = t.next[c];
+                        if (nx == c || nx == first || nx == -1 || count >= n_nodes)
+                            more = false;
+                        c = nx;
+                    }
+                    uint64_t key = kBiFar;
+                    if (mine >= 0) {
+                        const float dc = dist_sq_ref(mv, q, mine);
+                        if (dc <= r2)
+                            key = ((uint64_t)__float_as_uint(dc) << 32) | (uint32_t)(base + lane);
+                    }
+                    const uint64_t wmin = wave_min_u64(key);
+                    if (wmin < best) {
+                        const int src = __ffsll((unsigned long long)__ballot(key == wmin)) - 1;
+                        best = wmin;
+                        bnode = __shfl(mine, src, 64);
+                    }
+                    if (!more)
+                        break;
+                }
+                if (lane == 0) {
+                    s_cover = bnode;
+                    s_wnode[0] = count;
+                }
+            }
+            __syncthreads();
+            ev += 1 + s_wnode[0];
+        }
+        __syncthreads();
+        p = s_cover;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        S->cover[i] = (int32_t)p;
+        S->key[i] = kBiFar;
+        S->ev_pre[i] = ev;
     }
 }
 
@@ -513,6 +614,8 @@ __global__ void __launch_bounds__(kBiResThreads)
             ev = 1 + S->len[tid];
             node = S->node[tid];
         }
+        ev += S->ev_pre[tid];
+        S->ev_pre[tid] = 0;
     }
     wg_add_1024(ev, s_red, &s_acc[0]);
     int nn;
@@ -986,7 +1089,13 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
         const int64_t b0 = s_begin + b * chunk;
         const int64_t bn = n - b0 < chunk ? n - b0 : chunk;
         bi_begin_kernel<<<1, kBiResThreads, 0, st>>>(links, state.ptr, b0, bn);
-        for (int32_t level = 9; level >= -10; level--) {
+        int32_t top = 9;
+        if constexpr (KIND == kMetricResidual) {
+            bi_descend_kernel<<<(unsigned)bn, kBiParThreads, 0, st>>>(mv, links, state.ptr,
+                                                                     rk.ptr, t.n);
+            top = -1;
+        }
+        for (int32_t level = top; level >= -10; level--) {
             const float r = ldexpf(1.0f, level), r2 = r * r;
             bi_scan_kernel<KIND>
                 <<<kBiParBlocks, kBiParThreads, 0, st>>>(mv, links, state.ptr, rk.ptr, r2, t.n);
