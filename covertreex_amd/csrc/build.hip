@@ -176,6 +176,18 @@ struct BiState {
     int32_t node[kBiMaxChunk], len[kBiMaxChunk];
     // distance evaluations of levels 9..0 per candidate when one launch ran them
     int64_t ev_pre[kBiMaxChunk];
+    // Incremental scans.  stay[i] = the candidate's previous scan chose its
+    // cover itself (position 0), so the cover is unchanged and every child it
+    // scanned then had d^2 >= d0[i] = d^2(candidate, cover): none of them can
+    // win now (position 0 wins ties; if d0 > r^2 they are beyond r^2 too), and
+    // only the children linked since (the list before phead[i], or root
+    // positions 1 .. nrk - plen[i]) are scored.  plen[i] = the list length at
+    // that scan, which the evaluation count still includes.
+    int32_t stay[kBiMaxChunk], plen[kBiMaxChunk], phead[kBiMaxChunk];
+    float d0[kBiMaxChunk];
+    // what this level's scan saw: d^2(candidate, cover) and the list head
+    float sd0[kBiMaxChunk];
+    int32_t shead[kBiMaxChunk];
 };
 
 // Offset of far row i's first stored word (word i / 64) in the packed upper
@@ -229,6 +241,7 @@ __global__ void __launch_bounds__(kBiResThreads)
         S->cover[i] = 0;
         S->key[i] = kBiFar;
         S->ev_pre[i] = 0;
+        S->stay[i] = 0;
     }
     if (tid == 0) {
         if (s == 0) {
@@ -251,7 +264,9 @@ __global__ void __launch_bounds__(kBiResThreads)
 // children...] (one per lane) and folds its best into key[i] with a 64-bit
 // atomic min, which picks the smallest d^2 and, among equal ones, the first
 // position, as the sequential strict-< scan does; otherwise group 0 walks
-// the cover's child list, 64 positions per pass, and stores the result.
+// the cover's child list, 64 positions per pass, and stores the result.  A
+// candidate that stayed at its cover scores only the children linked since
+// its previous scan (BiState::stay).
 template <int KIND>
 __global__ void __launch_bounds__(kBiParThreads)
     bi_scan_kernel(MetricView mv, TreeLinks t, BiState *S, const int32_t *__restrict__ rk,
@@ -269,12 +284,22 @@ __global__ void __launch_bounds__(kBiParThreads)
          it += nw) {
         const int i = (int)(it / R), g = (int)(it % R);
         const int64_t q = S->cand[i], p = S->cover[i];
+        const bool st = S->stay[i] != 0;
         if (p == 0) {
+            // positions past kn are children an earlier scan already lost to d0
+            const int kn = st ? nrk - S->plen[i] : nrk;
+            if (g * 64 > kn)
+                continue;
             const int pos = g * 64 + lane;
             uint64_t key = kBiFar;
-            if (pos <= nrk) {
-                const int64_t c = pos == 0 ? 0 : rk[nrk - pos];
-                const float dc = build_dist_sq<KIND>(mv, q, c);
+            if (pos <= kn) {
+                float dc;
+                if (pos == 0) {
+                    dc = st ? S->d0[i] : build_dist_sq<KIND>(mv, q, 0);
+                    S->sd0[i] = dc;
+                } else {
+                    dc = build_dist_sq<KIND>(mv, q, rk[nrk - pos]);
+                }
                 if (dc <= r2)
                     key = ((uint64_t)__float_as_uint(dc) << 32) | (uint32_t)pos;
             }
@@ -286,11 +311,14 @@ __global__ void __launch_bounds__(kBiParThreads)
         if (g != 0)
             continue;
         const int64_t first = t.children[p];
+        // a stayed candidate walks only up to the head its previous scan saw
+        const int64_t stop = st ? (int64_t)S->phead[i] : (int64_t)-2;
         int64_t c = first;
-        bool more = c != -1;
+        bool more = c != -1 && c != stop;
         uint64_t best = kBiFar;
         int64_t bnode = -1;
         int32_t count = 0;
+        float d0 = 0.0f;
         for (int base = 0;; base += 64) {
             int64_t mine = (base == 0 && lane == 0) ? p : -1;
             for (int l = base == 0 ? 1 : 0; l < 64 && more; l++) {
@@ -298,13 +326,19 @@ __global__ void __launch_bounds__(kBiParThreads)
                     mine = c;
                 count++;
                 const int64_t nx = t.next[c];
-                if (nx == c || nx == first || nx == -1 || count >= n_nodes)
+                if (nx == c || nx == first || nx == -1 || nx == stop || count >= n_nodes)
                     more = false;
                 c = nx;
             }
             uint64_t key = kBiFar;
             if (mine >= 0) {
-                const float dc = build_dist_sq<KIND>(mv, q, mine);
+                float dc;
+                if (base == 0 && lane == 0) {
+                    dc = st ? S->d0[i] : build_dist_sq<KIND>(mv, q, mine);
+                    d0 = dc;
+                } else {
+                    dc = build_dist_sq<KIND>(mv, q, mine);
+                }
                 if (dc <= r2)
                     key = ((uint64_t)__float_as_uint(dc) << 32) | (uint32_t)(base + lane);
             }
@@ -320,7 +354,9 @@ __global__ void __launch_bounds__(kBiParThreads)
         if (lane == 0) {
             S->key[i] = best;
             S->node[i] = (int32_t)bnode;
-            S->len[i] = count;
+            S->len[i] = count + (st ? S->plen[i] : 0);
+            S->sd0[i] = d0;
+            S->shead[i] = (int32_t)first;
         }
     }
 }
@@ -328,17 +364,20 @@ __global__ void __launch_bounds__(kBiParThreads)
 // Levels 9..0 under the residual metric.  d^2 = 1 - |clamp(rho)| lies in
 // [0, 1] <= 4^level for level >= 0, so at these levels every candidate is
 // near, no leader is linked and each candidate's descent depends only on the
-// tree as the batch found it: one workgroup per candidate runs the ten scans
-// back to back (the same positions, keys and first-minimum choice as
-// bi_scan_kernel), then leaves the candidate at its level-0 cover with its
-// distance evaluations in ev_pre.
+// tree as the batch found it: one workgroup per candidate runs the scans back
+// to back (the same positions, keys and first-minimum choice as
+// bi_scan_kernel).  Once a scan chooses the cover itself, every later scan
+// down to level 0 would repeat it (same list, every d^2 within r^2), so they
+// are only counted.  The candidate is left at its level-0 cover with its
+// evaluations in ev_pre and its incremental-scan state for level -1.
 __global__ void __launch_bounds__(kBiParThreads)
     bi_descend_kernel(MetricView mv, TreeLinks t, BiState *S, const int32_t *__restrict__ rk,
                       int64_t n_nodes)
 {
     __shared__ unsigned long long s_wmin[kBiParWaves];
-    __shared__ int64_t s_wnode[kBiParWaves];
-    __shared__ int64_t s_cover;
+    __shared__ int64_t s_cover, s_len, s_head;
+    __shared__ int32_t s_pos;
+    __shared__ float s_d0;
     const int i = (int)blockIdx.x;
     if (i >= S->nc)
         return;
@@ -353,6 +392,8 @@ __global__ void __launch_bounds__(kBiParThreads)
             for (int pos = tid; pos <= nrk; pos += kBiParThreads) {
                 const int64_t c = pos == 0 ? 0 : rk[nrk - pos];
                 const float dc = dist_sq_ref(mv, q, c);
+                if (pos == 0)
+                    s_d0 = dc;
                 if (dc <= r2) {
                     const uint64_t key = ((uint64_t)__float_as_uint(dc) << 32) | (uint32_t)pos;
                     best = key < best ? key : best;
@@ -368,58 +409,71 @@ __global__ void __launch_bounds__(kBiParThreads)
                     b = s_wmin[w] < b ? s_wmin[w] : b;
                 const int pos = (int)(uint32_t)b;
                 s_cover = pos == 0 ? 0 : rk[nrk - pos];
+                s_pos = pos;
+                s_len = nrk;
+                s_head = -1;
             }
-            ev += 1 + nrk;
-        } else {
-            if (wid == 0) {
-                const int64_t first = t.children[p];
-                int64_t c = first;
-                bool more = c != -1;
-                uint64_t best = kBiFar;
-                int64_t bnode = -1;
-                int32_t count = 0;
-                for (int base = 0;; base += 64) {
-                    int64_t mine = (base == 0 && lane == 0) ? p : -1;
-                    for (int l = base == 0 ? 1 : 0; l < 64 && more; l++) {
-                        if (lane == l)
-                            mine = c;
-                        count++;
-                        const int64_t nx = t.next[c];
-                        if (nx == c || nx == first || nx == -1 || count >= n_nodes)
-                            more = false;
-                        c = nx;
-                    }
-                    uint64_t key = kBiFar;
-                    if (mine >= 0) {
-                        const float dc = dist_sq_ref(mv, q, mine);
-                        if (dc <= r2)
-                            key = ((uint64_t)__float_as_uint(dc) << 32) | (uint32_t)(base + lane);
-                    }
-                    const uint64_t wmin = wave_min_u64(key);
-                    if (wmin < best) {
-                        const int src = __ffsll((unsigned long long)__ballot(key == wmin)) - 1;
-                        best = wmin;
-                        bnode = __shfl(mine, src, 64);
-                    }
-                    if (!more)
-                        break;
+        } else if (wid == 0) {
+            const int64_t first = t.children[p];
+            int64_t c = first;
+            bool more = c != -1;
+            uint64_t best = kBiFar;
+            int64_t bnode = -1;
+            int32_t count = 0;
+            for (int base = 0;; base += 64) {
+                int64_t mine = (base == 0 && lane == 0) ? p : -1;
+                for (int l = base == 0 ? 1 : 0; l < 64 && more; l++) {
+                    if (lane == l)
+                        mine = c;
+                    count++;
+                    const int64_t nx = t.next[c];
+                    if (nx == c || nx == first || nx == -1 || count >= n_nodes)
+                        more = false;
+                    c = nx;
                 }
-                if (lane == 0) {
-                    s_cover = bnode;
-                    s_wnode[0] = count;
+                uint64_t key = kBiFar;
+                if (mine >= 0) {
+                    const float dc = dist_sq_ref(mv, q, mine);
+                    if (base == 0 && lane == 0)
+                        s_d0 = dc;
+                    if (dc <= r2)
+                        key = ((uint64_t)__float_as_uint(dc) << 32) | (uint32_t)(base + lane);
                 }
+                const uint64_t wmin = wave_min_u64(key);
+                if (wmin < best) {
+                    const int src = __ffsll((unsigned long long)__ballot(key == wmin)) - 1;
+                    best = wmin;
+                    bnode = __shfl(mine, src, 64);
+                }
+                if (!more)
+                    break;
             }
-            __syncthreads();
-            ev += 1 + s_wnode[0];
+            if (lane == 0) {
+                s_cover = bnode;
+                s_pos = (int32_t)(uint32_t)best;
+                s_len = count;
+                s_head = first;
+            }
         }
         __syncthreads();
+        const int64_t len = s_len;
         p = s_cover;
+        const bool stayed = s_pos == 0;
         __syncthreads();
-    }
-    if (tid == 0) {
-        S->cover[i] = (int32_t)p;
-        S->key[i] = kBiFar;
-        S->ev_pre[i] = ev;
+        // this level and, when the cover stayed, levels level-1 .. 0 as well
+        ev += (stayed ? (int64_t)level + 1 : 1) * (1 + len);
+        if (stayed || level == 0) {
+            if (tid == 0) {
+                S->cover[i] = (int32_t)p;
+                S->key[i] = kBiFar;
+                S->ev_pre[i] = ev;
+                S->stay[i] = stayed ? 1 : 0;
+                S->d0[i] = s_d0;
+                S->plen[i] = (int32_t)len;
+                S->phead[i] = (int32_t)s_head;
+            }
+            return;
+        }
     }
 }
 
@@ -578,6 +632,8 @@ __global__ void __launch_bounds__(kBiResThreads)
                       int64_t *__restrict__ bs)
 {
     __shared__ int32_t s_nxc[kBiMaxChunk], s_nxv[kBiMaxChunk]; // next level
+    __shared__ int32_t s_nxst[kBiMaxChunk], s_nxpl[kBiMaxChunk], s_nxph[kBiMaxChunk];
+    __shared__ float s_nxd0[kBiMaxChunk];
     __shared__ int32_t s_far[kBiMaxChunk], s_fcov[kBiMaxChunk];
     __shared__ int32_t s_lead[kBiMaxChunk], s_lfar[kBiMaxChunk], s_lpar[kBiMaxChunk];
     __shared__ int32_t s_nlf[kBiMaxChunk];
@@ -599,21 +655,28 @@ __global__ void __launch_bounds__(kBiResThreads)
     int64_t ev = 0;
     bool near = false;
     int32_t cand = 0, cover = 0, node = -1;
+    // the next level's incremental-scan state of a near candidate
+    int32_t nst = 0, npl = 0, nph = -1;
+    float nd0 = 0.0f;
     if (tid < nc) {
         cand = S->cand[tid];
         cover = S->cover[tid];
         const uint64_t key = S->key[tid];
         near = key != kBiFar;
+        const int pos = (int)(uint32_t)key;
         if (cover == 0) {
             ev = 1 + nrk0;
-            if (near) {
-                const int pos = (int)(uint32_t)key;
+            npl = nrk0;
+            if (near)
                 node = pos == 0 ? 0 : rk[nrk0 - pos];
-            }
         } else {
-            ev = 1 + S->len[tid];
+            npl = S->len[tid];
+            ev = 1 + npl;
             node = S->node[tid];
+            nph = S->shead[tid];
         }
+        nst = near && pos == 0;
+        nd0 = S->sd0[tid];
         ev += S->ev_pre[tid];
         S->ev_pre[tid] = 0;
     }
@@ -623,6 +686,10 @@ __global__ void __launch_bounds__(kBiResThreads)
     if (near) {
         s_nxc[pn] = cand;
         s_nxv[pn] = node;
+        s_nxst[pn] = nst;
+        s_nxd0[pn] = nd0;
+        s_nxpl[pn] = npl;
+        s_nxph[pn] = nph;
     } else if (tid < nc) {
         s_far[tid - pn] = cand;
         s_fcov[tid - pn] = cover;
@@ -723,6 +790,7 @@ __global__ void __launch_bounds__(kBiResThreads)
         if (keep) {
             s_nxc[nn + pk] = s_far[s_nlf[tid]];
             s_nxv[nn + pk] = s_lead[(int)(uint32_t)s_nlkey[tid]];
+            s_nxst[nn + pk] = 0;
         }
         orphans = nnl - nk;
         ev_nl = (int64_t)nnl * nl;
@@ -734,6 +802,10 @@ __global__ void __launch_bounds__(kBiResThreads)
         S->cand[i] = s_nxc[i];
         S->cover[i] = s_nxv[i];
         S->key[i] = kBiFar;
+        S->stay[i] = s_nxst[i];
+        S->d0[i] = s_nxd0[i];
+        S->plen[i] = s_nxpl[i];
+        S->phead[i] = s_nxph[i];
     }
     if (level == -10)
         link_in_order(t, rk, s_nxc, s_nxv, nn, level, L);

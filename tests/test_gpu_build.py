@@ -13,13 +13,13 @@ from oracle import fp64
 pytestmark = pytest.mark.gpu
 
 
-def _data(n, d, m, kt, seed=0, dist="normal"):
+def _data(n, d, m, kt, seed=0, dist="normal", ls=1.0):
     rng = np.random.default_rng(seed)
     if dist == "normal":
         x = rng.standard_normal((n, d)).astype(np.float32)
     else:
         x = rng.uniform(-3, 3, (n, d)).astype(np.float32)
-    v, p, *_ = fp64.nystrom(x, inducing_count=m, seed=0, kernel_type=kt)
+    v, p, *_ = fp64.nystrom(x, inducing_count=m, seed=0, kernel_type=kt, lengthscale=ls)
     return x, v, p
 
 
@@ -31,6 +31,10 @@ BUILD_CASES = [
     dict(n=1500, d=3, m=40, kt=1, chunk=1024, gs=4.0),
     dict(n=1, d=3, m=1, kt=1, chunk=512, gs=1.0),
     dict(n=513, d=3, m=16, kt=0, chunk=512, gs=1.0),
+    # smooth field: p ~ 1e-7 for most rows, residual correlations at rounding
+    # level, many candidates kept at their cover level after level (the
+    # incremental-scan path) and a root with thousands of children
+    dict(n=8000, d=2, m=256, kt=0, chunk=512, gs=1.0, ls=1.5),
 ]
 
 
@@ -52,11 +56,12 @@ def _same_sets_modulo_cut_ties(hi, hd, oi, od):
 def _tree_pair(hip, case, seed=0):
     from covertreex_amd.native import HipTree
 
+    ls = case.get("ls", 1.0)
     x, v, p = _data(case["n"], case["d"], case["m"], case["kt"], seed=seed,
-                    dist=case.get("dist", "normal"))
-    ot = oracle.build_residual_tree(x, v, p, kernel_type=case["kt"], chunk=case["chunk"],
+                    dist=case.get("dist", "normal"), ls=ls)
+    ot = oracle.build_residual_tree(x, v, p, ls=ls, kernel_type=case["kt"], chunk=case["chunk"],
                                     grid_scale=case["gs"], exp_mode=oracle.EXP_CR)
-    ht = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32),
+    ht = HipTree.build_residual(v, p, x, 1.0, np.full(1, ls, np.float32),
                                 chunk_size=case["chunk"], grid_whiten_scale=case["gs"],
                                 kernel_type=case["kt"])
     return x, v, p, ot, ht
