@@ -15,7 +15,9 @@ TreeState::~TreeState()
 {
     if (stream)
         (void)hipStreamSynchronize(stream);
-    // DevBuf members release themselves; the stream goes last
+    if (aux_stream)
+        (void)hipStreamSynchronize(aux_stream);
+    // DevBuf members release themselves; the streams go last
     v.release();
     scaled.release();
     raw.release();
@@ -47,9 +49,13 @@ TreeState::~TreeState()
         release_host_word(host_word);
     if (sync_event)
         release_event(sync_event);
+    if (join_event)
+        release_event(join_event);
     for (int i = 0; i < kMaxMarks; i++)
         if (marks[i])
             (void)hipEventDestroy(marks[i]);
+    if (aux_stream)
+        release_stream(device, aux_stream);
     if (stream)
         release_stream(device, stream);
 }

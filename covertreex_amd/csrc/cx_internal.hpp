@@ -221,6 +221,9 @@ struct TreeState {
     // pinned host word for small device-to-host reads (hipHostMalloc)
     int32_t *host_word = nullptr;
     hipEvent_t sync_event = nullptr; // marks a small device-to-host read
+    // second stream of the mixed kNN (overflowed queries beside the rescore)
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t join_event = nullptr;
     int32_t last_hard = 0; // overflowed queries of the last mixed kNN
     float p_min = 1.0f;    // smallest p_diag, host-side at build
     double xs_max = 0.0;   // largest |coordinate / lengthscale|, host-side at build

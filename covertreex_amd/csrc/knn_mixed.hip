@@ -1049,7 +1049,7 @@ __global__ void __launch_bounds__(kRsThreads, 3) rescore_kernel(RescoreArgs A)
 // the XCD's L2 for the other queries); a merge kernel combines the segments'
 // top-k lists.
 
-constexpr int kHSeg = 1024;  // candidates per item
+constexpr int kHSeg = 512;   // candidates per item
 
 struct HardArgs {
     RescoreArgs r;
@@ -1448,6 +1448,16 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     const int32_t n_hard = *t.host_word;
     t.last_hard = n_hard;
     if (n_hard > 0) {
+        // the overflowed queries run on a second stream beside the rescore
+        // (disjoint output rows; both read only the tree and the hard list,
+        // which ev orders); the tree's stream waits for them at the end
+        if (!t.aux_stream)
+            t.aux_stream = acquire_stream(t.device);
+        if (!t.join_event)
+            t.join_event = acquire_event();
+        const hipStream_t hs = t.aux_stream;
+        StreamScope hs_scope(hs);
+        CX_HIP(hipStreamWaitEvent(hs, ev, 0));
         HardArgs ha{};
         ha.r = ra;
         ha.hard = hard;
@@ -1457,7 +1467,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
         const int32_t batch = (int32_t)std::max<int64_t>(
             1, std::min<int64_t>(n_hard, kPartBudget / query_bytes));
         if (t.part.count < (size_t)(batch * query_bytes)) {
-            CX_HIP(hipStreamSynchronize(st));
+            CX_HIP(hipStreamSynchronize(hs));
             t.part.alloc((size_t)(batch * query_bytes));
         }
         ha.part = (HardEntry *)t.part.ptr;
@@ -1471,17 +1481,19 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
             const unsigned mg =
                 (unsigned)std::min<int64_t>(((int64_t)ha.hn + kWaves - 1) / kWaves, 2048);
             if (kb == 1) {
-                hard_tile_kernel<1><<<tg, kRsThreads, 0, st>>>(ha);
-                hard_merge_kernel<1><<<mg, kRsThreads, 0, st>>>(ha);
+                hard_tile_kernel<1><<<tg, kRsThreads, 0, hs>>>(ha);
+                hard_merge_kernel<1><<<mg, kRsThreads, 0, hs>>>(ha);
             } else if (kb == 3) {
-                hard_tile_kernel<3><<<tg, kRsThreads, 0, st>>>(ha);
-                hard_merge_kernel<3><<<mg, kRsThreads, 0, st>>>(ha);
+                hard_tile_kernel<3><<<tg, kRsThreads, 0, hs>>>(ha);
+                hard_merge_kernel<3><<<mg, kRsThreads, 0, hs>>>(ha);
             } else {
-                hard_tile_kernel<7><<<tg, kRsThreads, 0, st>>>(ha);
-                hard_merge_kernel<7><<<mg, kRsThreads, 0, st>>>(ha);
+                hard_tile_kernel<7><<<tg, kRsThreads, 0, hs>>>(ha);
+                hard_merge_kernel<7><<<mg, kRsThreads, 0, hs>>>(ha);
             }
             CX_HIP(hipGetLastError());
         }
+        CX_HIP(hipEventRecord(t.join_event, hs));
+        CX_HIP(hipStreamWaitEvent(st, t.join_event, 0));
     }
     if (t.telem)
         mixed_telemetry_kernel<<<qb, 256, 0, st>>>(qs, cnt, nq, n, predecessor ? 1 : 0, t.telem);
