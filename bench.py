@@ -75,8 +75,8 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-requests", type=int, default=512)
+    ap.add_argument("--serve-rate", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -154,7 +154,9 @@ def main() -> int:
     if args.serve_requests > 0:
         from covertreex_amd.serve import Server, latency_summary, make_stream, run_poisson
 
-        reqs = make_stream(args.serve_requests, seed=0, n_min=2048, n_max=32768)
+        print(f"bench.py: serving stream, {args.serve_requests} requests at "
+              f"{args.serve_rate}/s", file=sys.stderr, flush=True)
+        reqs = make_stream(args.serve_requests, seed=0, n_min=2048, n_max=131072)
         server = Server(max_inflight=4, hbm_budget_bytes=64 << 30)
         try:
             res = run_poisson(server, reqs, rate_per_s=args.serve_rate, seed=0)
@@ -164,7 +166,7 @@ def main() -> int:
                  for k_, v_ in latency_summary(res).items()}
         serve.update({"rate_per_s": args.serve_rate, "max_inflight": 4,
                       "peak_inflight": server.peak_inflight,
-                      "workload": "Poisson arrivals; N log-uniform [2048, 32768], D in {2,3,5}, "
+                      "workload": "Poisson arrivals; N log-uniform [2048, 131072], D in {2,3,5}, "
                                   "Matern/RBF l~U[0.5,2], m in {64,128,256}, k in {10,30,50}, "
                                   "predecessor; build + all-points kNN per request"})
 
