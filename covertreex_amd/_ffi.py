@@ -45,6 +45,7 @@ CX_QT_FRONTIER = 14
 CX_QT_EXPANDED = 78
 CX_QT_BLOCKS = 142
 CX_QT_HARD_LISTED = 145
+CX_QT_RESCREENED = 146
 # cover-tree invariant counts (CX_INV_*)
 CX_INV_FIELDS = 8
 INV_FIELDS = ("covering", "nesting", "separation", "forest", "nodes", "pairs", "batches")

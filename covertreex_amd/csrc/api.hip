@@ -43,6 +43,9 @@ TreeState::~TreeState()
     isp_ds.release();
     codes.release();
     part.release();
+    vsl.release();
+    vexp.release();
+    hs_buf.release();
     work.release();
     telem_buf.release();
     if (host_word)

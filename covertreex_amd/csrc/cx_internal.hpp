@@ -221,6 +221,12 @@ struct TreeState {
     // pinned host word for small device-to-host reads (hipHostMalloc)
     int32_t *host_word = nullptr;
     hipEvent_t sync_event = nullptr; // marks a small device-to-host read
+    // exact-product screen of overflowed queries: int8 slices of V and row
+    // exponents (dataset order, built on first use), bounds workspace
+    DevBuf<int8_t> vsl;
+    DevBuf<int32_t> vexp;
+    bool slices_ready = false;
+    DevBuf<uint8_t> hs_buf;
     // second stream of the mixed kNN (overflowed queries beside the rescore)
     hipStream_t aux_stream = nullptr;
     hipEvent_t join_event = nullptr;

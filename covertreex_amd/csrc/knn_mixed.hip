@@ -641,11 +641,11 @@ static inline float kernel_error(float var, int32_t d)
 // the square root of max(p_q p_j, 1e-18) and the division add 2^-42 |rho|;
 // s = 1 - |rho| adds 2^-43; then |sqrt(s) - sqrt(s')| <= min(sqrt(e_s),
 // e_s / (sqrt(s) + sqrt(max(s - e_s, 0)))), plus 2^-43 sqrt(s) for df_sqrt.
-__device__ __forceinline__ df finish_distance(const RescoreArgs &A, int64_t qt, int64_t jt, df dot,
-                                              float dot_err, float *err)
+__device__ __forceinline__ df finish_distance_at(const RescoreArgs &A, const float *xq, float pq,
+                                                 const float *xj, float pj, df dot, float dot_err,
+                                                 float *err)
 {
     const int32_t d = A.d;
-    const float *xq = A.raw + qt * d, *xj = A.raw + jt * d;
     df d2 = {0.0f, 0.0f};
     for (int c = 0; c < d; c++) {
         const df diff = two_sum(xq[c], -xj[c]);
@@ -664,7 +664,7 @@ __device__ __forceinline__ df finish_distance(const RescoreArgs &A, int64_t qt, 
     }
     kv = df_mul_f(kv, A.var);
     const df num = df_sub(kv, dot);
-    df pp = two_prod(A.p[qt], A.p[jt]);
+    df pp = two_prod(pq, pj);
     if (pp.hi < 1e-18f)
         pp = {1e-18f, 0.0f};
     const df den = df_sqrt(pp);
@@ -688,6 +688,14 @@ __device__ __forceinline__ df finish_distance(const RescoreArgs &A, int64_t qt, 
     e_d = e_d < e_sq ? e_d : e_sq;
     *err = (e_d + sd * 1.2e-13f) * 1.01f + sd * 6e-8f;
     return dist;
+}
+
+// the same for tree rows qt and jt
+__device__ __forceinline__ df finish_distance(const RescoreArgs &A, int64_t qt, int64_t jt, df dot,
+                                              float dot_err, float *err)
+{
+    return finish_distance_at(A, A.raw + qt * A.d, A.p[qt], A.raw + jt * A.d, A.p[jt], dot,
+                              dot_err, err);
 }
 
 // One Dot2 step: TwoProduct a*b, TwoSum into s, the two errors into c.
@@ -1190,6 +1198,410 @@ __global__ void mixed_telemetry_kernel(const int64_t *__restrict__ qsorted,
     }
 }
 
+// ---- exact-product screen of overflowed queries --------------------------------
+//
+// A query whose list overflowed (tiny p makes w = 1/sqrt(p_q p_j) large, so
+// the BF16 interval is wide) gets a second certified screen before the
+// rescore when such queries are few.  Each V row is split into six int8
+// slices under a power-of-two scale 2^e, |V_i| < 2^e:
+//   V_i = 2^e (A1 2^-6 + A2 2^-13 + ... + A6 2^-41) + r_i,  |A| <= 64, |r_i| <= 2^(e-42)
+// (each slice is the nearest integer of the scaled remainder; every step is
+// exact in FP32).  The int8 MFMA sums slice products exactly in int32; with
+// G_t = sum_{a+b=t} A_a(q) . A_b(j) for t = 2..7,
+//   V_q . V_j = 2^(e_q+e_j) sum_t 2^-(12+7(t-2)) G_t + E,
+//   |E| <= 2^(e_q+e_j-42) m (5.04 [pairs a+b >= 8] + 2.01 [r terms]
+//                            + 1.6 [double-float sum of the six terms]),
+// |A_a . A_b| <= 2^12 m.  finish_distance (the rescore's double-float
+// evaluation and error bound) turns the dot into d with a certified error,
+// so [d - err, d + err] holds the exact distance.  Pass 1 stores the squared
+// bounds of every valid pair; tau, the k-th smallest upper bound rounded up
+// to a 2^-19 bin edge, bounds the k-th exact d^2; pass 2 lists every pair
+// whose lower bound is <= tau.  A query whose list
+// now fits kCap is rescored with the others; one that still overflows takes
+// the exhaustive path.
+
+constexpr int kSlices = 6;
+constexpr int kHsBins = 1024;          // bins per level of the tau selection
+constexpr int kHsTile = 128;           // candidates per tile (four wavefronts x 32)
+constexpr int kHsSeg = 2 * kHsTile;    // candidates per workgroup
+constexpr int kHsRow = 48;             // LDS bytes per staged 32-byte slice row (padded)
+constexpr int64_t kHsBudget = (int64_t)512 << 20; // bytes of stored bounds per batch
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// Slices and row exponent of dataset row blockIdx.x (tree row inv_order[.]).
+__global__ void slice_kernel(const float *__restrict__ v, const int64_t *__restrict__ inv_order,
+                             int64_t n, int32_t m, int32_t mp, int8_t *__restrict__ sl,
+                             int32_t *__restrict__ sexp)
+{
+    __shared__ float s_max[4];
+    const int64_t row = blockIdx.x;
+    if (row >= n)
+        return;
+    const float *src = v + inv_order[row] * (int64_t)m;
+    float mx = 0.0f;
+    for (int32_t c = threadIdx.x; c < m; c += blockDim.x)
+        mx = __builtin_fmaxf(mx, __builtin_fabsf(src[c]));
+    for (int o = 32; o > 0; o >>= 1)
+        mx = __builtin_fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0)
+        s_max[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = __builtin_fmaxf(__builtin_fmaxf(s_max[0], s_max[1]), __builtin_fmaxf(s_max[2], s_max[3]));
+    int e = 0;
+    if (mx > 0.0f)
+        (void)__builtin_frexpf(mx, &e); // mx = f 2^e, f in [0.5, 1): |V| < 2^e
+    if (threadIdx.x == 0)
+        sexp[row] = e;
+    int8_t *dst = sl + row * (int64_t)kSlices * mp;
+    for (int32_t c = threadIdx.x; c < mp; c += blockDim.x) {
+        float r = c < m ? __builtin_ldexpf(src[c], -e) : 0.0f;
+        float s = 64.0f;
+#pragma unroll
+        for (int a = 0; a < kSlices; a++) {
+            const float y = r * s;
+            const float q = __builtin_rintf(y);
+            dst[(int64_t)a * mp + c] = (int8_t)(int)q;
+            r = y - q; // exact: the fractional part of y
+            s = 128.0f;
+        }
+    }
+}
+
+struct HsArgs {
+    RescoreArgs r;         // tree-order data (finish_distance), queries, lists
+    const int8_t *sl;      // npad x kSlices x mp, dataset order
+    const int32_t *sexp;   // npad row exponents
+    const int32_t *hard;   // sorted positions of the overflowed queries
+    int32_t h0, hn;        // this batch: hard[h0 .. h0 + hn)
+    int32_t mp;
+    float2 *bounds;        // hn x n squared (lower, upper) bounds
+    float *tau;            // hn
+    int32_t *hcnt;         // hn
+    int32_t *list;         // kCap slots per sorted query
+    int32_t *cnt;          // per sorted query
+    int64_t *telem;        // query counters or null
+};
+
+// double-float of an int32 (exact: both parts are integers below 2^24)
+__device__ __forceinline__ df df_of_int(int32_t g)
+{
+    const float hi = (float)g;
+    return {hi, (float)(g - (int32_t)hi)};
+}
+
+// Pass 1.  Workgroup = 32 overflowed queries (MFMA columns) x a segment of
+// 256 candidates in tiles of 128 (four wavefronts x 32 MFMA rows).
+__global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
+{
+    __shared__ __attribute__((aligned(16))) int8_t sA[kSlices * kHsTile * kHsRow];
+    __shared__ __attribute__((aligned(16))) int8_t sB[kSlices * 32 * kHsRow];
+    __shared__ int32_t sQrow[32], sEj[kHsTile];
+    __shared__ float sXj[kHsTile * kMaxD], sPj[kHsTile];
+    const RescoreArgs &A = H.r;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t n = A.n;
+    const int32_t mp = H.mp;
+    const int hq = (int)blockIdx.x * 32 + (lane & 31); // this lane's query column
+    const bool qlive = hq < H.hn;
+    const int64_t s = qlive ? H.hard[H.h0 + hq] : 0;
+    const int64_t lim = qlive ? hard_limit(A, s) : 0;
+    const int32_t qrow = qlive ? A.qrow[s] : 0;
+    const int64_t qt = A.inv_order[qrow];
+    const int32_t eq = H.sexp[qrow];
+    const float *xq = A.raw + qt * A.d;
+    const float pq = A.p[qt];
+    if (tid < 32) {
+        const int h = (int)blockIdx.x * 32 + tid;
+        sQrow[tid] = h < H.hn ? A.qrow[H.hard[H.h0 + h]] : 0;
+    }
+    // the segment ends at the largest limit of the 32 queries
+    int64_t lmax = lim;
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) {
+        const int64_t u = __shfl_xor(lmax, o, 64);
+        lmax = u > lmax ? u : lmax;
+    }
+    __shared__ int64_t s_lmax[4];
+    if (lane == 0)
+        s_lmax[w] = lmax;
+    __syncthreads();
+    lmax = s_lmax[0];
+    for (int i = 1; i < 4; i++)
+        lmax = s_lmax[i] > lmax ? s_lmax[i] : lmax;
+    const int64_t seg0 = (int64_t)blockIdx.y * kHsSeg;
+    const int64_t seg1 = seg0 + kHsSeg < lmax ? seg0 + kHsSeg : lmax;
+    const float m9 = 9.0f * (float)A.m;
+    for (int64_t c0 = seg0; c0 < seg1; c0 += kHsTile) {
+        __syncthreads(); // the previous tile's epilogue still reads sEj
+        if (tid < kHsTile) {
+            // the tile's row exponents and tree-order coordinates and p
+            const int64_t j = c0 + tid < n ? c0 + tid : 0;
+            sEj[tid] = H.sexp[j];
+            const int64_t jt = A.inv_order[j];
+            for (int c = 0; c < A.d; c++)
+                sXj[tid * A.d + c] = A.raw[jt * A.d + c];
+            sPj[tid] = A.p[jt];
+        }
+        v16i G[6];
+#pragma unroll
+        for (int t = 0; t < 6; t++)
+            G[t] = v16i{};
+        for (int32_t kc = 0; kc < mp; kc += 32) {
+            __syncthreads();
+            // candidate slices: 128 rows x 6 slices x 2 halves of 16 bytes
+            for (int u = tid; u < kHsTile * kSlices * 2; u += 256) {
+                const int row = u / (kSlices * 2), rem = u % (kSlices * 2);
+                const int a = rem >> 1, half = rem & 1;
+                const int64_t j = c0 + row;
+                const int64_t jr = j < n ? j : 0;
+                const uint4 val =
+                    *(const uint4 *)(H.sl + (jr * kSlices + a) * mp + kc + 16 * half);
+                *(uint4 *)(sA + (a * kHsTile + row) * kHsRow + 16 * half) = val;
+            }
+            for (int u = tid; u < 32 * kSlices * 2; u += 256) {
+                const int row = u / (kSlices * 2), rem = u % (kSlices * 2);
+                const int a = rem >> 1, half = rem & 1;
+                const uint4 val =
+                    *(const uint4 *)(H.sl + ((int64_t)sQrow[row] * kSlices + a) * mp + kc +
+                                     16 * half);
+                *(uint4 *)(sB + (a * 32 + row) * kHsRow + 16 * half) = val;
+            }
+            __syncthreads();
+            // any consistent K order works for A and B: the sums are exact
+            const int koff = 16 * (lane >> 5);
+            v4i av[kSlices], bv[kSlices];
+#pragma unroll
+            for (int a = 0; a < kSlices; a++) {
+                av[a] = *(const v4i *)(sA + (a * kHsTile + 32 * w + (lane & 31)) * kHsRow + koff);
+                bv[a] = *(const v4i *)(sB + (a * 32 + (lane & 31)) * kHsRow + koff);
+            }
+#pragma unroll
+            for (int t = 2; t <= 7; t++)
+#pragma unroll
+                for (int a = 1; a <= 6; a++) {
+                    const int b = t - a;
+                    if (b >= 1 && b <= 6)
+                        G[t - 2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a - 1], bv[b - 1],
+                                                                         G[t - 2], 0, 0, 0);
+                }
+        }
+        // register r of lane l: candidate row 8(r/4) + 4(l/32) + r%4 of the
+        // wavefront's 32, query column l%32
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int rw = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int64_t j = c0 + 32 * w + rw;
+            if (!qlive || j >= lim)
+                continue;
+            const int32_t ej = sEj[32 * w + rw];
+            const int es = eq + ej;
+            df dot = {0.0f, 0.0f};
+#pragma unroll
+            for (int t = 0; t < 6; t++) {
+                const df g = df_of_int(G[t][r]);
+                const int sh = es - (12 + 7 * t);
+                dot = df_add(dot, df{__builtin_ldexpf(g.hi, sh), __builtin_ldexpf(g.lo, sh)});
+            }
+            const float e_dot = __builtin_ldexpf(m9, es - 42) + 1e-37f;
+            float err;
+            const int ci = 32 * w + rw;
+            const df dist =
+                finish_distance_at(A, xq, pq, sXj + ci * A.d, sPj[ci], dot, e_dot, &err);
+            const float e2 = err + __builtin_fabsf(dist.lo);
+            const float lo = dist.hi - e2 > 0.0f ? dist.hi - e2 : 0.0f;
+            const float hi = dist.hi + e2;
+            const float lb2 = lo * lo * (1.0f - 2.4e-7f);
+            const float ub2 = hi * hi * (1.0f + 2.4e-7f);
+            H.bounds[(int64_t)hq * n + j] = make_float2(lb2, ub2);
+        }
+    }
+}
+
+// tau per query (one workgroup each): the k-th smallest upper bound, rounded
+// up to a bin edge, from two histogram levels over the stored bounds -- 1024
+// bins of width 2^-9 over [0, 2), then 1024 sub-bins of the bin holding the
+// k-th (width 2^-19).  Binning is exact: v * 2^9 and (v - b/2^9) * 2^19 are
+// exact in FP32 for v in bin b, and tau = (1024 b1 + b2 + 1) 2^-19 is
+// representable, so every bound counted at or below the chosen sub-bin is
+// <= tau.  +inf when the query has fewer than k pairs.
+__device__ __forceinline__ int hs_find(const int32_t *hist, int32_t need, int32_t *below, int *s_w,
+                                       int32_t *s_tot)
+{
+    // first bin where the running count reaches need; *below = count before it
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int32_t c[4], sum = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        c[i] = hist[4 * tid + i];
+        sum += c[i];
+    }
+    int32_t incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t u = __shfl_up(incl, o, 64);
+        if (lane >= o)
+            incl += u;
+    }
+    if (lane == 63)
+        s_tot[wid] = incl;
+    __syncthreads();
+    int32_t off = 0;
+    for (int w = 0; w < wid; w++)
+        off += s_tot[w];
+    incl += off;
+    const int32_t excl = incl - sum;
+    if (tid == 0)
+        *s_w = kHsBins;
+    __syncthreads();
+    if (excl < need && incl >= need) {
+        int32_t run = excl;
+        for (int i = 0; i < 4; i++) {
+            run += c[i];
+            if (run >= need) {
+                *s_w = 4 * tid + i;
+                *below = run - c[i];
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    const int r = *s_w;
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(256) hs_tau_kernel(HsArgs H, int32_t k)
+{
+    __shared__ int32_t s_hist[kHsBins];
+    __shared__ int32_t s_tot[4], s_below;
+    __shared__ int s_bin;
+    const RescoreArgs &A = H.r;
+    const int h = (int)blockIdx.x, tid = threadIdx.x;
+    const int64_t n = A.n;
+    const int64_t lim = hard_limit(A, H.hard[H.h0 + h]);
+    const float2 *bd = H.bounds + (int64_t)h * n;
+    for (int i = tid; i < kHsBins; i += 256)
+        s_hist[i] = 0;
+    __syncthreads();
+    for (int64_t j = tid; j < lim; j += 256) {
+        const float v = bd[j].y;
+        const int b = (v >= 0.0f && v < 2.0f) ? (int)(v * 512.0f) : kHsBins - 1;
+        atomicAdd(&s_hist[b], 1);
+    }
+    __syncthreads();
+    if (tid == 0)
+        s_below = 0;
+    const int b1 = hs_find(s_hist, k, &s_below, &s_bin, s_tot);
+    if (b1 >= kHsBins - 1) { // fewer than k pairs, or the k-th is past 2 - 2^-9
+        if (tid == 0)
+            H.tau[h] = __builtin_inff();
+        return;
+    }
+    const int32_t below1 = s_below;
+    const float lo1 = (float)b1 * (1.0f / 512.0f);
+    for (int i = tid; i < kHsBins; i += 256)
+        s_hist[i] = 0;
+    __syncthreads();
+    for (int64_t j = tid; j < lim; j += 256) {
+        const float v = bd[j].y;
+        if (v >= lo1 && v < lo1 + (1.0f / 512.0f)) {
+            const int b = min((int)((v - lo1) * 524288.0f), kHsBins - 1);
+            atomicAdd(&s_hist[b], 1);
+        }
+    }
+    __syncthreads();
+    const int b2 = hs_find(s_hist, k - below1, &s_below, &s_bin, s_tot);
+    if (tid == 0)
+        H.tau[h] = (float)(1024 * b1 + min(b2, kHsBins - 1) + 1) * (1.0f / 524288.0f);
+}
+
+// Pass 2: list every valid pair with lower bound <= tau (slots past kCap are
+// counted, not written).
+__global__ void hs_list_kernel(HsArgs H)
+{
+    const RescoreArgs &A = H.r;
+    const int64_t n = A.n;
+    const int h = (int)blockIdx.y;
+    const int64_t s = H.hard[H.h0 + h];
+    const int64_t lim = hard_limit(A, s);
+    const float tau = H.tau[h];
+    int32_t *dst = H.list + s * kCap;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < lim;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        if (H.bounds[(int64_t)h * n + j].x <= tau) {
+            const int32_t p = atomicAdd(&H.hcnt[h], 1);
+            if (p < kCap)
+                dst[p] = (int32_t)j;
+        }
+    }
+}
+
+// The new counts replace the overflow marks: a query whose list fits is
+// rescored with the others.
+__global__ void hs_commit_kernel(HsArgs H)
+{
+    const int h = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (h < H.hn)
+        H.cnt[H.hard[H.h0 + h]] = H.hcnt[h];
+    if (h == 0 && H.telem)
+        atomicAdd((unsigned long long *)&H.telem[CX_QT_RESCREENED], (unsigned long long)H.hn);
+}
+
+// The overflowed queries hard[0 .. n_hard) get the exact-product screen;
+// the ones whose new list fits kCap become listed queries (cnt rewritten).
+static void rescreen_hard(TreeState &t, const RescoreArgs &ra, int32_t *hard, int32_t n_hard,
+                          int32_t k, int32_t *cnt, int32_t *list, hipStream_t st)
+{
+    const int64_t n = t.n;
+    if (!t.slices_ready) {
+        const int64_t npad = ((n + kCT - 1) / kCT) * kCT + kCT;
+        t.vsl.alloc((size_t)npad * kSlices * t.mp);
+        t.vexp.alloc(npad);
+        CX_HIP(hipMemsetAsync(t.vsl.ptr, 0, t.vsl.bytes(), st));
+        CX_HIP(hipMemsetAsync(t.vexp.ptr, 0, t.vexp.bytes(), st));
+        slice_kernel<<<(unsigned)n, 256, 0, st>>>(t.v.ptr, t.inv_order.ptr, n, t.m, t.mp,
+                                                  t.vsl.ptr, t.vexp.ptr);
+        CX_HIP(hipGetLastError());
+        t.slices_ready = true;
+    }
+    const int64_t per_query = n * (int64_t)sizeof(float2);
+    const int32_t batch =
+        (int32_t)std::max<int64_t>(1, std::min<int64_t>(n_hard, kHsBudget / per_query));
+    const size_t b_bounds = (size_t)batch * per_query;
+    const size_t b_small = (size_t)batch * sizeof(int32_t);
+    const size_t need = b_bounds + 2 * b_small + 1024;
+    if (t.hs_buf.count < need) {
+        CX_HIP(hipStreamSynchronize(st));
+        t.hs_buf.alloc(need);
+    }
+    uint8_t *base = t.hs_buf.ptr;
+    HsArgs H{};
+    H.r = ra;
+    H.sl = t.vsl.ptr;
+    H.sexp = t.vexp.ptr;
+    H.hard = hard;
+    H.mp = t.mp;
+    H.bounds = (float2 *)base;
+    H.tau = (float *)(base + b_bounds);
+    H.hcnt = (int32_t *)(base + b_bounds + b_small);
+    H.list = list;
+    H.cnt = cnt;
+    H.telem = t.telem;
+    for (int32_t h0 = 0; h0 < n_hard; h0 += batch) {
+        H.h0 = h0;
+        H.hn = std::min(batch, n_hard - h0);
+        CX_HIP(hipMemsetAsync(H.hcnt, 0, (size_t)H.hn * sizeof(int32_t), st));
+        const dim3 g1((unsigned)((H.hn + 31) / 32), (unsigned)((n + kHsSeg - 1) / kHsSeg));
+        hs_bounds_kernel<<<g1, 256, 0, st>>>(H);
+        hs_tau_kernel<<<(unsigned)H.hn, 256, 0, st>>>(H, k);
+        const dim3 g2((unsigned)std::min<int64_t>((n + 255) / 256, 64), (unsigned)H.hn);
+        hs_list_kernel<<<g2, 256, 0, st>>>(H);
+        hs_commit_kernel<<<(unsigned)((H.hn + 255) / 256), 256, 0, st>>>(H);
+        CX_HIP(hipGetLastError());
+    }
+}
+
 // ---- driver -----------------------------------------------------------------
 
 __global__ void query_rows_kernel(const int64_t *__restrict__ qsorted, int64_t nq, int64_t n,
@@ -1227,6 +1639,7 @@ static void prepare_mixed(TreeState &t)
     CX_HIP(hipMemsetAsync(t.vlo.ptr, 0, t.vlo.bytes(), st));
     t.vnorm.alloc(npad);
     CX_HIP(hipMemsetAsync(t.vnorm.ptr, 0, t.vnorm.bytes(), st));
+    t.slices_ready = false; // the exact-product screen's slices follow V
     split_kernel<<<(unsigned)n, 256, 0, st>>>(t.v.ptr, t.inv_order.ptr, n, t.m, t.mp, t.vhi.ptr,
                                               t.vlo.ptr, t.vnorm.ptr);
     t.x_ds.alloc((size_t)npad * t.d);
@@ -1429,6 +1842,18 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
         t.sync_event = acquire_event();
     hipEvent_t ev = t.sync_event;
     CX_HIP(hipEventRecord(ev, st));
+    // a few overflowed queries: the exact-product screen first, then the
+    // ones still overflowing
+    CX_HIP(hipEventSynchronize(ev));
+    const int32_t n_over = *t.host_word;
+    if (n_over > 0 && (int64_t)n_over <= std::max<int64_t>(256, n / 16)) {
+        rescreen_hard(t, ra, hard, n_over, k, cnt, list, st);
+        hard_flag_kernel<<<qb, 256, 0, st>>>(cnt, nq, flag);
+        CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan_bytes, flag, pos, (int)nq, st));
+        hard_scatter_kernel<<<qb, 256, 0, st>>>(flag, pos, nq, hard, n_hard_d);
+        CX_HIP(hipMemcpyAsync(t.host_word, n_hard_d, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        CX_HIP(hipEventRecord(ev, st));
+    }
     // ... while the listed queries are rescored
     // top-k held as KB entries per lane: k <= 64 KB
     const int kb = k <= 64 ? 1 : k <= 192 ? 3 : 7;

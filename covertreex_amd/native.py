@@ -164,6 +164,7 @@ class HipTree:
             "block_sizes": [],
             "block_size_counts": {size: int(c) for size, c in zip((16, 32, 64), blocks)},
             "hard_listed": int(out[_ffi.CX_QT_HARD_LISTED]),
+            "rescreened_queries": int(out[_ffi.CX_QT_RESCREENED]),
         })
         return t
 

@@ -249,6 +249,7 @@ cx_status cx_tree_build_telemetry(const cx_tree *t, int64_t *out, int64_t cap_ba
 #define CX_QT_EXPANDED 78       /* [CX_QT_LEVELS] children gathered per level */
 #define CX_QT_BLOCKS 142        /* [3] tiles of 16, 32, 64 candidates */
 #define CX_QT_HARD_LISTED 145   /* mixed: pairs the screen kept for the hard queries */
+#define CX_QT_RESCREENED 146   /* mixed: overflowed queries given the exact-product screen */
 cx_status cx_tree_set_query_telemetry(cx_tree *t, int32_t enable);
 /* Cover-tree invariants (the properties batch insert establishes; the
  * oracle's tests check the same on the CPU, tests/test_oracle_tree.py):

@@ -108,8 +108,10 @@ def test_c2_shape_sample(hip):
     t.set_query_telemetry(True)
     idx, dst, err = t.knn_residual(q, 50, predecessor_mode=True, mode="mixed", return_error=True)
     # the inducing rows (p at the jitter floor) overflow the screen's lists and
-    # take the hard-query path: a quarter of them join the random sample
-    assert t.last_query_telemetry()["hard_queries"] > 100
+    # take the exact-product screen: a quarter of them join the random sample
+    tel = t.last_query_telemetry()
+    assert tel["rescreened_queries"] > 100
+    assert tel["hard_queries"] < tel["rescreened_queries"] // 4
     sample = np.concatenate([np.random.default_rng(1).choice(32768, 256, replace=False),
                              ind[1::4]])
     worst = _check(idx[sample], dst[sample], err[sample], x, v, p, sample, 50, True, 1)
@@ -120,3 +122,26 @@ def test_c2_shape_sample(hip):
     assert np.all(np.where(valid, idx, -1) < q[:, None])
     dd = np.where(valid, dst, 2.0)
     assert np.all(np.diff(dd, axis=1) >= 0)
+
+
+def test_smooth_field_exhaustive_path(hip):
+    # RBF, l = 1.5 in D = 2 with m = 256: p ~ 1e-7 on nearly every row, so
+    # nearly every query overflows and (too many for the exact-product
+    # screen) is scored against all its candidates
+    from covertreex_amd.native import HipTree
+
+    n, k = 8000, 30
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((n, 2)).astype(np.float32)
+    v, p, *_ = fp64.nystrom(x, inducing_count=256, seed=0, kernel_type=0, lengthscale=1.5)
+    t = HipTree.build_residual(v, p, x, 1.0, np.full(1, 1.5, np.float32), kernel_type=0)
+    t.set_query_telemetry(True)
+    q = np.arange(n)
+    idx, dst, err = t.knn_residual(q, k, predecessor_mode=True, mode="mixed", return_error=True)
+    tel = t.last_query_telemetry()
+    assert tel["hard_queries"] > n // 2 and tel["rescreened_queries"] == 0
+    sample = rng.choice(n, 200, replace=False)
+    worst = _check(idx[sample], dst[sample], err[sample], x, v, p, sample, k, True, 0, ls=1.5)
+    print(f"smooth field max |d - d64| = {worst:.3e}", flush=True)
+    valid = idx >= 0
+    assert np.array_equal(valid.sum(1), np.minimum(k, q))
