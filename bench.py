@@ -67,6 +67,37 @@ def cpu_baseline(x, v, p, sample_q: int):
             "mean_evals_per_query": float(evals.mean())}
 
 
+def measure_peaks(torch, dev) -> dict:
+    """What this device sustains on two plain library operations: a 1 GiB
+    device copy (read + write bytes / time) and a dense 8192^3 BF16 GEMM
+    (hipBLASLt through torch.mm), each timed with events over repeated calls."""
+    out = {}
+    a = torch.ones((1 << 28,), dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    out["hbm_copy_GBps"] = round(5 * 2 * a.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+    del a, b
+    n = 8192
+    x = torch.randn((n, n), dtype=torch.bfloat16, device=dev)
+    y = torch.randn((n, n), dtype=torch.bfloat16, device=dev)
+    z = torch.mm(x, y)
+    e0.record()
+    for _ in range(10):
+        z = torch.mm(x, y)
+    e1.record()
+    torch.cuda.synchronize()
+    out["bf16_gemm_TFLOPs"] = round(10 * 2 * n ** 3 / (e0.elapsed_time(e1) * 1e-3) / 1e12, 1)
+    del x, y, z
+    torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,7 +107,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512)
-    ap.add_argument("--serve-rate", type=float, default=10.0)
+    ap.add_argument("--serve-rate", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -103,6 +134,7 @@ def main() -> int:
     from covertreex_amd.native import HipTree
 
     dev = torch.device("cuda:0")
+    peaks = measure_peaks(torch, dev)
     x, v, p = make_inputs()
     t0 = time.perf_counter()
     tree = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
@@ -213,6 +245,8 @@ def main() -> int:
                      "algorithmic_flops_per_launch": N * flops_per_q},
         "cpu_baseline": cpu,
         "serve": serve,
+        "measured_peaks": dict(peaks, screen_frac_of_measured_bf16_gemm=round(
+            achieved / peaks["bf16_gemm_TFLOPs"], 4) if peaks.get("bf16_gemm_TFLOPs") else None),
     }
     print(json.dumps(out))
     return 0
