@@ -50,8 +50,12 @@ TreeState::~TreeState()
     telem_buf.release();
     if (host_word)
         release_host_word(host_word);
+    if (host_word2)
+        release_host_word(host_word2);
     if (sync_event)
         release_event(sync_event);
+    if (sync_event2)
+        release_event(sync_event2);
     if (join_event)
         release_event(join_event);
     for (int i = 0; i < kMaxMarks; i++)

@@ -219,8 +219,8 @@ struct TreeState {
     // reusable query workspace
     DevBuf<uint8_t> work;
     // pinned host word for small device-to-host reads (hipHostMalloc)
-    int32_t *host_word = nullptr;
-    hipEvent_t sync_event = nullptr; // marks a small device-to-host read
+    int32_t *host_word = nullptr, *host_word2 = nullptr;
+    hipEvent_t sync_event = nullptr, sync_event2 = nullptr; // small device-to-host reads
     // exact-product screen of overflowed queries: int8 slices of V and row
     // exponents (dataset order, built on first use), bounds workspace
     DevBuf<int8_t> vsl;

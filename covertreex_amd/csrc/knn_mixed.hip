@@ -594,8 +594,10 @@ struct RescoreArgs {
     const int64_t *qsorted;
     const int64_t *qperm; // sorted position -> original query position
     const int32_t *qrow;
-    const int32_t *cnt;   // listed candidates per sorted query
+    const int32_t *cnt;   // listed candidates per sorted query (per qlist entry if set)
     const int32_t *list;  // kCap slots per sorted query
+    const int32_t *qlist; // optional: rescore only these sorted positions
+    int64_t nlist;
     const float *vnorm;   // |V_i| rounded up, dataset order
     int64_t n, nq;
     int32_t d, m, kernel, pred, k;
@@ -1030,10 +1032,11 @@ __global__ void __launch_bounds__(kRsThreads, 3) rescore_kernel(RescoreArgs A)
     const int x = blockIdx.x % kXcds;
     {
         const int64_t local = (int64_t)(blockIdx.x / kXcds) * kWaves + w;
-        const int64_t s = (x + kXcds * (local / kXcdQueries)) * kXcdQueries + local % kXcdQueries;
-        if (s >= A.nq)
+        const int64_t ix = (x + kXcds * (local / kXcdQueries)) * kXcdQueries + local % kXcdQueries;
+        if (ix >= (A.qlist ? A.nlist : A.nq))
             return;
-        const int32_t count = A.cnt[s];
+        const int64_t s = A.qlist ? (int64_t)A.qlist[ix] : ix;
+        const int32_t count = A.cnt[ix];
         if (count > kCap)
             return; // scored over all its candidates by the hard kernels
         const int32_t qrow = A.qrow[s];
@@ -1160,6 +1163,38 @@ __global__ void hard_scatter_kernel(const int32_t *__restrict__ flag,
         *n_hard = pos[s] + flag[s];
 }
 
+// ordered compaction over the overflowed-query list: the queries whose new
+// list still overflows (cnt2 > kCap) go to dst; the count to n_out
+__global__ void over_flag_kernel(const int32_t *__restrict__ cnt2, int64_t n, int32_t *flag)
+{
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < n)
+        flag[h] = cnt2[h] > kCap ? 1 : 0;
+}
+
+__global__ void over_scatter_kernel(const int32_t *__restrict__ flag,
+                                    const int32_t *__restrict__ pos, int64_t n,
+                                    const int32_t *__restrict__ src, int32_t *dst,
+                                    int32_t *n_out)
+{
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= n)
+        return;
+    if (flag[h])
+        dst[pos[h]] = src[h];
+    if (h == n - 1)
+        *n_out = pos[h] + flag[h];
+}
+
+// after the main rescore: the overflowed queries' final counts
+__global__ void over_commit_kernel(const int32_t *__restrict__ hard,
+                                   const int32_t *__restrict__ cnt2, int64_t n, int32_t *cnt)
+{
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < n)
+        cnt[hard[h]] = cnt2[h];
+}
+
 // Query telemetry of a mixed call: pairs screened (every valid candidate of
 // every query), candidates listed for rescoring, hard queries and their
 // pairs, the longest list.
@@ -1279,7 +1314,7 @@ struct HsArgs {
     float *tau;            // hn
     int32_t *hcnt;         // hn
     int32_t *list;         // kCap slots per sorted query
-    int32_t *cnt;          // per sorted query
+    int32_t *cnt;          // new count per position of the overflowed-query list
     int64_t *telem;        // query counters or null
 };
 
@@ -1537,13 +1572,12 @@ __global__ void hs_list_kernel(HsArgs H)
     }
 }
 
-// The new counts replace the overflow marks: a query whose list fits is
-// rescored with the others.
+// The new counts, per position of the overflowed-query list.
 __global__ void hs_commit_kernel(HsArgs H)
 {
     const int h = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (h < H.hn)
-        H.cnt[H.hard[H.h0 + h]] = H.hcnt[h];
+        H.cnt[H.h0 + h] = H.hcnt[h];
     if (h == 0 && H.telem)
         atomicAdd((unsigned long long *)&H.telem[CX_QT_RESCREENED], (unsigned long long)H.hn);
 }
@@ -1697,6 +1731,8 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     const size_t o_flag = take(sizeof(int32_t) * nq);
     const size_t o_pos = take(sizeof(int32_t) * nq);
     const size_t o_hard = take(sizeof(int32_t) * nq);
+    const size_t o_hard2 = take(sizeof(int32_t) * nq);
+    const size_t o_cnt2 = take(sizeof(int32_t) * nq);
     size_t sort_bytes = 0, scan_bytes = 0;
     CX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int64_t *)nullptr,
                                               (int64_t *)nullptr, (const int64_t *)nullptr,
@@ -1713,8 +1749,11 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     int32_t *hist = (int32_t *)(ws + o_hist);
     int32_t *flag = (int32_t *)(ws + o_flag), *pos = (int32_t *)(ws + o_pos);
     int32_t *hard = (int32_t *)(ws + o_hard);
-    // ctr[0]: the hard-query count
-    int32_t *n_hard_d = ctr;
+    // overflowed queries still overflowing after the exact-product screen,
+    // and the new counts per position of the overflowed-query list
+    int32_t *hard2 = (int32_t *)(ws + o_hard2), *cnt2 = (int32_t *)(ws + o_cnt2);
+    // ctr[0]: the overflowed-query count; ctr[1]: those still overflowing
+    int32_t *n_hard_d = ctr, *n_hard2_d = ctr + 1;
 
     const unsigned qb = (unsigned)((nq + 255) / 256);
     iota64_kernel<<<qb, 256, 0, st>>>(iota, nq);
@@ -1842,40 +1881,34 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
         t.sync_event = acquire_event();
     hipEvent_t ev = t.sync_event;
     CX_HIP(hipEventRecord(ev, st));
-    // a few overflowed queries: the exact-product screen first, then the
-    // ones still overflowing
-    CX_HIP(hipEventSynchronize(ev));
-    const int32_t n_over = *t.host_word;
-    if (n_over > 0 && (int64_t)n_over <= std::max<int64_t>(256, n / 16)) {
-        rescreen_hard(t, ra, hard, n_over, k, cnt, list, st);
-        hard_flag_kernel<<<qb, 256, 0, st>>>(cnt, nq, flag);
-        CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan_bytes, flag, pos, (int)nq, st));
-        hard_scatter_kernel<<<qb, 256, 0, st>>>(flag, pos, nq, hard, n_hard_d);
-        CX_HIP(hipMemcpyAsync(t.host_word, n_hard_d, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        CX_HIP(hipEventRecord(ev, st));
-    }
     // ... while the listed queries are rescored
     // top-k held as KB entries per lane: k <= 64 KB
     const int kb = k <= 64 ? 1 : k <= 192 ? 3 : 7;
-    // workgroups: each XCD gets ceil(chunks / 8) chunks of kXcdQueries queries
-    const int64_t n_chunks = (nq + kXcdQueries - 1) / kXcdQueries;
-    const unsigned rs_grid =
-        (unsigned)(kXcds * ((n_chunks + kXcds - 1) / kXcds) * (kXcdQueries / kWaves));
-    if (kb == 1)
-        rescore_kernel<1><<<rs_grid, kRsThreads, 0, st>>>(ra);
-    else if (kb == 3)
-        rescore_kernel<3><<<rs_grid, kRsThreads, 0, st>>>(ra);
-    else
-        rescore_kernel<7><<<rs_grid, kRsThreads, 0, st>>>(ra);
-    CX_HIP(hipGetLastError());
+    auto launch_rescore = [&](const RescoreArgs &r, int64_t count, hipStream_t sm) {
+        // workgroups: each XCD gets ceil(chunks / 8) chunks of kXcdQueries queries
+        const int64_t n_chunks = (count + kXcdQueries - 1) / kXcdQueries;
+        const unsigned grid =
+            (unsigned)(kXcds * ((n_chunks + kXcds - 1) / kXcds) * (kXcdQueries / kWaves));
+        if (kb == 1)
+            rescore_kernel<1><<<grid, kRsThreads, 0, sm>>>(r);
+        else if (kb == 3)
+            rescore_kernel<3><<<grid, kRsThreads, 0, sm>>>(r);
+        else
+            rescore_kernel<7><<<grid, kRsThreads, 0, sm>>>(r);
+        CX_HIP(hipGetLastError());
+    };
+    launch_rescore(ra, nq, st);
     prof_mark(t, 6);
     CX_HIP(hipEventSynchronize(ev));
-    const int32_t n_hard = *t.host_word;
-    t.last_hard = n_hard;
-    if (n_hard > 0) {
-        // the overflowed queries run on a second stream beside the rescore
-        // (disjoint output rows; both read only the tree and the hard list,
-        // which ev orders); the tree's stream waits for them at the end
+    const int32_t n_over = *t.host_word;
+    int32_t n_hard = n_over;
+    if (n_over > 0) {
+        // The overflowed queries run on a second stream beside the rescore,
+        // which skips them (disjoint output rows; everything they read is
+        // ordered by ev).  A few of them get the exact-product screen and
+        // are rescored from their new lists; the ones still overflowing, or
+        // all of them when there are many, are scored exhaustively.  Their
+        // final counts are written after the rescore, on the tree's stream.
         if (!t.aux_stream)
             t.aux_stream = acquire_stream(t.device);
         if (!t.join_event)
@@ -1883,43 +1916,73 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
         const hipStream_t hs = t.aux_stream;
         StreamScope hs_scope(hs);
         CX_HIP(hipStreamWaitEvent(hs, ev, 0));
-        HardArgs ha{};
-        ha.r = ra;
-        ha.hard = hard;
-        ha.n_hard = n_hard;
-        ha.nseg = (int32_t)((n + kHSeg - 1) / kHSeg);
-        const int64_t query_bytes = (int64_t)ha.nseg * k * sizeof(HardEntry);
-        const int32_t batch = (int32_t)std::max<int64_t>(
-            1, std::min<int64_t>(n_hard, kPartBudget / query_bytes));
-        if (t.part.count < (size_t)(batch * query_bytes)) {
-            CX_HIP(hipStreamSynchronize(hs));
-            t.part.alloc((size_t)(batch * query_bytes));
+        const int32_t *exhaustive = hard;
+        const bool rescreen = (int64_t)n_over <= std::max<int64_t>(256, n / 16);
+        const unsigned ob = (unsigned)((n_over + 255) / 256);
+        if (rescreen) {
+            rescreen_hard(t, ra, hard, n_over, k, cnt2, list, hs);
+            RescoreArgs sub = ra;
+            sub.cnt = cnt2;
+            sub.qlist = hard;
+            sub.nlist = n_over;
+            launch_rescore(sub, n_over, hs);
+            over_flag_kernel<<<ob, 256, 0, hs>>>(cnt2, n_over, flag);
+            CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan_bytes, flag, pos,
+                                                    (int)n_over, hs));
+            over_scatter_kernel<<<ob, 256, 0, hs>>>(flag, pos, n_over, hard, hard2, n_hard2_d);
+            if (!t.host_word2)
+                t.host_word2 = acquire_host_word();
+            if (!t.sync_event2)
+                t.sync_event2 = acquire_event();
+            CX_HIP(hipMemcpyAsync(t.host_word2, n_hard2_d, sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, hs));
+            CX_HIP(hipEventRecord(t.sync_event2, hs));
+            CX_HIP(hipEventSynchronize(t.sync_event2));
+            n_hard = *t.host_word2;
+            exhaustive = hard2;
         }
-        ha.part = (HardEntry *)t.part.ptr;
-        for (int32_t h0 = 0; h0 < n_hard; h0 += batch) {
-            ha.h0 = h0;
-            ha.hn = std::min(batch, n_hard - h0);
-            // workgroups: each XCD gets ceil(nseg / 8) segments x hn queries
-            const int64_t per_xcd = ((ha.nseg + kXcds - 1) / kXcds) * (int64_t)ha.hn;
-            const unsigned tg = (unsigned)(kXcds * ((per_xcd + kWaves - 1) / kWaves));
-            // one wavefront per hard query of the batch
-            const unsigned mg =
-                (unsigned)std::min<int64_t>(((int64_t)ha.hn + kWaves - 1) / kWaves, 2048);
-            if (kb == 1) {
-                hard_tile_kernel<1><<<tg, kRsThreads, 0, hs>>>(ha);
-                hard_merge_kernel<1><<<mg, kRsThreads, 0, hs>>>(ha);
-            } else if (kb == 3) {
-                hard_tile_kernel<3><<<tg, kRsThreads, 0, hs>>>(ha);
-                hard_merge_kernel<3><<<mg, kRsThreads, 0, hs>>>(ha);
-            } else {
-                hard_tile_kernel<7><<<tg, kRsThreads, 0, hs>>>(ha);
-                hard_merge_kernel<7><<<mg, kRsThreads, 0, hs>>>(ha);
+        if (n_hard > 0) {
+            HardArgs ha{};
+            ha.r = ra;
+            ha.hard = exhaustive;
+            ha.n_hard = n_hard;
+            ha.nseg = (int32_t)((n + kHSeg - 1) / kHSeg);
+            const int64_t query_bytes = (int64_t)ha.nseg * k * sizeof(HardEntry);
+            const int32_t batch = (int32_t)std::max<int64_t>(
+                1, std::min<int64_t>(n_hard, kPartBudget / query_bytes));
+            if (t.part.count < (size_t)(batch * query_bytes)) {
+                CX_HIP(hipStreamSynchronize(hs));
+                t.part.alloc((size_t)(batch * query_bytes));
             }
-            CX_HIP(hipGetLastError());
+            ha.part = (HardEntry *)t.part.ptr;
+            for (int32_t h0 = 0; h0 < n_hard; h0 += batch) {
+                ha.h0 = h0;
+                ha.hn = std::min(batch, n_hard - h0);
+                // workgroups: each XCD gets ceil(nseg / 8) segments x hn queries
+                const int64_t per_xcd = ((ha.nseg + kXcds - 1) / kXcds) * (int64_t)ha.hn;
+                const unsigned tg = (unsigned)(kXcds * ((per_xcd + kWaves - 1) / kWaves));
+                // one wavefront per query of the batch
+                const unsigned mg =
+                    (unsigned)std::min<int64_t>(((int64_t)ha.hn + kWaves - 1) / kWaves, 2048);
+                if (kb == 1) {
+                    hard_tile_kernel<1><<<tg, kRsThreads, 0, hs>>>(ha);
+                    hard_merge_kernel<1><<<mg, kRsThreads, 0, hs>>>(ha);
+                } else if (kb == 3) {
+                    hard_tile_kernel<3><<<tg, kRsThreads, 0, hs>>>(ha);
+                    hard_merge_kernel<3><<<mg, kRsThreads, 0, hs>>>(ha);
+                } else {
+                    hard_tile_kernel<7><<<tg, kRsThreads, 0, hs>>>(ha);
+                    hard_merge_kernel<7><<<mg, kRsThreads, 0, hs>>>(ha);
+                }
+                CX_HIP(hipGetLastError());
+            }
         }
         CX_HIP(hipEventRecord(t.join_event, hs));
         CX_HIP(hipStreamWaitEvent(st, t.join_event, 0));
+        if (rescreen)
+            over_commit_kernel<<<ob, 256, 0, st>>>(hard, cnt2, n_over, cnt);
     }
+    t.last_hard = n_hard;
     if (t.telem)
         mixed_telemetry_kernel<<<qb, 256, 0, st>>>(qs, cnt, nq, n, predecessor ? 1 : 0, t.telem);
     prof_mark(t, 7);
