@@ -643,9 +643,9 @@ static inline float kernel_error(float var, int32_t d)
 // the square root of max(p_q p_j, 1e-18) and the division add 2^-42 |rho|;
 // s = 1 - |rho| adds 2^-43; then |sqrt(s) - sqrt(s')| <= min(sqrt(e_s),
 // e_s / (sqrt(s) + sqrt(max(s - e_s, 0)))), plus 2^-43 sqrt(s) for df_sqrt.
-__device__ __forceinline__ df finish_distance_at(const RescoreArgs &A, const float *xq, float pq,
-                                                 const float *xj, float pj, df dot, float dot_err,
-                                                 float *err)
+// var * k(x_q, x_j) in double-float from raw coordinates (within A.e_kern)
+__device__ __forceinline__ df kernel_value_df(const RescoreArgs &A, const float *xq,
+                                              const float *xj)
 {
     const int32_t d = A.d;
     df d2 = {0.0f, 0.0f};
@@ -664,7 +664,14 @@ __device__ __forceinline__ df finish_distance_at(const RescoreArgs &A, const flo
             df_add(df_add(df{1.0f, 0.0f}, a), df_mul(df_mul(a, a), df{third.hi, third.lo}));
         kv = df_mul(poly, df_exp(df_neg(a)));
     }
-    kv = df_mul_f(kv, A.var);
+    return df_mul_f(kv, A.var);
+}
+
+__device__ __forceinline__ df finish_distance_at(const RescoreArgs &A, const float *xq, float pq,
+                                                 const float *xj, float pj, df dot, float dot_err,
+                                                 float *err)
+{
+    const df kv = kernel_value_df(A, xq, xj);
     const df num = df_sub(kv, dot);
     df pp = two_prod(pq, pj);
     if (pp.hi < 1e-18f)
@@ -1246,9 +1253,10 @@ __global__ void mixed_telemetry_kernel(const int64_t *__restrict__ qsorted,
 //   V_q . V_j = 2^(e_q+e_j) sum_t 2^-(12+7(t-2)) G_t + E,
 //   |E| <= 2^(e_q+e_j-42) m (5.04 [pairs a+b >= 8] + 2.01 [r terms]
 //                            + 1.6 [double-float sum of the six terms]),
-// |A_a . A_b| <= 2^12 m.  finish_distance (the rescore's double-float
-// evaluation and error bound) turns the dot into d with a certified error,
-// so [d - err, d + err] holds the exact distance.  Pass 1 stores the squared
+// |A_a . A_b| <= 2^12 m.  With the kernel value in double-float
+// (kernel_value_df, the rescore's evaluation) the numerator k - dot carries
+// a certified error; |rho| = |k - dot| / sqrt(p_q p_j) follows in FP32 with
+// its relative rounding, which bounds d^2 = 1 - |rho| from both sides.  Pass 1 stores the squared
 // bounds of every valid pair; tau, the k-th smallest upper bound rounded up
 // to a 2^-19 bin edge, bounds the k-th exact d^2; pass 2 lists every pair
 // whose lower bound is <= tau.  A query whose list
@@ -1307,6 +1315,7 @@ struct HsArgs {
     RescoreArgs r;         // tree-order data (finish_distance), queries, lists
     const int8_t *sl;      // npad x kSlices x mp, dataset order
     const int32_t *sexp;   // npad row exponents
+    const float *isp;      // n: 1/sqrt(p), dataset order
     const int32_t *hard;   // sorted positions of the overflowed queries
     int32_t h0, hn;        // this batch: hard[h0 .. h0 + hn)
     int32_t mp;
@@ -1332,7 +1341,7 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
     __shared__ __attribute__((aligned(16))) int8_t sA[kSlices * kHsTile * kHsRow];
     __shared__ __attribute__((aligned(16))) int8_t sB[kSlices * 32 * kHsRow];
     __shared__ int32_t sQrow[32], sEj[kHsTile];
-    __shared__ float sXj[kHsTile * kMaxD], sPj[kHsTile];
+    __shared__ float sXj[kHsTile * kMaxD], sWj[kHsTile];
     const RescoreArgs &A = H.r;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t n = A.n;
@@ -1345,7 +1354,7 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
     const int64_t qt = A.inv_order[qrow];
     const int32_t eq = H.sexp[qrow];
     const float *xq = A.raw + qt * A.d;
-    const float pq = A.p[qt];
+    const float qisp = H.isp[qrow];
     if (tid < 32) {
         const int h = (int)blockIdx.x * 32 + tid;
         sQrow[tid] = h < H.hn ? A.qrow[H.hard[H.h0 + h]] : 0;
@@ -1376,7 +1385,7 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
             const int64_t jt = A.inv_order[j];
             for (int c = 0; c < A.d; c++)
                 sXj[tid * A.d + c] = A.raw[jt * A.d + c];
-            sPj[tid] = A.p[jt];
+            sWj[tid] = H.isp[j];
         }
         v16i G[6];
 #pragma unroll
@@ -1439,15 +1448,23 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
                 dot = df_add(dot, df{__builtin_ldexpf(g.hi, sh), __builtin_ldexpf(g.lo, sh)});
             }
             const float e_dot = __builtin_ldexpf(m9, es - 42) + 1e-37f;
-            float err;
+            // |rho| = |k - dot| w, w = min(1/sqrt(p_q p_j), 1e9): the
+            // numerator in double-float (its error: the dot's, the kernel
+            // value's and the subtraction's, as in finish_distance), w and the
+            // product in FP32 (relative error < 6e-7 together)
             const int ci = 32 * w + rw;
-            const df dist =
-                finish_distance_at(A, xq, pq, sXj + ci * A.d, sPj[ci], dot, e_dot, &err);
-            const float e2 = err + __builtin_fabsf(dist.lo);
-            const float lo = dist.hi - e2 > 0.0f ? dist.hi - e2 : 0.0f;
-            const float hi = dist.hi + e2;
-            const float lb2 = lo * lo * (1.0f - 2.4e-7f);
-            const float ub2 = hi * hi * (1.0f + 2.4e-7f);
+            const df kv = kernel_value_df(A, xq, sXj + ci * A.d);
+            const df num = df_sub(kv, dot);
+            const float en = __builtin_fabsf(num.lo) + e_dot + A.e_kern +
+                             5.7e-14f * (__builtin_fabsf(kv.hi) + __builtin_fabsf(dot.hi));
+            const float wq = __builtin_fminf(qisp * sWj[ci], 1e9f);
+            const float ar = __builtin_fabsf(num.hi) * wq;
+            const float er = en * wq * 1.000001f + ar * 6e-7f;
+            const float rlo = __builtin_fminf(__builtin_fmaxf(ar - er, 0.0f), 1.0f);
+            const float rhi = __builtin_fminf(ar + er, 1.0f);
+            // d^2 = 1 - |rho|; 1.2e-7 covers the rounding of 1 - r
+            const float lb2 = __builtin_fmaxf((1.0f - rhi) - 1.2e-7f, 0.0f);
+            const float ub2 = (1.0f - rlo) + 1.2e-7f;
             H.bounds[(int64_t)hq * n + j] = make_float2(lb2, ub2);
         }
     }
@@ -1614,6 +1631,7 @@ static void rescreen_hard(TreeState &t, const RescoreArgs &ra, int32_t *hard, in
     H.r = ra;
     H.sl = t.vsl.ptr;
     H.sexp = t.vexp.ptr;
+    H.isp = t.isp_ds.ptr;
     H.hard = hard;
     H.mp = t.mp;
     H.bounds = (float2 *)base;
