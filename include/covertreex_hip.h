@@ -168,7 +168,8 @@ cx_status cx_nystrom_residual(const double *points, int64_t n, int32_t d,
  * HIP events are recorded on the tree's stream at phase boundaries.
  * cx_tree_last_profile writes up to cap durations (ms):
  *   CX_KNN_MIXED:         [prep, query sort, screen pass 1, tau, filter,
- *                          rescore, hard queries]
+ *                          rescore, overflowed queries past the rescore's end
+ *                          (they run on a second stream beside it)]
  *   CX_KNN_REFERENCE_F32: [traversal]                                         */
 cx_status cx_tree_set_profiling(cx_tree *t, int32_t enable);
 cx_status cx_tree_last_profile(cx_tree *t, float *ms_out, int32_t cap, int32_t *n_out);
@@ -241,14 +242,14 @@ cx_status cx_tree_build_telemetry(const cx_tree *t, int64_t *out, int64_t cap_ba
 #define CX_QT_BUDGET_ESC 7      /* budget_escalations */
 #define CX_QT_BUDGET_EARLY 8    /* budget_early_terminate */
 #define CX_QT_LISTED 9          /* mixed: candidates listed for rescoring */
-#define CX_QT_HARD_QUERIES 10   /* mixed: queries scored over all candidates */
+#define CX_QT_HARD_QUERIES 10   /* mixed: queries scored exhaustively (over all candidates) */
 #define CX_QT_HARD_PAIRS 11     /* mixed: pairs scored for those queries */
 #define CX_QT_MAX_LIST 12       /* mixed: longest list */
 #define CX_QT_N_LEVELS 13       /* levels recorded below (<= CX_QT_LEVELS) */
 #define CX_QT_FRONTIER 14       /* [CX_QT_LEVELS] frontier size per level, summed */
 #define CX_QT_EXPANDED 78       /* [CX_QT_LEVELS] children gathered per level */
 #define CX_QT_BLOCKS 142        /* [3] tiles of 16, 32, 64 candidates */
-#define CX_QT_HARD_LISTED 145   /* mixed: pairs the screen kept for the hard queries */
+#define CX_QT_HARD_LISTED 145   /* mixed: pairs the last screen kept for those queries */
 #define CX_QT_RESCREENED 146   /* mixed: overflowed queries given the exact-product screen */
 cx_status cx_tree_set_query_telemetry(cx_tree *t, int32_t enable);
 /* Cover-tree invariants (the properties batch insert establishes; the
