@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per screen_kernel<3, 1> launch on this workload: rocprofv3
 # FETCH_SIZE (x2.000, calibrated on a 1 GiB copy) + WRITE_SIZE (x1.000),
 # recorded in profiles/mixed_knn_c2_n32768_m256_k50.md
-SCREEN_PASS1_HBM_BYTES = 3.085e9
+SCREEN_PASS1_HBM_BYTES = 3.076e9
 
 
 def make_inputs():
@@ -240,6 +240,8 @@ def main() -> int:
                      "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4),
                      "traffic": SCREEN_PASS1_HBM_BYTES,
+                     # the launch's calibrated HBM bytes over its measured time
+                     "hbm_frac": round(SCREEN_PASS1_HBM_BYTES / (screen_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
                      "traffic_source": "profiles/mixed_knn_c2_n32768_m256_k50.md",
                      "kernel": "screen_kernel<3, 1>",
                      "algorithmic_flops_per_launch": N * flops_per_q},
