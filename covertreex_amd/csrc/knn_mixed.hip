@@ -1334,8 +1334,9 @@ __device__ __forceinline__ df df_of_int(int32_t g)
     return {hi, (float)(g - (int32_t)hi)};
 }
 
-// Pass 1.  Workgroup = 32 overflowed queries (MFMA columns) x a segment of
-// 256 candidates in tiles of 128 (four wavefronts x 32 MFMA rows).
+// Pass 1.  Workgroup (segment x, query tile y) = 32 overflowed queries (MFMA
+// columns) x a segment of 256 candidates in tiles of 128 (four wavefronts x
+// 32 MFMA rows).
 __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
 {
     __shared__ __attribute__((aligned(16))) int8_t sA[kSlices * kHsTile * kHsRow];
@@ -1346,7 +1347,7 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t n = A.n;
     const int32_t mp = H.mp;
-    const int hq = (int)blockIdx.x * 32 + (lane & 31); // this lane's query column
+    const int hq = (int)blockIdx.y * 32 + (lane & 31); // this lane's query column
     const bool qlive = hq < H.hn;
     const int64_t s = qlive ? H.hard[H.h0 + hq] : 0;
     const int64_t lim = qlive ? hard_limit(A, s) : 0;
@@ -1356,7 +1357,7 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
     const float *xq = A.raw + qt * A.d;
     const float qisp = H.isp[qrow];
     if (tid < 32) {
-        const int h = (int)blockIdx.x * 32 + tid;
+        const int h = (int)blockIdx.y * 32 + tid;
         sQrow[tid] = h < H.hn ? A.qrow[H.hard[H.h0 + h]] : 0;
     }
     // the segment ends at the largest limit of the 32 queries
@@ -1373,7 +1374,7 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
     lmax = s_lmax[0];
     for (int i = 1; i < 4; i++)
         lmax = s_lmax[i] > lmax ? s_lmax[i] : lmax;
-    const int64_t seg0 = (int64_t)blockIdx.y * kHsSeg;
+    const int64_t seg0 = (int64_t)blockIdx.x * kHsSeg;
     const int64_t seg1 = seg0 + kHsSeg < lmax ? seg0 + kHsSeg : lmax;
     const float m9 = 9.0f * (float)A.m;
     for (int64_t c0 = seg0; c0 < seg1; c0 += kHsTile) {
@@ -1644,7 +1645,8 @@ static void rescreen_hard(TreeState &t, const RescoreArgs &ra, int32_t *hard, in
         H.h0 = h0;
         H.hn = std::min(batch, n_hard - h0);
         CX_HIP(hipMemsetAsync(H.hcnt, 0, (size_t)H.hn * sizeof(int32_t), st));
-        const dim3 g1((unsigned)((H.hn + 31) / 32), (unsigned)((n + kHsSeg - 1) / kHsSeg));
+        // x: candidate segments (up to 2^31 - 1), y: tiles of 32 queries
+        const dim3 g1((unsigned)((n + kHsSeg - 1) / kHsSeg), (unsigned)((H.hn + 31) / 32));
         hs_bounds_kernel<<<g1, 256, 0, st>>>(H);
         hs_tau_kernel<<<(unsigned)H.hn, 256, 0, st>>>(H, k);
         const dim3 g2((unsigned)std::min<int64_t>((n + 255) / 256, 64), (unsigned)H.hn);
