@@ -1722,6 +1722,9 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     CX_REQUIRE(t.d <= kMaxD, "mixed-precision kNN supports d <= 16");
     CX_REQUIRE(t.n < ((int64_t)1 << 31) - 4 * kSeg,
                "mixed-precision kNN supports fewer than 2^31 - 8192 points");
+    // the screen's grid y counts 2048-candidate segments (at most 65535)
+    CX_REQUIRE(t.n <= (int64_t)65535 * kSeg,
+               "mixed-precision kNN supports at most 65535 * 2048 points");
     if (nq <= 0)
         return;
     prof_mark(t, 0);
