@@ -1334,15 +1334,14 @@ __device__ __forceinline__ df df_of_int(int32_t g)
     return {hi, (float)(g - (int32_t)hi)};
 }
 
-// Pass 1.  Workgroup (segment x, query tile y) = 32 overflowed queries (MFMA
+// Pass 1a.  Workgroup (segment x, query tile y) = 32 overflowed queries (MFMA
 // columns) x a segment of 256 candidates in tiles of 128 (four wavefronts x
-// 32 MFMA rows).
+// 32 MFMA rows); stores each valid pair's dot as a double-float.
 __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
 {
     __shared__ __attribute__((aligned(16))) int8_t sA[kSlices * kHsTile * kHsRow];
     __shared__ __attribute__((aligned(16))) int8_t sB[kSlices * 32 * kHsRow];
     __shared__ int32_t sQrow[32], sEj[kHsTile];
-    __shared__ float sXj[kHsTile * kMaxD], sWj[kHsTile];
     const RescoreArgs &A = H.r;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t n = A.n;
@@ -1352,10 +1351,7 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
     const int64_t s = qlive ? H.hard[H.h0 + hq] : 0;
     const int64_t lim = qlive ? hard_limit(A, s) : 0;
     const int32_t qrow = qlive ? A.qrow[s] : 0;
-    const int64_t qt = A.inv_order[qrow];
     const int32_t eq = H.sexp[qrow];
-    const float *xq = A.raw + qt * A.d;
-    const float qisp = H.isp[qrow];
     if (tid < 32) {
         const int h = (int)blockIdx.y * 32 + tid;
         sQrow[tid] = h < H.hn ? A.qrow[H.hard[H.h0 + h]] : 0;
@@ -1376,17 +1372,11 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
         lmax = s_lmax[i] > lmax ? s_lmax[i] : lmax;
     const int64_t seg0 = (int64_t)blockIdx.x * kHsSeg;
     const int64_t seg1 = seg0 + kHsSeg < lmax ? seg0 + kHsSeg : lmax;
-    const float m9 = 9.0f * (float)A.m;
     for (int64_t c0 = seg0; c0 < seg1; c0 += kHsTile) {
         __syncthreads(); // the previous tile's epilogue still reads sEj
         if (tid < kHsTile) {
-            // the tile's row exponents and tree-order coordinates and p
             const int64_t j = c0 + tid < n ? c0 + tid : 0;
             sEj[tid] = H.sexp[j];
-            const int64_t jt = A.inv_order[j];
-            for (int c = 0; c < A.d; c++)
-                sXj[tid * A.d + c] = A.raw[jt * A.d + c];
-            sWj[tid] = H.isp[j];
         }
         v16i G[6];
 #pragma unroll
@@ -1448,26 +1438,47 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
                 const int sh = es - (12 + 7 * t);
                 dot = df_add(dot, df{__builtin_ldexpf(g.hi, sh), __builtin_ldexpf(g.lo, sh)});
             }
-            const float e_dot = __builtin_ldexpf(m9, es - 42) + 1e-37f;
-            // |rho| = |k - dot| w, w = min(1/sqrt(p_q p_j), 1e9): the
-            // numerator in double-float (its error: the dot's, the kernel
-            // value's and the subtraction's, as in finish_distance), w and the
-            // product in FP32 (relative error < 6e-7 together)
-            const int ci = 32 * w + rw;
-            const df kv = kernel_value_df(A, xq, sXj + ci * A.d);
-            const df num = df_sub(kv, dot);
-            const float en = __builtin_fabsf(num.lo) + e_dot + A.e_kern +
-                             5.7e-14f * (__builtin_fabsf(kv.hi) + __builtin_fabsf(dot.hi));
-            const float wq = __builtin_fminf(qisp * sWj[ci], 1e9f);
-            const float ar = __builtin_fabsf(num.hi) * wq;
-            const float er = en * wq * 1.000001f + ar * 6e-7f;
-            const float rlo = __builtin_fminf(__builtin_fmaxf(ar - er, 0.0f), 1.0f);
-            const float rhi = __builtin_fminf(ar + er, 1.0f);
-            // d^2 = 1 - |rho|; 1.2e-7 covers the rounding of 1 - r
-            const float lb2 = __builtin_fmaxf((1.0f - rhi) - 1.2e-7f, 0.0f);
-            const float ub2 = (1.0f - rlo) + 1.2e-7f;
-            H.bounds[(int64_t)hq * n + j] = make_float2(lb2, ub2);
+            H.bounds[(int64_t)hq * n + j] = make_float2(dot.hi, dot.lo);
         }
+    }
+}
+
+// Pass 1b, one thread per (query, candidate): the interval of d^2 from the
+// stored dot, in place.  |rho| = |k - dot| w with w = min(1/sqrt(p_q p_j),
+// 1e9): the numerator in double-float (its error: the dot's, the kernel
+// value's and the subtraction's, as in finish_distance), w and the product in
+// FP32 (relative error < 6e-7 together); d^2 = 1 - |rho|, with 1.2e-7 for
+// the rounding of 1 - r.
+__global__ void __launch_bounds__(256) hs_interval_kernel(HsArgs H)
+{
+    const RescoreArgs &A = H.r;
+    const int64_t n = A.n;
+    const int h = (int)blockIdx.y;
+    const int64_t s = H.hard[H.h0 + h];
+    const int64_t lim = hard_limit(A, s);
+    const int32_t qrow = A.qrow[s];
+    const float *xq = A.raw + A.inv_order[qrow] * A.d;
+    const float qisp = H.isp[qrow];
+    const int32_t eq = H.sexp[qrow];
+    const float m9 = 9.0f * (float)A.m;
+    float2 *row = H.bounds + (int64_t)h * n;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < lim;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const float2 dv = row[j];
+        const df dot = {dv.x, dv.y};
+        const float e_dot = __builtin_ldexpf(m9, eq + H.sexp[j] - 42) + 1e-37f;
+        const df kv = kernel_value_df(A, xq, A.raw + A.inv_order[j] * A.d);
+        const df num = df_sub(kv, dot);
+        const float en = __builtin_fabsf(num.lo) + e_dot + A.e_kern +
+                         5.7e-14f * (__builtin_fabsf(kv.hi) + __builtin_fabsf(dot.hi));
+        const float wq = __builtin_fminf(qisp * H.isp[j], 1e9f);
+        const float ar = __builtin_fabsf(num.hi) * wq;
+        const float er = en * wq * 1.000001f + ar * 6e-7f;
+        const float rlo = __builtin_fminf(__builtin_fmaxf(ar - er, 0.0f), 1.0f);
+        const float rhi = __builtin_fminf(ar + er, 1.0f);
+        const float lb2 = __builtin_fmaxf((1.0f - rhi) - 1.2e-7f, 0.0f);
+        const float ub2 = (1.0f - rlo) + 1.2e-7f;
+        row[j] = make_float2(lb2, ub2);
     }
 }
 
@@ -1648,8 +1659,9 @@ static void rescreen_hard(TreeState &t, const RescoreArgs &ra, int32_t *hard, in
         // x: candidate segments (up to 2^31 - 1), y: tiles of 32 queries
         const dim3 g1((unsigned)((n + kHsSeg - 1) / kHsSeg), (unsigned)((H.hn + 31) / 32));
         hs_bounds_kernel<<<g1, 256, 0, st>>>(H);
-        hs_tau_kernel<<<(unsigned)H.hn, 256, 0, st>>>(H, k);
         const dim3 g2((unsigned)std::min<int64_t>((n + 255) / 256, 64), (unsigned)H.hn);
+        hs_interval_kernel<<<g2, 256, 0, st>>>(H);
+        hs_tau_kernel<<<(unsigned)H.hn, 256, 0, st>>>(H, k);
         hs_list_kernel<<<g2, 256, 0, st>>>(H);
         hs_commit_kernel<<<(unsigned)((H.hn + 255) / 256), 256, 0, st>>>(H);
         CX_HIP(hipGetLastError());
