@@ -145,3 +145,25 @@ def test_smooth_field_exhaustive_path(hip):
     print(f"smooth field max |d - d64| = {worst:.3e}", flush=True)
     valid = idx >= 0
     assert np.array_equal(valid.sum(1), np.minimum(k, q))
+
+
+@pytest.mark.parametrize("pred,k", [(False, 100), (True, 24)])
+def test_overflowed_queries_rescreened(hip, pred, k):
+    # C1 shape (N=4096, D=3, Matern, m=64): the inducing rows overflow the
+    # BF16 screen and are listed again by the exact-product screen, then
+    # rescored with the others (k = 100 takes the three-entries-per-lane
+    # top-k; without the mask the query's own row is a candidate)
+    from covertreex_amd.native import HipTree
+
+    x = np.random.default_rng(0).standard_normal((4096, 3)).astype(np.float32)
+    v, p, _, _, _, ind = fp64.nystrom(x, inducing_count=64, seed=0, kernel_type=1)
+    t = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
+    t.set_query_telemetry(True)
+    q = np.arange(4096)
+    idx, dst, err = t.knn_residual(q, k, predecessor_mode=pred, mode="mixed", return_error=True)
+    tel = t.last_query_telemetry()
+    assert tel["rescreened_queries"] > 0
+    sample = np.concatenate([ind, np.random.default_rng(2).choice(4096, 64, replace=False)])
+    worst = _check(idx[sample], dst[sample], err[sample], x, v, p, sample, k, pred, 1)
+    print(f"rescreened {tel['rescreened_queries']} exhaustive {tel['hard_queries']} "
+          f"max |d - d64| = {worst:.3e}", flush=True)
