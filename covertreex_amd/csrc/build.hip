@@ -361,12 +361,127 @@ __global__ void __launch_bounds__(kBiParThreads)
     }
 }
 
+// Level 9 of a residual batch: every candidate starts at the root, so the
+// scan is the dense block (candidates) x (root, root's children).  A
+// workgroup takes 64 candidates x 64 positions (position tiles grid-strided),
+// a thread 4 x 4 pairs: the V dots in the reference's f32 order (dot_slice:
+// chunks of eight, each summed as two in-order quadruples and added in order,
+// then the tail) from LDS-staged chunks, so each row is read once per tile
+// rather than once per pair; then the rest of dist_sq_ref.  Each candidate's
+// best (d^2 bits, position) is folded into its key with an atomic min (first
+// minimum wins, as in the sequential scan); d^2 to the root itself goes to
+// sd0.
+constexpr int kRtTile = 64;
+// below this many root children the descent scans the root's list itself
+// (one workgroup per candidate is then quicker than the tiled block)
+constexpr int kRtMin = 2048;
+__global__ void __launch_bounds__(256)
+    bi_rootscan_kernel(MetricView mv, BiState *S, const int32_t *__restrict__ rk, float r2)
+{
+    __shared__ float sC[kRtTile][9], sP[kRtTile][9]; // a chunk of eight (+1 pad)
+    __shared__ int64_t sCand[kRtTile], sNode[kRtTile];
+    const int nc = S->nc;
+    const int c0 = (int)blockIdx.y * kRtTile;
+    if (c0 >= nc || S->nrk < kRtMin)
+        return;
+    const int nrk = S->nrk, np = nrk + 1;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int32_t m = mv.m;
+    const int32_t tail8 = (m / 8) * 8;
+    if (tid < kRtTile)
+        sCand[tid] = S->cand[c0 + tid < nc ? c0 + tid : c0];
+    for (int p0 = (int)blockIdx.x * kRtTile; p0 < np; p0 += (int)gridDim.x * kRtTile) {
+        __syncthreads(); // the previous tile is done with sNode
+        if (tid < kRtTile) {
+            const int pos = p0 + tid;
+            sNode[tid] = pos < np ? (pos == 0 ? 0 : rk[nrk - pos]) : 0;
+        }
+        __syncthreads();
+        float acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                acc[a][b] = 0.0f;
+        for (int32_t k0 = 0; k0 < tail8; k0 += 8) {
+            {
+                // threads 0..127 stage the candidate rows, 128..255 the
+                // position rows: four floats each
+                const int r = (tid & 127) >> 1, h = tid & 1;
+                const int64_t row = tid < 128 ? sCand[r] : sNode[r];
+                const float *src = mv.v + row * m + k0 + 4 * h;
+                float *dst = tid < 128 ? &sC[r][4 * h] : &sP[r][4 * h];
+                if ((m & 3) == 0) {
+                    const float4 q = *(const float4 *)src;
+                    dst[0] = q.x;
+                    dst[1] = q.y;
+                    dst[2] = q.z;
+                    dst[3] = q.w;
+                } else {
+                    dst[0] = src[0];
+                    dst[1] = src[1];
+                    dst[2] = src[2];
+                    dst[3] = src[3];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    float pr[8];
+#pragma unroll
+                    for (int l = 0; l < 8; l++)
+                        pr[l] = sC[4 * ty + a][l] * sP[4 * tx + b][l];
+                    acc[a][b] += reduce8(pr);
+                }
+            __syncthreads();
+        }
+        for (int32_t jt = tail8; jt < m; jt++)
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[a][b] += mv.v[sCand[4 * ty + a] * m + jt] * mv.v[sNode[4 * tx + b] * m + jt];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const int i = c0 + 4 * ty + a;
+            uint64_t best = kBiFar;
+            if (i < nc) {
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int pos = p0 + 4 * tx + b;
+                    if (pos >= np)
+                        continue;
+                    const float ds = dist_sq_ref_dot(mv, sCand[4 * ty + a], sNode[4 * tx + b],
+                                                     acc[a][b]);
+                    if (pos == 0)
+                        S->sd0[i] = ds;
+                    if (ds <= r2) {
+                        const uint64_t key = ((uint64_t)__float_as_uint(ds) << 32) | (uint32_t)pos;
+                        best = key < best ? key : best;
+                    }
+                }
+            }
+            // the 16 lanes of this candidate row (lanes 16 ty' .. 16 ty' + 15)
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) {
+                const uint64_t u = __shfl_xor(best, o, 16);
+                best = u < best ? u : best;
+            }
+            if (tx == 0 && best != kBiFar)
+                atomicMin((unsigned long long *)&S->key[i], (unsigned long long)best);
+        }
+    }
+}
+
 // Levels 9..0 under the residual metric.  d^2 = 1 - |clamp(rho)| lies in
 // [0, 1] <= 4^level for level >= 0, so at these levels every candidate is
 // near, no leader is linked and each candidate's descent depends only on the
-// tree as the batch found it: one workgroup per candidate runs the scans back
-// to back (the same positions, keys and first-minimum choice as
-// bi_scan_kernel).  Once a scan chooses the cover itself, every later scan
+// tree as the batch found it: one workgroup per candidate runs the scans
+// back to back (the same positions, keys and first-minimum choice as
+// bi_scan_kernel), taking level 9 from bi_rootscan_kernel when the root's
+// list is long.  Once a scan chooses the cover itself, every later scan
 // down to level 0 would repeat it (same list, every d^2 within r^2), so they
 // are only counted.  The candidate is left at its level-0 cover with its
 // evaluations in ev_pre and its incremental-scan state for level -1.
@@ -385,7 +500,27 @@ __global__ void __launch_bounds__(kBiParThreads)
     const int64_t q = S->cand[i];
     const int nrk = S->nrk;
     int64_t p = 0, ev = 0;
-    for (int32_t level = 9; level >= 0; --level) {
+    int32_t top = 9;
+    if (nrk >= kRtMin) {
+        // level 9 from bi_rootscan_kernel: a root child, or the root itself
+        const int pos = (int)(uint32_t)S->key[i];
+        ev = 1 + nrk;
+        if (pos == 0) { // every scan down to level 0 keeps the root
+            if (tid == 0) {
+                S->cover[i] = 0;
+                S->key[i] = kBiFar;
+                S->ev_pre[i] = 10 * (int64_t)(1 + nrk);
+                S->stay[i] = 1;
+                S->d0[i] = S->sd0[i];
+                S->plen[i] = nrk;
+                S->phead[i] = -1;
+            }
+            return;
+        }
+        p = rk[nrk - pos];
+        top = 8;
+    }
+    for (int32_t level = top; level >= 0; --level) {
         const float r = ldexpf(1.0f, level), r2 = r * r;
         if (p == 0) {
             uint64_t best = kBiFar;
@@ -1163,6 +1298,9 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
         bi_begin_kernel<<<1, kBiResThreads, 0, st>>>(links, state.ptr, b0, bn);
         int32_t top = 9;
         if constexpr (KIND == kMetricResidual) {
+            const float r9 = ldexpf(1.0f, 9), r9sq = r9 * r9;
+            bi_rootscan_kernel<<<dim3(128, (unsigned)((bn + kRtTile - 1) / kRtTile)), 256, 0,
+                                 st>>>(mv, state.ptr, rk.ptr, r9sq);
             bi_descend_kernel<<<(unsigned)bn, kBiParThreads, 0, st>>>(mv, links, state.ptr,
                                                                      rk.ptr, t.n);
             top = -1;

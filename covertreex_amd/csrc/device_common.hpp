@@ -156,13 +156,14 @@ __device__ __forceinline__ float clamp_rho(float rho)
     return rho > 1.0f ? 1.0f : rho;
 }
 
-// distance_sq_idx_f32 (src/metric.rs:489-563): 1 - |rho|, no sqrt.  Used by
-// the build, the si_cache pass, root seeding and predecessor-failed children.
-__device__ __forceinline__ float dist_sq_ref(const MetricView &mv, int64_t i, int64_t j)
+// dist_sq_ref (below) with the V-row dot already evaluated (the same
+// dot_slice value)
+__device__ __forceinline__ float dist_sq_ref_dot(const MetricView &mv, int64_t i, int64_t j,
+                                                 float dot)
 {
     if (i == j)
         return 0.0f;
-    const int32_t d = mv.d, m = mv.m;
+    const int32_t d = mv.d;
     float dot_scaled = dot_slice(mv.scaled + i * d, mv.scaled + j * d, d);
     float d2 = mv.norms[i] + mv.norms[j] - 2.0f * dot_scaled;
     if (d2 < 0.0f)
@@ -182,13 +183,23 @@ __device__ __forceinline__ float dist_sq_ref(const MetricView &mv, int64_t i, in
     if (!(den2 >= floor_sq))
         den2 = floor_sq;
     float denom = __builtin_sqrtf(den2);
-    // V rows start 16-byte aligned when m % 4 == 0 (allocation base + i*m floats)
-    float dot = (m & 3) == 0 ? dot_slice_v4(mv.v + i * m, mv.v + j * m, m)
-                             : dot_slice(mv.v + i * m, mv.v + j * m, m);
     float rho = denom > 0.0f ? (kval - dot) / denom : 0.0f;
     rho = clamp_rho(rho);
     float ds = 1.0f - __builtin_fabsf(rho);
     return ds > 0.0f ? ds : 0.0f;
+}
+
+// distance_sq_idx_f32 (src/metric.rs:489-563): 1 - |rho|, no sqrt.  Used by
+// the build, the si_cache pass, root seeding and predecessor-failed children.
+__device__ __forceinline__ float dist_sq_ref(const MetricView &mv, int64_t i, int64_t j)
+{
+    if (i == j)
+        return 0.0f;
+    const int32_t m = mv.m;
+    // V rows start 16-byte aligned when m % 4 == 0 (allocation base + i*m floats)
+    const float dot = (m & 3) == 0 ? dot_slice_v4(mv.v + i * m, mv.v + j * m, m)
+                                   : dot_slice(mv.v + i * m, mv.v + j * m, m);
+    return dist_sq_ref_dot(mv, i, j, dot);
 }
 
 // distances_f32_batch_idx_into_with_kth (src/metric.rs:376-487): the
