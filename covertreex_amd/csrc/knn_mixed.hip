@@ -1940,10 +1940,10 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     if (n_over > 0) {
         // The overflowed queries run on a second stream beside the rescore,
         // which skips them (disjoint output rows; everything they read is
-        // ordered by ev).  A few of them get the exact-product screen and
-        // are rescored from their new lists; the ones still overflowing, or
-        // all of them when there are many, are scored exhaustively.  Their
-        // final counts are written after the rescore, on the tree's stream.
+        // ordered by ev).  They get the exact-product screen and are
+        // rescored from their new lists; the ones still overflowing are
+        // scored exhaustively.  Their final counts are written after the
+        // rescore, on the tree's stream.
         if (!t.aux_stream)
             t.aux_stream = acquire_stream(t.device);
         if (!t.join_event)
@@ -1951,10 +1951,8 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
         const hipStream_t hs = t.aux_stream;
         StreamScope hs_scope(hs);
         CX_HIP(hipStreamWaitEvent(hs, ev, 0));
-        const int32_t *exhaustive = hard;
-        const bool rescreen = (int64_t)n_over <= std::max<int64_t>(256, n / 16);
         const unsigned ob = (unsigned)((n_over + 255) / 256);
-        if (rescreen) {
+        {
             rescreen_hard(t, ra, hard, n_over, k, cnt2, list, hs);
             RescoreArgs sub = ra;
             sub.cnt = cnt2;
@@ -1974,12 +1972,11 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
             CX_HIP(hipEventRecord(t.sync_event2, hs));
             CX_HIP(hipEventSynchronize(t.sync_event2));
             n_hard = *t.host_word2;
-            exhaustive = hard2;
         }
         if (n_hard > 0) {
             HardArgs ha{};
             ha.r = ra;
-            ha.hard = exhaustive;
+            ha.hard = hard2;
             ha.n_hard = n_hard;
             ha.nseg = (int32_t)((n + kHSeg - 1) / kHSeg);
             const int64_t query_bytes = (int64_t)ha.nseg * k * sizeof(HardEntry);
@@ -2014,8 +2011,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
         }
         CX_HIP(hipEventRecord(t.join_event, hs));
         CX_HIP(hipStreamWaitEvent(st, t.join_event, 0));
-        if (rescreen)
-            over_commit_kernel<<<ob, 256, 0, st>>>(hard, cnt2, n_over, cnt);
+        over_commit_kernel<<<ob, 256, 0, st>>>(hard, cnt2, n_over, cnt);
     }
     t.last_hard = n_hard;
     if (t.telem)

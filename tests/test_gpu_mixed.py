@@ -124,10 +124,11 @@ def test_c2_shape_sample(hip):
     assert np.all(np.diff(dd, axis=1) >= 0)
 
 
-def test_smooth_field_exhaustive_path(hip):
+def test_smooth_field_rescreened(hip):
     # RBF, l = 1.5 in D = 2 with m = 256: p ~ 1e-7 on nearly every row, so
-    # nearly every query overflows and (too many for the exact-product
-    # screen) is scored against all its candidates
+    # nearly every query overflows the BF16 screen and is listed again by the
+    # exact-product screen (at larger N some lists still overflow, with many
+    # candidates tied at distance 0, and take the exhaustive path)
     from covertreex_amd.native import HipTree
 
     n, k = 8000, 30
@@ -139,10 +140,38 @@ def test_smooth_field_exhaustive_path(hip):
     q = np.arange(n)
     idx, dst, err = t.knn_residual(q, k, predecessor_mode=True, mode="mixed", return_error=True)
     tel = t.last_query_telemetry()
-    assert tel["hard_queries"] > n // 2 and tel["rescreened_queries"] == 0
+    print(f"smooth field: {tel['rescreened_queries']} rescreened, "
+          f"{tel['hard_queries']} scored exhaustively", flush=True)
+    assert tel["rescreened_queries"] > n // 2
     sample = rng.choice(n, 200, replace=False)
     worst = _check(idx[sample], dst[sample], err[sample], x, v, p, sample, k, True, 0, ls=1.5)
     print(f"smooth field max |d - d64| = {worst:.3e}", flush=True)
+    valid = idx >= 0
+    assert np.array_equal(valid.sum(1), np.minimum(k, q))
+
+
+def test_tied_rows_exhaustive_path(hip):
+    # 3,000 copies of one point: a query late among them has more than 2048
+    # predecessors at one and the same distance, which no certified interval
+    # separates, so its list overflows after the exact-product screen too and
+    # it is scored against all its candidates
+    from covertreex_amd.native import HipTree
+
+    n, k = 12000, 20
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((n, 3)).astype(np.float32)
+    x[2000:5000] = x[1999]
+    v, p, *_ = fp64.nystrom(x, inducing_count=64, seed=0, kernel_type=1)
+    t = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
+    t.set_query_telemetry(True)
+    q = np.arange(n)
+    idx, dst, err = t.knn_residual(q, k, predecessor_mode=True, mode="mixed", return_error=True)
+    tel = t.last_query_telemetry()
+    print(f"tied rows: {tel['rescreened_queries']} rescreened, "
+          f"{tel['hard_queries']} scored exhaustively", flush=True)
+    assert tel["hard_queries"] >= 900
+    sample = np.concatenate([np.arange(4900, 5000), rng.choice(n, 100, replace=False)])
+    _check(idx[sample], dst[sample], err[sample], x, v, p, sample, k, True, 1)
     valid = idx >= 0
     assert np.array_equal(valid.sum(1), np.minimum(k, q))
 
