@@ -1269,6 +1269,9 @@ constexpr int kHsTile = 128;           // candidates per tile (four wavefronts x
 constexpr int kHsSeg = 2 * kHsTile;    // candidates per workgroup
 constexpr int kHsRow = 48;             // LDS bytes per staged 32-byte slice row (padded)
 constexpr int64_t kHsBudget = (int64_t)512 << 20; // bytes of stored bounds per batch
+// list counters one 128-byte line apart: each takes one atomic per listed
+// pair, and counters sharing a line would serialize on one L2 channel
+constexpr int kHsCntStride = 32;
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
@@ -1321,7 +1324,7 @@ struct HsArgs {
     int32_t mp;
     float2 *bounds;        // hn x n squared (lower, upper) bounds
     float *tau;            // hn
-    int32_t *hcnt;         // hn
+    int32_t *hcnt;         // hn counters, kHsCntStride ints apart
     int32_t *list;         // kCap slots per sorted query
     int32_t *cnt;          // new count per position of the overflowed-query list
     int64_t *telem;        // query counters or null
@@ -1594,7 +1597,7 @@ __global__ void hs_list_kernel(HsArgs H)
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < lim;
          j += (int64_t)gridDim.x * blockDim.x) {
         if (H.bounds[(int64_t)h * n + j].x <= tau) {
-            const int32_t p = atomicAdd(&H.hcnt[h], 1);
+            const int32_t p = atomicAdd(&H.hcnt[h * kHsCntStride], 1);
             if (p < kCap)
                 dst[p] = (int32_t)j;
         }
@@ -1606,7 +1609,7 @@ __global__ void hs_commit_kernel(HsArgs H)
 {
     const int h = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (h < H.hn)
-        H.cnt[H.h0 + h] = H.hcnt[h];
+        H.cnt[H.h0 + h] = H.hcnt[h * kHsCntStride];
     if (h == 0 && H.telem)
         atomicAdd((unsigned long long *)&H.telem[CX_QT_RESCREENED], (unsigned long long)H.hn);
 }
@@ -1633,7 +1636,7 @@ static void rescreen_hard(TreeState &t, const RescoreArgs &ra, int32_t *hard, in
         (int32_t)std::max<int64_t>(1, std::min<int64_t>(n_hard, kHsBudget / per_query));
     const size_t b_bounds = (size_t)batch * per_query;
     const size_t b_small = (size_t)batch * sizeof(int32_t);
-    const size_t need = b_bounds + 2 * b_small + 1024;
+    const size_t need = b_bounds + b_small * (1 + kHsCntStride) + 1024;
     if (t.hs_buf.count < need) {
         CX_HIP(hipStreamSynchronize(st));
         t.hs_buf.alloc(need);
@@ -1655,7 +1658,7 @@ static void rescreen_hard(TreeState &t, const RescoreArgs &ra, int32_t *hard, in
     for (int32_t h0 = 0; h0 < n_hard; h0 += batch) {
         H.h0 = h0;
         H.hn = std::min(batch, n_hard - h0);
-        CX_HIP(hipMemsetAsync(H.hcnt, 0, (size_t)H.hn * sizeof(int32_t), st));
+        CX_HIP(hipMemsetAsync(H.hcnt, 0, (size_t)H.hn * kHsCntStride * sizeof(int32_t), st));
         // x: candidate segments (up to 2^31 - 1), y: tiles of 32 queries
         const dim3 g1((unsigned)((n + kHsSeg - 1) / kHsSeg), (unsigned)((H.hn + 31) / 32));
         hs_bounds_kernel<<<g1, 256, 0, st>>>(H);
