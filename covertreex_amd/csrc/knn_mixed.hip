@@ -1272,6 +1272,8 @@ constexpr int64_t kHsBudget = (int64_t)512 << 20; // bytes of stored bounds per 
 // list counters one 128-byte line apart: each takes one atomic per listed
 // pair, and counters sharing a line would serialize on one L2 channel
 constexpr int kHsCntStride = 32;
+// overflowed queries above which hs_bounds_kernel prefetches its next K step
+constexpr int32_t kHsPrefetchMin = 1024;
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
@@ -1340,6 +1342,7 @@ __device__ __forceinline__ df df_of_int(int32_t g)
 // Pass 1a.  Workgroup (segment x, query tile y) = 32 overflowed queries (MFMA
 // columns) x a segment of 256 candidates in tiles of 128 (four wavefronts x
 // 32 MFMA rows); stores each valid pair's dot as a double-float.
+template <bool kPf>
 __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
 {
     __shared__ __attribute__((aligned(16))) int8_t sA[kSlices * kHsTile * kHsRow];
@@ -1385,25 +1388,75 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
 #pragma unroll
         for (int t = 0; t < 6; t++)
             G[t] = v16i{};
+        // candidate slices: 128 rows x 6 slices x 2 halves of 16 bytes (six
+        // per thread), query slices: 32 rows x 6 x 2 (one or two per thread).
+        // kPf: the next K step's loads fly while this one is multiplied.  That
+        // pays when many queries overflowed; a few run beside the rescore,
+        // which the form without prefetch slows less.
+        constexpr int kAU = kHsTile * kSlices * 2 / 256, kBU = (32 * kSlices * 2 + 255) / 256;
+        static_assert(kAU * 256 == kHsTile * kSlices * 2, "candidate staging splits evenly");
+        // byte offsets of staged segment i of this thread: global (without
+        // the K offset) and LDS; query segments past the 384 load a valid row
+        // and are not stored
+#define HS_GA(i) ((((c0 + (tid + 256 * (i)) / (kSlices * 2)) < n                                   \
+                        ? c0 + (tid + 256 * (i)) / (kSlices * 2)                                    \
+                        : (int64_t)0) * kSlices + ((tid + 256 * (i)) % (kSlices * 2) >> 1)) * mp +  \
+                  16 * ((tid + 256 * (i)) & 1))
+#define HS_LA(i) ((((tid + 256 * (i)) % (kSlices * 2) >> 1) * kHsTile +                            \
+                   (tid + 256 * (i)) / (kSlices * 2)) * kHsRow + 16 * ((tid + 256 * (i)) & 1))
+#define HS_BU(i) (tid + 256 * (i) < 32 * kSlices * 2 ? tid + 256 * (i) : tid)
+#define HS_GB(i) (((int64_t)sQrow[HS_BU(i) / (kSlices * 2)] * kSlices +                            \
+                   (HS_BU(i) % (kSlices * 2) >> 1)) * mp + 16 * (HS_BU(i) & 1))
+#define HS_LB(i) ((HS_BU(i) % (kSlices * 2) >> 1) * 32 + HS_BU(i) / (kSlices * 2)) * kHsRow +    \
+                 16 * (HS_BU(i) & 1)
+        // named registers, not arrays: a loop-carried array lands in scratch
+        static_assert(kAU == 6 && kBU == 2, "six candidate and two query segments per thread");
+        [[maybe_unused]] uint4 pa0, pa1, pa2, pa3, pa4, pa5, pb0, pb1;
+#define HS_LOAD(o)                                                                                \
+    do {                                                                                          \
+        pa0 = *(const uint4 *)(H.sl + HS_GA(0) + (o));                                            \
+        pa1 = *(const uint4 *)(H.sl + HS_GA(1) + (o));                                            \
+        pa2 = *(const uint4 *)(H.sl + HS_GA(2) + (o));                                            \
+        pa3 = *(const uint4 *)(H.sl + HS_GA(3) + (o));                                            \
+        pa4 = *(const uint4 *)(H.sl + HS_GA(4) + (o));                                            \
+        pa5 = *(const uint4 *)(H.sl + HS_GA(5) + (o));                                            \
+        pb0 = *(const uint4 *)(H.sl + HS_GB(0) + (o));                                            \
+        pb1 = *(const uint4 *)(H.sl + HS_GB(1) + (o));                                            \
+    } while (0)
+        if constexpr (kPf)
+            HS_LOAD(0);
         for (int32_t kc = 0; kc < mp; kc += 32) {
             __syncthreads();
-            // candidate slices: 128 rows x 6 slices x 2 halves of 16 bytes
-            for (int u = tid; u < kHsTile * kSlices * 2; u += 256) {
-                const int row = u / (kSlices * 2), rem = u % (kSlices * 2);
-                const int a = rem >> 1, half = rem & 1;
-                const int64_t j = c0 + row;
-                const int64_t jr = j < n ? j : 0;
-                const uint4 val =
-                    *(const uint4 *)(H.sl + (jr * kSlices + a) * mp + kc + 16 * half);
-                *(uint4 *)(sA + (a * kHsTile + row) * kHsRow + 16 * half) = val;
-            }
-            for (int u = tid; u < 32 * kSlices * 2; u += 256) {
-                const int row = u / (kSlices * 2), rem = u % (kSlices * 2);
-                const int a = rem >> 1, half = rem & 1;
-                const uint4 val =
-                    *(const uint4 *)(H.sl + ((int64_t)sQrow[row] * kSlices + a) * mp + kc +
-                                     16 * half);
-                *(uint4 *)(sB + (a * 32 + row) * kHsRow + 16 * half) = val;
+            if constexpr (kPf) {
+                *(uint4 *)(sA + HS_LA(0)) = pa0;
+                *(uint4 *)(sA + HS_LA(1)) = pa1;
+                *(uint4 *)(sA + HS_LA(2)) = pa2;
+                *(uint4 *)(sA + HS_LA(3)) = pa3;
+                *(uint4 *)(sA + HS_LA(4)) = pa4;
+                *(uint4 *)(sA + HS_LA(5)) = pa5;
+                *(uint4 *)(sB + HS_LB(0)) = pb0; // tid < 256 <= 32 * 12: always a segment
+                if (tid + 256 < 32 * kSlices * 2)
+                    *(uint4 *)(sB + HS_LB(1)) = pb1;
+                if (kc + 32 < mp)
+                    HS_LOAD(kc + 32);
+            } else {
+                for (int u = tid; u < kHsTile * kSlices * 2; u += 256) {
+                    const int row = u / (kSlices * 2), rem = u % (kSlices * 2);
+                    const int a = rem >> 1, half = rem & 1;
+                    const int64_t j = c0 + row;
+                    const int64_t jr = j < n ? j : 0;
+                    const uint4 val =
+                        *(const uint4 *)(H.sl + (jr * kSlices + a) * mp + kc + 16 * half);
+                    *(uint4 *)(sA + (a * kHsTile + row) * kHsRow + 16 * half) = val;
+                }
+                for (int u = tid; u < 32 * kSlices * 2; u += 256) {
+                    const int row = u / (kSlices * 2), rem = u % (kSlices * 2);
+                    const int a = rem >> 1, half = rem & 1;
+                    const uint4 val =
+                        *(const uint4 *)(H.sl + ((int64_t)sQrow[row] * kSlices + a) * mp + kc +
+                                         16 * half);
+                    *(uint4 *)(sB + (a * 32 + row) * kHsRow + 16 * half) = val;
+                }
             }
             __syncthreads();
             // any consistent K order works for A and B: the sums are exact
@@ -1424,6 +1477,12 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
                                                                          G[t - 2], 0, 0, 0);
                 }
         }
+#undef HS_LOAD
+#undef HS_GA
+#undef HS_LA
+#undef HS_BU
+#undef HS_GB
+#undef HS_LB
         // register r of lane l: candidate row 8(r/4) + 4(l/32) + r%4 of the
         // wavefront's 32, query column l%32
 #pragma unroll
@@ -1661,7 +1720,10 @@ static void rescreen_hard(TreeState &t, const RescoreArgs &ra, int32_t *hard, in
         CX_HIP(hipMemsetAsync(H.hcnt, 0, (size_t)H.hn * kHsCntStride * sizeof(int32_t), st));
         // x: candidate segments (up to 2^31 - 1), y: tiles of 32 queries
         const dim3 g1((unsigned)((n + kHsSeg - 1) / kHsSeg), (unsigned)((H.hn + 31) / 32));
-        hs_bounds_kernel<<<g1, 256, 0, st>>>(H);
+        if (n_hard > kHsPrefetchMin)
+            hs_bounds_kernel<true><<<g1, 256, 0, st>>>(H);
+        else
+            hs_bounds_kernel<false><<<g1, 256, 0, st>>>(H);
         const dim3 g2((unsigned)std::min<int64_t>((n + 255) / 256, 64), (unsigned)H.hn);
         hs_interval_kernel<<<g2, 256, 0, st>>>(H);
         hs_tau_kernel<<<(unsigned)H.hn, 256, 0, st>>>(H, k);
