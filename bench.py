@@ -108,6 +108,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=15.0)
+    ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -189,14 +190,14 @@ def main() -> int:
         print(f"bench.py: serving stream, {args.serve_requests} requests at "
               f"{args.serve_rate}/s", file=sys.stderr, flush=True)
         reqs = make_stream(args.serve_requests, seed=0, n_min=2048, n_max=131072)
-        server = Server(max_inflight=4, hbm_budget_bytes=64 << 30)
+        server = Server(max_inflight=args.serve_inflight, hbm_budget_bytes=64 << 30)
         try:
             res = run_poisson(server, reqs, rate_per_s=args.serve_rate, seed=0)
         finally:
             server.close()
         serve = {k_: (round(v_, 3) if isinstance(v_, float) else v_)
                  for k_, v_ in latency_summary(res).items()}
-        serve.update({"rate_per_s": args.serve_rate, "max_inflight": 4,
+        serve.update({"rate_per_s": args.serve_rate, "max_inflight": args.serve_inflight,
                       "peak_inflight": server.peak_inflight,
                       "workload": "Poisson arrivals; N log-uniform [2048, 131072], D in {2,3,5}, "
                                   "Matern/RBF l~U[0.5,2], m in {64,128,256}, k in {10,30,50}, "
