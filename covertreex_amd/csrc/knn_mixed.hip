@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "cx_internal.hpp"
@@ -1025,8 +1026,10 @@ __device__ __forceinline__ void score_query(const RescoreArgs &A, int64_t qt, in
 // One wavefront per query (from its XCD's chunk of sorted queries) scores its
 // list.  Queries whose list overflowed kCap are left to the hard kernels.
 
+// four wavefronts per SIMD for one kept entry per lane (k <= 64): the row
+// gathers are latency-bound; three for the wider top-k
 template <int KB>
-__global__ void __launch_bounds__(kRsThreads, 3) rescore_kernel(RescoreArgs A)
+__global__ void __launch_bounds__(kRsThreads, KB == 1 ? 4 : 3) rescore_kernel(RescoreArgs A)
 {
     __shared__ __attribute__((aligned(16))) float s_vq[kWaves][kStageM];
     __shared__ float s_dh[kWaves][64], s_dl[kWaves][64];
@@ -1551,18 +1554,16 @@ __global__ void __launch_bounds__(256) hs_interval_kernel(HsArgs H)
 // exact in FP32 for v in bin b, and tau = (1024 b1 + b2 + 1) 2^-19 is
 // representable, so every bound counted at or below the chosen sub-bin is
 // <= tau.  +inf when the query has fewer than k pairs.
+constexpr int kHsTauThreads = 1024; // one bin per thread in hs_find
+
 __device__ __forceinline__ int hs_find(const int32_t *hist, int32_t need, int32_t *below, int *s_w,
                                        int32_t *s_tot)
 {
     // first bin where the running count reaches need; *below = count before it
+    static_assert(kHsBins == kHsTauThreads, "one bin per thread");
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    int32_t c[4], sum = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        c[i] = hist[4 * tid + i];
-        sum += c[i];
-    }
-    int32_t incl = sum;
+    const int32_t c = hist[tid];
+    int32_t incl = c;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const int32_t u = __shfl_up(incl, o, 64);
@@ -1576,20 +1577,13 @@ __device__ __forceinline__ int hs_find(const int32_t *hist, int32_t need, int32_
     for (int w = 0; w < wid; w++)
         off += s_tot[w];
     incl += off;
-    const int32_t excl = incl - sum;
+    const int32_t excl = incl - c;
     if (tid == 0)
         *s_w = kHsBins;
     __syncthreads();
     if (excl < need && incl >= need) {
-        int32_t run = excl;
-        for (int i = 0; i < 4; i++) {
-            run += c[i];
-            if (run >= need) {
-                *s_w = 4 * tid + i;
-                *below = run - c[i];
-                break;
-            }
-        }
+        *s_w = tid;
+        *below = excl;
     }
     __syncthreads();
     const int r = *s_w;
@@ -1597,20 +1591,20 @@ __device__ __forceinline__ int hs_find(const int32_t *hist, int32_t need, int32_
     return r;
 }
 
-__global__ void __launch_bounds__(256) hs_tau_kernel(HsArgs H, int32_t k)
+__global__ void __launch_bounds__(kHsTauThreads) hs_tau_kernel(HsArgs H, int32_t k)
 {
     __shared__ int32_t s_hist[kHsBins];
-    __shared__ int32_t s_tot[4], s_below;
+    __shared__ int32_t s_tot[kHsTauThreads / 64], s_below;
     __shared__ int s_bin;
     const RescoreArgs &A = H.r;
     const int h = (int)blockIdx.x, tid = threadIdx.x;
     const int64_t n = A.n;
     const int64_t lim = hard_limit(A, H.hard[H.h0 + h]);
     const float2 *bd = H.bounds + (int64_t)h * n;
-    for (int i = tid; i < kHsBins; i += 256)
+    for (int i = tid; i < kHsBins; i += kHsTauThreads)
         s_hist[i] = 0;
     __syncthreads();
-    for (int64_t j = tid; j < lim; j += 256) {
+    for (int64_t j = tid; j < lim; j += kHsTauThreads) {
         const float v = bd[j].y;
         const int b = (v >= 0.0f && v < 2.0f) ? (int)(v * 512.0f) : kHsBins - 1;
         atomicAdd(&s_hist[b], 1);
@@ -1626,10 +1620,10 @@ __global__ void __launch_bounds__(256) hs_tau_kernel(HsArgs H, int32_t k)
     }
     const int32_t below1 = s_below;
     const float lo1 = (float)b1 * (1.0f / 512.0f);
-    for (int i = tid; i < kHsBins; i += 256)
+    for (int i = tid; i < kHsBins; i += kHsTauThreads)
         s_hist[i] = 0;
     __syncthreads();
-    for (int64_t j = tid; j < lim; j += 256) {
+    for (int64_t j = tid; j < lim; j += kHsTauThreads) {
         const float v = bd[j].y;
         if (v >= lo1 && v < lo1 + (1.0f / 512.0f)) {
             const int b = min((int)((v - lo1) * 524288.0f), kHsBins - 1);
@@ -1675,8 +1669,11 @@ __global__ void hs_commit_kernel(HsArgs H)
 
 // The overflowed queries hard[0 .. n_hard) get the exact-product screen;
 // the ones whose new list fits kCap become listed queries (cnt rewritten).
+// after_first runs on the host once the first batch's hs_bounds_kernel is
+// queued (the caller queues work of its own there).
 static void rescreen_hard(TreeState &t, const RescoreArgs &ra, int32_t *hard, int32_t n_hard,
-                          int32_t k, int32_t *cnt, int32_t *list, hipStream_t st)
+                          int32_t k, int32_t *cnt, int32_t *list, hipStream_t st,
+                          const std::function<void()> &after_first)
 {
     const int64_t n = t.n;
     if (!t.slices_ready) {
@@ -1724,9 +1721,11 @@ static void rescreen_hard(TreeState &t, const RescoreArgs &ra, int32_t *hard, in
             hs_bounds_kernel<true><<<g1, 256, 0, st>>>(H);
         else
             hs_bounds_kernel<false><<<g1, 256, 0, st>>>(H);
+        if (h0 == 0)
+            after_first();
         const dim3 g2((unsigned)std::min<int64_t>((n + 255) / 256, 64), (unsigned)H.hn);
         hs_interval_kernel<<<g2, 256, 0, st>>>(H);
-        hs_tau_kernel<<<(unsigned)H.hn, 256, 0, st>>>(H, k);
+        hs_tau_kernel<<<(unsigned)H.hn, kHsTauThreads, 0, st>>>(H, k);
         hs_list_kernel<<<g2, 256, 0, st>>>(H);
         hs_commit_kernel<<<(unsigned)((H.hn + 255) / 256), 256, 0, st>>>(H);
         CX_HIP(hipGetLastError());
@@ -1997,12 +1996,13 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
             rescore_kernel<7><<<grid, kRsThreads, 0, sm>>>(r);
         CX_HIP(hipGetLastError());
     };
-    launch_rescore(ra, nq, st);
-    prof_mark(t, 6);
     CX_HIP(hipEventSynchronize(ev));
     const int32_t n_over = *t.host_word;
     int32_t n_hard = n_over;
-    if (n_over > 0) {
+    if (n_over == 0) {
+        launch_rescore(ra, nq, st);
+        prof_mark(t, 6);
+    } else {
         // The overflowed queries run on a second stream beside the rescore,
         // which skips them (disjoint output rows; everything they read is
         // ordered by ev).  They get the exact-product screen and are
@@ -2018,7 +2018,13 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
         CX_HIP(hipStreamWaitEvent(hs, ev, 0));
         const unsigned ob = (unsigned)((n_over + 255) / 256);
         {
-            rescreen_hard(t, ra, hard, n_over, k, cnt2, list, hs);
+            // the main rescore is queued right after the first exact-product
+            // screen kernel: that kernel's workgroups reach the CUs first,
+            // and the rest of this chain is queued while both run
+            rescreen_hard(t, ra, hard, n_over, k, cnt2, list, hs, [&] {
+                launch_rescore(ra, nq, st);
+                prof_mark(t, 6);
+            });
             RescoreArgs sub = ra;
             sub.cnt = cnt2;
             sub.qlist = hard;
