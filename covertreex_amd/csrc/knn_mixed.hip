@@ -1399,8 +1399,9 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
         constexpr int kAU = kHsTile * kSlices * 2 / 256, kBU = (32 * kSlices * 2 + 255) / 256;
         static_assert(kAU * 256 == kHsTile * kSlices * 2, "candidate staging splits evenly");
         // byte offsets of staged segment i of this thread: global (without
-        // the K offset) and LDS; query segments past the 384 load a valid row
-        // and are not stored
+        // the K offset) and LDS; the query slices are 32 x 12 = 384 segments,
+        // so a thread's second query segment past them reloads a valid one
+        // and is not stored
 #define HS_GA(i) ((((c0 + (tid + 256 * (i)) / (kSlices * 2)) < n                                   \
                         ? c0 + (tid + 256 * (i)) / (kSlices * 2)                                    \
                         : (int64_t)0) * kSlices + ((tid + 256 * (i)) % (kSlices * 2) >> 1)) * mp +  \
@@ -1410,8 +1411,8 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
 #define HS_BU(i) (tid + 256 * (i) < 32 * kSlices * 2 ? tid + 256 * (i) : tid)
 #define HS_GB(i) (((int64_t)sQrow[HS_BU(i) / (kSlices * 2)] * kSlices +                            \
                    (HS_BU(i) % (kSlices * 2) >> 1)) * mp + 16 * (HS_BU(i) & 1))
-#define HS_LB(i) ((HS_BU(i) % (kSlices * 2) >> 1) * 32 + HS_BU(i) / (kSlices * 2)) * kHsRow +    \
-                 16 * (HS_BU(i) & 1)
+#define HS_LB(i) (((HS_BU(i) % (kSlices * 2) >> 1) * 32 + HS_BU(i) / (kSlices * 2)) * kHsRow +   \
+                  16 * (HS_BU(i) & 1))
         // named registers, not arrays: a loop-carried array lands in scratch
         static_assert(kAU == 6 && kBU == 2, "six candidate and two query segments per thread");
         [[maybe_unused]] uint4 pa0, pa1, pa2, pa3, pa4, pa5, pb0, pb1;
