@@ -46,6 +46,7 @@ TreeState::~TreeState()
     vsl.release();
     vexp.release();
     hs_buf.release();
+    list_big.release();
     work.release();
     telem_buf.release();
     if (host_word)

@@ -227,6 +227,7 @@ struct TreeState {
     DevBuf<int32_t> vexp;
     bool slices_ready = false;
     DevBuf<uint8_t> hs_buf;
+    DevBuf<int32_t> list_big; // second listing of still-overflowing queries
     // second stream of the mixed kNN (overflowed queries beside the rescore)
     hipStream_t aux_stream = nullptr;
     hipEvent_t join_event = nullptr;

@@ -46,6 +46,9 @@ constexpr int kKP = kKC + 8; // padded LDS row (bf16)
 constexpr int kBins = 128;   // histogram bins of the squared upper bound over [0, 1)
 constexpr float kHistCap = 0.99f; // squared bounds at or above this are not counted
 constexpr int kCap = 2048;   // listed candidates per query before overflow
+// second listing of queries still overflowing after the exact-product screen
+constexpr int kCapBig = 8192;
+constexpr int64_t kBigListBudget = (int64_t)1 << 30; // bytes of those lists per call
 constexpr int kMaxD = 16;
 constexpr int kMixMaxK = 256;
 constexpr int64_t kSeg = 16 * kCT; // candidates per workgroup (grid y)
@@ -596,9 +599,11 @@ struct RescoreArgs {
     const int64_t *qperm; // sorted position -> original query position
     const int32_t *qrow;
     const int32_t *cnt;   // listed candidates per sorted query (per qlist entry if set)
-    const int32_t *list;  // kCap slots per sorted query
+    const int32_t *list;  // cap slots per sorted query (per qlist entry if list_by_ix)
     const int32_t *qlist; // optional: rescore only these sorted positions
     int64_t nlist;
+    int32_t cap;          // list slots per query; a larger count is not listed
+    int32_t list_by_ix;
     const float *vnorm;   // |V_i| rounded up, dataset order
     int64_t n, nq;
     int32_t d, m, kernel, pred, k;
@@ -1047,8 +1052,8 @@ __global__ void __launch_bounds__(kRsThreads, KB == 1 ? 4 : 3) rescore_kernel(Re
             return;
         const int64_t s = A.qlist ? (int64_t)A.qlist[ix] : ix;
         const int32_t count = A.cnt[ix];
-        if (count > kCap)
-            return; // scored over all its candidates by the hard kernels
+        if (count > A.cap)
+            return; // listed again with more slots, or scored by the hard kernels
         const int32_t qrow = A.qrow[s];
         const int64_t qt = A.inv_order[qrow];
         wave_sync();
@@ -1056,7 +1061,8 @@ __global__ void __launch_bounds__(kRsThreads, KB == 1 ? 4 : 3) rescore_kernel(Re
         wave_sync();
         WaveTopK<KB> top;
         top.init();
-        score_query<KB>(A, qt, qrow, A.vnorm[qrow], A.list + s * kCap, 0, count, A.n, s_vq[w],
+        score_query<KB>(A, qt, qrow, A.vnorm[qrow], A.list + (A.list_by_ix ? ix : s) * A.cap, 0,
+                        count, A.n, s_vq[w],
                         s_dh[w], s_dl[w], top, lane);
         const int64_t pos = A.qperm[s];
         top.emit(k, lane, [&](int r, const Ent &e) { put_output(A, pos, r, e); });
@@ -1174,12 +1180,13 @@ __global__ void hard_scatter_kernel(const int32_t *__restrict__ flag,
 }
 
 // ordered compaction over the overflowed-query list: the queries whose new
-// list still overflows (cnt2 > kCap) go to dst; the count to n_out
-__global__ void over_flag_kernel(const int32_t *__restrict__ cnt2, int64_t n, int32_t *flag)
+// list still overflows (cnt2 > cap) go to dst; the count to n_out
+__global__ void over_flag_kernel(const int32_t *__restrict__ cnt2, int64_t n, int32_t cap,
+                                 int32_t *flag)
 {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h < n)
-        flag[h] = cnt2[h] > kCap ? 1 : 0;
+        flag[h] = cnt2[h] > cap ? 1 : 0;
 }
 
 __global__ void over_scatter_kernel(const int32_t *__restrict__ flag,
@@ -1210,7 +1217,7 @@ __global__ void over_commit_kernel(const int32_t *__restrict__ hard,
 // pairs, the longest list.
 __global__ void mixed_telemetry_kernel(const int64_t *__restrict__ qsorted,
                                        const int32_t *__restrict__ cnt, int64_t nq, int64_t n,
-                                       int32_t pred, int64_t *telem)
+                                       int32_t pred, int32_t hard_above, int64_t *telem)
 {
     __shared__ unsigned long long s_acc[5];
     __shared__ int s_max;
@@ -1224,7 +1231,7 @@ __global__ void mixed_telemetry_kernel(const int64_t *__restrict__ qsorted,
         const int64_t qid = qsorted[s];
         const int64_t lim = pred ? (qid < 0 ? n : (qid < n ? qid : n)) : n;
         const int32_t c = cnt[s];
-        const bool hard = c > kCap;
+        const bool hard = c > hard_above; // scored over all its candidates
         atomicAdd(&s_acc[0], (unsigned long long)lim);
         atomicAdd(&s_acc[1], hard ? 0ull : (unsigned long long)c);
         atomicAdd(&s_acc[2], hard ? 1ull : 0ull);
@@ -1330,7 +1337,8 @@ struct HsArgs {
     float2 *bounds;        // hn x n squared (lower, upper) bounds
     float *tau;            // hn
     int32_t *hcnt;         // hn counters, kHsCntStride ints apart
-    int32_t *list;         // kCap slots per sorted query
+    int32_t *list;         // cap slots per sorted query, or per position (list_by_pos)
+    int32_t cap, list_by_pos;
     int32_t *cnt;          // new count per position of the overflowed-query list
     int64_t *telem;        // query counters or null
 };
@@ -1637,7 +1645,7 @@ __global__ void __launch_bounds__(kHsTauThreads) hs_tau_kernel(HsArgs H, int32_t
         H.tau[h] = (float)(1024 * b1 + min(b2, kHsBins - 1) + 1) * (1.0f / 524288.0f);
 }
 
-// Pass 2: list every valid pair with lower bound <= tau (slots past kCap are
+// Pass 2: list every valid pair with lower bound <= tau (slots past cap are
 // counted, not written).
 __global__ void hs_list_kernel(HsArgs H)
 {
@@ -1647,12 +1655,12 @@ __global__ void hs_list_kernel(HsArgs H)
     const int64_t s = H.hard[H.h0 + h];
     const int64_t lim = hard_limit(A, s);
     const float tau = H.tau[h];
-    int32_t *dst = H.list + s * kCap;
+    int32_t *dst = H.list + (H.list_by_pos ? (int64_t)(H.h0 + h) : s) * H.cap;
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < lim;
          j += (int64_t)gridDim.x * blockDim.x) {
         if (H.bounds[(int64_t)h * n + j].x <= tau) {
             const int32_t p = atomicAdd(&H.hcnt[h * kHsCntStride], 1);
-            if (p < kCap)
+            if (p < H.cap)
                 dst[p] = (int32_t)j;
         }
     }
@@ -1670,10 +1678,13 @@ __global__ void hs_commit_kernel(HsArgs H)
 
 // The overflowed queries hard[0 .. n_hard) get the exact-product screen;
 // the ones whose new list fits kCap become listed queries (cnt rewritten).
+// Lists of cap slots, per sorted query or (list_by_pos) per position of
+// hard; the second form lists queries already counted as rescreened.
 // after_first runs on the host once the first batch's hs_bounds_kernel is
 // queued (the caller queues work of its own there).
 static void rescreen_hard(TreeState &t, const RescoreArgs &ra, int32_t *hard, int32_t n_hard,
-                          int32_t k, int32_t *cnt, int32_t *list, hipStream_t st,
+                          int32_t k, int32_t *cnt, int32_t *list, int32_t cap,
+                          int32_t list_by_pos, hipStream_t st,
                           const std::function<void()> &after_first)
 {
     const int64_t n = t.n;
@@ -1711,7 +1722,9 @@ static void rescreen_hard(TreeState &t, const RescoreArgs &ra, int32_t *hard, in
     H.hcnt = (int32_t *)(base + b_bounds + b_small);
     H.list = list;
     H.cnt = cnt;
-    H.telem = t.telem;
+    H.cap = cap;
+    H.list_by_pos = list_by_pos;
+    H.telem = list_by_pos ? nullptr : t.telem;
     for (int32_t h0 = 0; h0 < n_hard; h0 += batch) {
         H.h0 = h0;
         H.hn = std::min(batch, n_hard - h0);
@@ -1833,6 +1846,8 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     const size_t o_hard = take(sizeof(int32_t) * nq);
     const size_t o_hard2 = take(sizeof(int32_t) * nq);
     const size_t o_cnt2 = take(sizeof(int32_t) * nq);
+    const size_t o_hard3 = take(sizeof(int32_t) * nq);
+    const size_t o_cnt3 = take(sizeof(int32_t) * nq);
     size_t sort_bytes = 0, scan_bytes = 0;
     CX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int64_t *)nullptr,
                                               (int64_t *)nullptr, (const int64_t *)nullptr,
@@ -1852,8 +1867,11 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     // overflowed queries still overflowing after the exact-product screen,
     // and the new counts per position of the overflowed-query list
     int32_t *hard2 = (int32_t *)(ws + o_hard2), *cnt2 = (int32_t *)(ws + o_cnt2);
-    // ctr[0]: the overflowed-query count; ctr[1]: those still overflowing
-    int32_t *n_hard_d = ctr, *n_hard2_d = ctr + 1;
+    // the same after the second listing (counts per position of hard2)
+    int32_t *hard3 = (int32_t *)(ws + o_hard3), *cnt3 = (int32_t *)(ws + o_cnt3);
+    // ctr[0]: the overflowed-query count; ctr[1]: those still overflowing;
+    // ctr[2]: those still overflowing the second listing
+    int32_t *n_hard_d = ctr, *n_hard2_d = ctr + 1, *n_hard3_d = ctr + 2;
 
     const unsigned qb = (unsigned)((nq + 255) / 256);
     iota64_kernel<<<qb, 256, 0, st>>>(iota, nq);
@@ -1951,6 +1969,8 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     ra.qrow = qrow;
     ra.cnt = cnt;
     ra.list = list;
+    ra.cap = kCap;
+    ra.list_by_ix = 0;
     ra.n = n;
     ra.nq = nq;
     ra.d = t.d;
@@ -1999,7 +2019,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     };
     CX_HIP(hipEventSynchronize(ev));
     const int32_t n_over = *t.host_word;
-    int32_t n_hard = n_over;
+    int32_t n_hard = n_over, n_big = 0;
     if (n_over == 0) {
         launch_rescore(ra, nq, st);
         prof_mark(t, 6);
@@ -2022,7 +2042,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
             // the main rescore is queued right after the first exact-product
             // screen kernel: that kernel's workgroups reach the CUs first,
             // and the rest of this chain is queued while both run
-            rescreen_hard(t, ra, hard, n_over, k, cnt2, list, hs, [&] {
+            rescreen_hard(t, ra, hard, n_over, k, cnt2, list, kCap, 0, hs, [&] {
                 launch_rescore(ra, nq, st);
                 prof_mark(t, 6);
             });
@@ -2031,7 +2051,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
             sub.qlist = hard;
             sub.nlist = n_over;
             launch_rescore(sub, n_over, hs);
-            over_flag_kernel<<<ob, 256, 0, hs>>>(cnt2, n_over, flag);
+            over_flag_kernel<<<ob, 256, 0, hs>>>(cnt2, n_over, kCap, flag);
             CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan_bytes, flag, pos,
                                                     (int)n_over, hs));
             over_scatter_kernel<<<ob, 256, 0, hs>>>(flag, pos, n_over, hard, hard2, n_hard2_d);
@@ -2045,10 +2065,42 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
             CX_HIP(hipEventSynchronize(t.sync_event2));
             n_hard = *t.host_word2;
         }
+        // the queries still overflowing are listed again into kCapBig slots
+        // each (by position) and rescored from there; beyond those, or
+        // when the lists would not fit kBigListBudget, exhaustively
+        const int32_t *exhaustive = hard2;
+        if (n_hard > 0 && (int64_t)n_hard * kCapBig * (int64_t)sizeof(int32_t) <= kBigListBudget) {
+            n_big = n_hard;
+            const size_t need = (size_t)n_big * kCapBig;
+            if (t.list_big.count < need) {
+                CX_HIP(hipStreamSynchronize(hs));
+                t.list_big.alloc(need);
+            }
+            rescreen_hard(t, ra, hard2, n_big, k, cnt3, t.list_big.ptr, kCapBig, 1, hs, [] {});
+            RescoreArgs big = ra;
+            big.cnt = cnt3;
+            big.qlist = hard2;
+            big.nlist = n_big;
+            big.list = t.list_big.ptr;
+            big.cap = kCapBig;
+            big.list_by_ix = 1;
+            launch_rescore(big, n_big, hs);
+            const unsigned bb = (unsigned)((n_big + 255) / 256);
+            over_flag_kernel<<<bb, 256, 0, hs>>>(cnt3, n_big, kCapBig, flag);
+            CX_HIP(hipcub::DeviceScan::ExclusiveSum(ws + o_tmp, scan_bytes, flag, pos,
+                                                    (int)n_big, hs));
+            over_scatter_kernel<<<bb, 256, 0, hs>>>(flag, pos, n_big, hard2, hard3, n_hard3_d);
+            CX_HIP(hipMemcpyAsync(t.host_word2, n_hard3_d, sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, hs));
+            CX_HIP(hipEventRecord(t.sync_event2, hs));
+            CX_HIP(hipEventSynchronize(t.sync_event2));
+            n_hard = *t.host_word2;
+            exhaustive = hard3;
+        }
         if (n_hard > 0) {
             HardArgs ha{};
             ha.r = ra;
-            ha.hard = hard2;
+            ha.hard = exhaustive;
             ha.n_hard = n_hard;
             ha.nseg = (int32_t)((n + kHSeg - 1) / kHSeg);
             const int64_t query_bytes = (int64_t)ha.nseg * k * sizeof(HardEntry);
@@ -2084,10 +2136,14 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
         CX_HIP(hipEventRecord(t.join_event, hs));
         CX_HIP(hipStreamWaitEvent(st, t.join_event, 0));
         over_commit_kernel<<<ob, 256, 0, st>>>(hard, cnt2, n_over, cnt);
+        if (n_big > 0)
+            over_commit_kernel<<<(unsigned)((n_big + 255) / 256), 256, 0, st>>>(hard2, cnt3, n_big,
+                                                                               cnt);
     }
     t.last_hard = n_hard;
     if (t.telem)
-        mixed_telemetry_kernel<<<qb, 256, 0, st>>>(qs, cnt, nq, n, predecessor ? 1 : 0, t.telem);
+        mixed_telemetry_kernel<<<qb, 256, 0, st>>>(qs, cnt, nq, n, predecessor ? 1 : 0,
+                                                   n_big > 0 ? kCapBig : kCap, t.telem);
     prof_mark(t, 7);
 }
 

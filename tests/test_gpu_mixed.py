@@ -151,16 +151,17 @@ def test_smooth_field_rescreened(hip):
 
 
 def test_tied_rows_exhaustive_path(hip):
-    # 3,000 copies of one point: a query late among them has more than 2048
-    # predecessors at one and the same distance, which no certified interval
-    # separates, so its list overflows after the exact-product screen too and
-    # it is scored against all its candidates
+    # 10,000 copies of one point: a query among them has as many predecessors
+    # at one and the same distance, which no certified interval separates.
+    # Between 2,048 and 8,192 of them the exact-product screen lists the
+    # query again with 8,192 slots; beyond that it is scored against all its
+    # candidates
     from covertreex_amd.native import HipTree
 
-    n, k = 12000, 20
+    n, k = 24000, 20
     rng = np.random.default_rng(8)
     x = rng.standard_normal((n, 3)).astype(np.float32)
-    x[2000:5000] = x[1999]
+    x[2000:12000] = x[1999]
     v, p, *_ = fp64.nystrom(x, inducing_count=64, seed=0, kernel_type=1)
     t = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
     t.set_query_telemetry(True)
@@ -169,8 +170,10 @@ def test_tied_rows_exhaustive_path(hip):
     tel = t.last_query_telemetry()
     print(f"tied rows: {tel['rescreened_queries']} rescreened, "
           f"{tel['hard_queries']} scored exhaustively", flush=True)
-    assert tel["hard_queries"] >= 900
-    sample = np.concatenate([np.arange(4900, 5000), rng.choice(n, 100, replace=False)])
+    # queries 10,192 .. 11,999 have more than 8,192 tied predecessors
+    assert 1700 <= tel["hard_queries"] <= 1900
+    sample = np.concatenate([np.arange(11900, 12000), np.arange(7000, 7100),
+                             rng.choice(n, 100, replace=False)])
     _check(idx[sample], dst[sample], err[sample], x, v, p, sample, k, True, 1)
     valid = idx >= 0
     assert np.array_equal(valid.sum(1), np.minimum(k, q))
