@@ -48,13 +48,19 @@ class Request:
 
     def footprint_bytes(self) -> int:
         """HBM held while the request runs: V (f32, tree order) + BF16 hi/lo
-        copy + coords/p/tree arrays + per-query lists and outputs."""
+        copy + coords/p/tree arrays + per-query lists and outputs, plus upper
+        bounds of the mixed kNN's work buffers: one-byte screen codes (at
+        most 4 GiB), the exact-product screen's int8 slices and stored bounds
+        (512 MiB), second-listing lists (at most 1 GiB, 32 KiB per query) and
+        exhaustive-path partial lists (512 MiB)."""
         n, m, d, k = self.n, self.m, self.coords.shape[1], self.k
         mp = (m + 31) // 32 * 32
         tree = n * (4 * m + 4 * mp + 12 * d + 120)
         query = n * (2048 * 4 + 64 * 4 + 60 + 12 * k)
         nystrom = n * (8 * m + 8 * d) + 8 * m * m
-        return int(tree + query + nystrom)
+        work = (min(4 << 30, n * n) + n * 6 * mp + (512 << 20) + min(1 << 30, n * 32768)
+                + (512 << 20))
+        return int(tree + query + nystrom + work)
 
 
 @dataclass
