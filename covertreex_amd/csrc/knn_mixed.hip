@@ -2028,8 +2028,9 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
         // which skips them (disjoint output rows; everything they read is
         // ordered by ev).  They get the exact-product screen and are
         // rescored from their new lists; the ones still overflowing are
-        // scored exhaustively.  Their final counts are written after the
-        // rescore, on the tree's stream.
+        // listed again with more slots, and only those past these are scored
+        // exhaustively.  Their final counts are written after the rescore,
+        // on the tree's stream.
         if (!t.aux_stream)
             t.aux_stream = acquire_stream(t.device);
         if (!t.join_event)
