@@ -322,6 +322,8 @@ __global__ void __launch_bounds__(kScreenThreads, 3) screen_kernel(ScreenArgs A)
                 v4 = *(const uint4 *)(vh + rq * 2u + o);
                 v5 = *(const uint4 *)(vl + rq * 2u + o);
             }
+            // MFMA issue ahead of other wavefronts' epilogues on this SIMD
+            __builtin_amdgcn_s_setprio(1);
 #pragma unroll
             for (int ks = 0; ks < kKC; ks += 16) {
                 const int koff = ks + 8 * (lane >> 5);
@@ -340,6 +342,7 @@ __global__ void __launch_bounds__(kScreenThreads, 3) screen_kernel(ScreenArgs A)
                     acc_x[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc_x[b], 0, 0, 0);
                 }
             }
+            __builtin_amdgcn_s_setprio(0);
         }
         // epilogue: output register r of block b is candidate row
         // rbase + 8*(r/4) + r%4, query column qcol[b]
