@@ -164,6 +164,11 @@ def main() -> int:
     elapsed = time.perf_counter() - t_start
     qps = N * args.steps / elapsed
 
+    # largest certified error bound over every returned distance of the last
+    # timed call (the bound holds against the FP64 evaluation on the same inputs)
+    valid = o_idx >= 0
+    max_err_bound = float(o_err[valid].max().item()) if bool(valid.any()) else 0.0
+
     # per-phase device timing on the tree's own stream (HIP events)
     tree.set_profiling(True)
     prof = []
@@ -233,6 +238,7 @@ def main() -> int:
                    "n": N, "d": D, "m": M, "k": K, "predecessor": True},
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "max_err_bound": max_err_bound,
         "plain_f32_gpu_qps": round(plain_qps, 1),
         "speedup_vs_plain_f32_gpu": round(qps / plain_qps, 2),
         "build_s": round(build_s, 3),

@@ -412,7 +412,9 @@ __global__ void __launch_bounds__(kScreenThreads, 3) screen_kernel(ScreenArgs A)
                     // computed bound or not at all, tau stays an upper bound
                     const float rlo = __builtin_fmaf(ar, 1.0f - 7.2e-7f, -wE);
                     if (valid && rlo > thr1) {
-                        const float ub2 = (1.0f - rlo) + 1.5e-7f;
+                        // rlo can exceed 1 when var or p do not match V
+                        // (|rho| is clamped to 1 there): bin such pairs at 0
+                        const float ub2 = __builtin_fmaxf((1.0f - rlo) + 1.5e-7f, 0.0f);
                         if (ub2 < kHistCap) {
                             const int bin = (int)(ub2 * (float)kBins);
                             atomicAdd(&sHist[(qcol[b] * kBins + bin) >> 1],
@@ -683,8 +685,11 @@ __device__ __forceinline__ df finish_distance_at(const RescoreArgs &A, const flo
     const df kv = kernel_value_df(A, xq, xj);
     const df num = df_sub(kv, dot);
     df pp = two_prod(pq, pj);
-    if (pp.hi < 1e-18f)
-        pp = {1e-18f, 0.0f};
+    // floor max(p_q p_j, 1e-18) with 1e-18 as a double-float pair (the
+    // FP32 literal alone is 1.0000000458e-18); hi + lo is within 8e-34 of it
+    constexpr float kFloorHi = 0x1.2725dep-60f, kFloorLo = -0x1.c5b78ap-85f;
+    if (pp.hi < kFloorHi || (pp.hi == kFloorHi && pp.lo < kFloorLo))
+        pp = {kFloorHi, kFloorLo};
     const df den = df_sqrt(pp);
     df rho = df_div(num, den);
     if (rho.hi < 0.0f)

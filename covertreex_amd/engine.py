@@ -21,6 +21,9 @@ from .residual import ResidualBackend, build_residual_backend
 
 DEFAULT_ENGINE = "hip"
 SUPPORTED_ENGINES = ("hip",)
+# kNN limits of the device kernels (the reference has none)
+MAX_K = 256
+MAX_D_MIXED = 16
 
 
 @dataclass(frozen=True)
@@ -173,6 +176,13 @@ class HipEngine:
         q = np.asarray(queries, dtype=np.int64).reshape(-1)
         precision = (runtime.knn_precision if runtime is not None
                      else tree.meta.get("precision", "mixed"))
+        # the device kernels keep a query's top-k in one wavefront's registers
+        # (k <= 256) and the screen's coordinates in fixed arrays (d <= 16)
+        if not 1 <= int(k) <= MAX_K:
+            raise ValueError(f"hip engine supports 1 <= k <= {MAX_K}; got k={k}")
+        if precision == "mixed" and handle.tree.info()["d"] > MAX_D_MIXED:
+            raise ValueError(f"hip engine's mixed precision supports d <= {MAX_D_MIXED}; "
+                             f"use precision='reference-f32' for d={handle.tree.info()['d']}")
         telemetry = _query_telemetry_enabled()
         handle.tree.set_query_telemetry(telemetry)
         idx, dist = handle.tree.knn_residual(q, int(k), predecessor_mode=predecessor_mode,

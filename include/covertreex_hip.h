@@ -84,13 +84,18 @@ cx_status cx_tree_build_residual(const float *V, const float *p, const float *co
  * nq x k certified bound on |d - d_exact| for CX_KNN_MIXED (the f32 error
  * of the reference arithmetic is not bounded: NaN in CX_KNN_REFERENCE_F32).
  * Out-of-range query indices map to node 0 as the reference does
- * (src/lib.rs:414-418). */
+ * (src/lib.rs:414-418).
+ * Limits (CX_ERR_INVALID with a message when exceeded; the reference has none):
+ * 1 <= k <= 256 in both modes, d <= 16 in CX_KNN_MIXED. */
 cx_status cx_tree_knn_residual(cx_tree *t, const int64_t *queries, int64_t nq, int32_t k,
                                int32_t predecessor, int32_t mode, int64_t *out_idx,
                                float *out_dist, float *out_err);
 
 /* Same with device-resident queries and outputs (HBM pointers), enqueued on
- * the tree's stream and synchronised before returning. */
+ * the tree's own stream and synchronised before returning.  That stream does
+ * not wait for any other stream: the caller must have finished writing
+ * d_queries (e.g. synchronised the stream that produced them) before the
+ * call; the outputs are complete when it returns. */
 cx_status cx_tree_knn_residual_device(cx_tree *t, const int64_t *d_queries, int64_t nq,
                                       int32_t k, int32_t predecessor, int32_t mode,
                                       int64_t *d_out_idx, float *d_out_dist,

@@ -122,6 +122,10 @@ def test_c2_shape_sample(hip):
     assert np.all(np.where(valid, idx, -1) < q[:, None])
     dd = np.where(valid, dst, 2.0)
     assert np.all(np.diff(dd, axis=1) >= 0)
+    # the certified error bound of every returned distance, all 32,768 rows
+    eb = float(err[valid].max())
+    print(f"C2 max error bound over all rows = {eb:.3e}", flush=True)
+    assert eb <= DIST_TOL
 
 
 def test_smooth_field_rescreened(hip):
@@ -199,3 +203,78 @@ def test_overflowed_queries_rescreened(hip, pred, k):
     worst = _check(idx[sample], dst[sample], err[sample], x, v, p, sample, k, pred, 1)
     print(f"rescreened {tel['rescreened_queries']} exhaustive {tel['hard_queries']} "
           f"max |d - d64| = {worst:.3e}", flush=True)
+
+
+@pytest.mark.parametrize("n,d,m,kt,k,pred", [
+    (3000, 4, 50, 1, 200, True),    # generic screen (d = 4), unstaged query row (m % 4 != 0),
+                                    # seven kept entries per lane (k in 193..256)
+    (2000, 8, 600, 0, 40, False),   # d = 8 and m > 512
+])
+def test_mixed_generic_paths_vs_fp64(hip, n, d, m, kt, k, pred):
+    from covertreex_amd.native import HipTree
+
+    print(f"n={n} d={d} m={m} kt={kt} k={k} pred={pred}", flush=True)
+    rng = np.random.default_rng(17)
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    v, p, *_ = fp64.nystrom(x, inducing_count=m, seed=0, kernel_type=kt)
+    t = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=kt)
+    q = rng.choice(n, 300, replace=False)
+    idx, dst, err = t.knn_residual(q, k, predecessor_mode=pred, mode="mixed", return_error=True)
+    _check(idx, dst, err, x, v, p, q, k, pred, kt)
+
+
+def test_mixed_variance_mismatched_to_v(hip):
+    # V and p built with variance 1, the tree with variance 2: |rho| exceeds 1
+    # for many pairs and is clamped (the FP64 oracle clips it the same way);
+    # the screen's bounds must stay inside their histogram
+    from covertreex_amd.native import HipTree
+
+    rng = np.random.default_rng(23)
+    n, k = 3000, 20
+    x = rng.standard_normal((n, 3)).astype(np.float32)
+    v, p, *_ = fp64.nystrom(x, inducing_count=64, seed=0, kernel_type=1)
+    t = HipTree.build_residual(v, p, x, 2.0, np.ones(1, np.float32), kernel_type=1)
+    q = rng.choice(n, 300, replace=False)
+    for pred in (True, False):
+        idx, dst, err = t.knn_residual(q, k, predecessor_mode=pred, mode="mixed",
+                                       return_error=True)
+        _check(idx, dst, err, x, v, p, q, k, pred, 1, var=2.0)
+
+
+def test_mixed_floored_p_pair(hip):
+    # two nearly coincident rows whose p both sit at the 1e-9 floor: p_q p_j
+    # falls below 1e-18 and the denominator is floored; the reported bound
+    # must cover the FP64 error of that near pair
+    from covertreex_amd.native import HipTree
+
+    rng = np.random.default_rng(29)
+    n, k = 1200, 10
+    x = rng.standard_normal((n, 3)).astype(np.float32)
+    x[901] = x[900] + np.float32(1e-3)
+    v, p, *_ = fp64.nystrom(x, inducing_count=64, seed=0, kernel_type=1)
+    p = p.copy()
+    p[[900, 901]] = np.float32(1e-9)
+    t = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
+    q = np.array([901, 900, 950, 1199])
+    idx, dst, err = t.knn_residual(q, k, predecessor_mode=True, mode="mixed", return_error=True)
+    _check(idx, dst, err, x, v, p, q, k, True, 1)
+    assert 900 in idx[0].tolist()
+
+
+def test_mixed_out_of_range_queries(hip):
+    # out-of-range query indices read tree node 0 (src/lib.rs:414-418) and,
+    # as usize bounds, admit every row in predecessor mode
+    from covertreex_amd.native import HipTree
+
+    rng = np.random.default_rng(31)
+    n, k = 2000, 12
+    x = rng.standard_normal((n, 3)).astype(np.float32)
+    v, p, *_ = fp64.nystrom(x, inducing_count=64, seed=0, kernel_type=1)
+    t = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
+    node0 = int(t.get_order()[0])
+    want_i, want_d = t.knn_residual(np.array([node0]), k, predecessor_mode=False, mode="mixed")
+    for pred in (True, False):
+        gi, gd = t.knn_residual(np.array([-1, n + 5, node0]), k, predecessor_mode=pred,
+                                mode="mixed")
+        for r in (0, 1):
+            assert np.array_equal(gi[r], want_i[0]) and np.array_equal(gd[r], want_d[0])
