@@ -242,23 +242,33 @@ def test_mixed_variance_mismatched_to_v(hip):
 
 
 def test_mixed_floored_p_pair(hip):
-    # two nearly coincident rows whose p both sit at the 1e-9 floor: p_q p_j
-    # falls below 1e-18 and the denominator is floored; the reported bound
-    # must cover the FP64 error of that near pair
+    # Every p at 1e-9, so p_q p_j (9.9999994e-19 in FP32) sits below the
+    # 1e-18 floor and the denominator is exactly 1e-9.  Coordinates coincide
+    # (k = var) and var = 1e-8 with V rows of norm ~0.95e-4 in a narrow cone,
+    # so the numerators var - V_q.V_j sit just below 1e-9 and 1 - |rho| spreads
+    # over [0, 2e-4]: distances up to 0.011, where taking the FP32 literal
+    # 1e-18f (1.0000000458e-18) for the floor would move them by up to 1.7e-6,
+    # far beyond the reported bound the test holds them to.
     from covertreex_amd.native import HipTree
 
     rng = np.random.default_rng(29)
-    n, k = 1200, 10
-    x = rng.standard_normal((n, 3)).astype(np.float32)
-    x[901] = x[900] + np.float32(1e-3)
-    v, p, *_ = fp64.nystrom(x, inducing_count=64, seed=0, kernel_type=1)
-    p = p.copy()
-    p[[900, 901]] = np.float32(1e-9)
-    t = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
-    q = np.array([901, 900, 950, 1199])
-    idx, dst, err = t.knn_residual(q, k, predecessor_mode=True, mode="mixed", return_error=True)
-    _check(idx, dst, err, x, v, p, q, k, True, 1)
-    assert 900 in idx[0].tolist()
+    # the library stores var in FP32; the FP64 oracle takes that stored value
+    n, k, m, var = 1500, 10, 8, float(np.float32(1e-8))
+    x = np.zeros((n, 3), np.float32)
+    r = np.sqrt(0.9) * (1.0 + rng.uniform(0.0, 1e-5, n))
+    th = rng.uniform(0.0, 1e-3, n)
+    v = np.zeros((n, m), np.float32)
+    v[:, 0] = 1e-4 * r * np.cos(th)
+    v[:, 1] = 1e-4 * r * np.sin(th)
+    p = np.full(n, 1e-9, np.float32)
+    t = HipTree.build_residual(v, p, x, var, np.ones(1, np.float32), kernel_type=1)
+    q = rng.choice(n, 200, replace=False)
+    for pred in (True, False):
+        idx, dst, err = t.knn_residual(q, k, predecessor_mode=pred, mode="mixed",
+                                       return_error=True)
+        _check(idx, dst, err, x, v, p, q, k, pred, 1, var=var)
+        got = dst[idx >= 0]
+        assert got.max() > 5e-3 and np.any((got > 0) & (got < 2e-3))
 
 
 def test_mixed_out_of_range_queries(hip):
