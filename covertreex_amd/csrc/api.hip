@@ -36,8 +36,7 @@ TreeState::~TreeState()
     roots.release();
     child_off.release();
     child_list.release();
-    vhi.release();
-    vlo.release();
+    vsw.release();
     vnorm.release();
     x_ds.release();
     isp_ds.release();

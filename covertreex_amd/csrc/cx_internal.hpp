@@ -190,7 +190,7 @@ struct TreeState {
 
     // mixed-precision engine data, built lazily on first CX_KNN_MIXED query
     bool mixed_ready = false;
-    DevBuf<uint16_t> vhi, vlo;     // n x mp bf16, zero padded to mp
+    DevBuf<uint16_t> vsw;          // npad x mp bf16 hi and lo, MFMA-fragment layout
     int32_t mp = 0;                // m padded to a multiple of 32
     DevBuf<float> vnorm;           // |V_i| rounded up, dataset order
     DevBuf<float> x_ds, isp_ds;    // raw coords and 1/sqrt(p) in dataset order

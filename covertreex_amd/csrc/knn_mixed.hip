@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <type_traits>
 #include <vector>
 
 #include "cx_internal.hpp"
@@ -40,8 +41,10 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int kQT = 64;      // queries per workgroup tile
-constexpr int kCT = 128;     // candidates per candidate tile
-constexpr int kKC = 32;      // K chunk staged in LDS
+constexpr int kCW = 64;      // candidates per wavefront block
+constexpr int kCT = 4 * kCW; // candidates per workgroup tile
+constexpr int kMpAlign = 32; // V columns padded to this (int8 slice K steps too)
+constexpr int kKC = 64;      // K chunk of query rows staged in LDS
 constexpr int kKP = kKC + 8; // padded LDS row (bf16)
 constexpr int kBins = 128;   // histogram bins of the squared upper bound over [0, 1)
 constexpr float kHistCap = 0.99f; // squared bounds at or above this are not counted
@@ -51,7 +54,7 @@ constexpr int kCapBig = 8192;
 constexpr int64_t kBigListBudget = (int64_t)1 << 30; // bytes of those lists per call
 constexpr int kMaxD = 16;
 constexpr int kMixMaxK = 256;
-constexpr int64_t kSeg = 16 * kCT; // candidates per workgroup (grid y)
+constexpr int64_t kSeg = 8 * kCT; // candidates per workgroup (grid y)
 constexpr int kScreenWaves = 4;
 constexpr int kScreenThreads = 64 * kScreenWaves;
 
@@ -66,22 +69,37 @@ __device__ __forceinline__ uint16_t bf16_rne(float f)
 
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// V (tree order) -> dataset-order hi/lo, zero padded to mp columns.
+// Offset (in bf16 elements) of V[row][k], part h (0 hi, 1 lo), in the
+// MFMA-fragment layout: rows in blocks of 32, K in steps of 16; block R, step
+// s and part h form one 1 KiB fragment whose lane l holds the 8 values of row
+// 32R + (l & 31) at k = 16s + 8(l >> 5) .. + 7, the operand layout of
+// v_mfma_f32_32x32x16_bf16, so a wavefront loads a fragment as one
+// contiguous 1 KiB read.
+__device__ __forceinline__ int64_t frag_off(int64_t row, int32_t k, int h, int32_t steps)
+{
+    const int64_t chunk = ((row >> 5) * steps + (k >> 4)) * 2 + h;
+    const int lane = (int)(row & 31) + 32 * ((k >> 3) & 1);
+    return (chunk * 64 + lane) * 8 + (k & 7);
+}
+
+// V (tree order) -> dataset-order hi/lo in the fragment layout (zero padded
+// to mp columns; rows past n stay zero), row norms |V_i| rounded up.
 __global__ void split_kernel(const float *__restrict__ v, const int64_t *__restrict__ inv_order,
-                             int64_t n, int32_t m, int32_t mp, uint16_t *__restrict__ vhi,
-                             uint16_t *__restrict__ vlo, float *__restrict__ vnorm)
+                             int64_t n, int32_t m, int32_t mp, uint16_t *__restrict__ vsw,
+                             float *__restrict__ vnorm)
 {
     const int64_t row = blockIdx.x; // dataset index
     if (row >= n)
         return;
     const float *src = v + inv_order[row] * (int64_t)m;
+    const int32_t steps = mp / 16;
     float acc = 0.0f;
     for (int32_t c = threadIdx.x; c < mp; c += blockDim.x) {
         float x = c < m ? src[c] : 0.0f;
         uint16_t h = bf16_rne(x);
         uint16_t l = bf16_rne(x - bf16_to_f(h));
-        vhi[row * (int64_t)mp + c] = h;
-        vlo[row * (int64_t)mp + c] = l;
+        vsw[frag_off(row, c, 0, steps)] = h;
+        vsw[frag_off(row, c, 1, steps)] = l;
         acc = __builtin_fmaf(x, x, acc);
     }
     __shared__ float s_acc[64];
@@ -127,7 +145,7 @@ __global__ void gather_meta_kernel(const float *__restrict__ raw, const float *_
 // ---- screen ------------------------------------------------------------------
 
 struct ScreenArgs {
-    const uint16_t *vhi, *vlo; // npad x mp, dataset order, zero rows past n
+    const uint16_t *vsw; // npad x mp hi and lo, dataset order, fragment layout
     const float *x;     // n x d coords / lengthscale, dataset order
     const float *isp;   // n: 1/sqrt(p), relative error <= 2^-23
     const float *vnorm; // n
@@ -163,23 +181,24 @@ template <int KT> __device__ __forceinline__ float fast_kernel(float d2, const S
 }
 
 // First word of the packed codes of (tile block, candidate tile, wavefront,
-// lane): eight consecutive words per lane, word 2g + b holding candidate rows
-// rbase + 8g .. + 3 (one byte each) against query column qcol[b], so a
-// wavefront stores and reloads 2 KiB contiguous per candidate tile.
+// lane): sixteen consecutive words per lane, word (4 rb + g) 2 + b holding
+// the wavefront's candidate rows 32 rb + 8 g + 4 (lane >> 5) .. + 3 (one byte
+// each) against query column 32 b + (lane & 31), so a wavefront stores and
+// reloads 4 KiB contiguous per candidate tile.
 __device__ __forceinline__ int64_t code_word(int64_t blk, int ct, int wid, int lane)
 {
-    return (((blk * (kSeg / kCT) + ct) * kScreenWaves + wid) * 64 + lane) * 8;
+    return (((blk * (kSeg / kCT) + ct) * kScreenWaves + wid) * 64 + lane) * 16;
 }
 
 // Per-lane query columns of a query tile (shared by the screen and the
 // filter): column, Vecchia limit on the candidate index, own dataset row.
-__device__ __forceinline__ void tile_queries(const ScreenArgs &A, int64_t s0, int qh, int lane,
+__device__ __forceinline__ void tile_queries(const ScreenArgs &A, int64_t s0, int lane,
                                              int32_t *qcol, int32_t *qlim, int32_t *qself,
                                              int32_t *qrowv)
 {
 #pragma unroll
     for (int b = 0; b < 2; b++) {
-        qcol[b] = 64 * qh + 32 * b + (lane & 31);
+        qcol[b] = 32 * b + (lane & 31);
         const int64_t s = s0 + qcol[b];
         const bool live = s < A.nq;
         const int64_t qid = live ? A.qsorted[s] : 0;
@@ -206,44 +225,84 @@ __device__ __forceinline__ void tile_range(const ScreenArgs &A, int64_t s0, int3
     *c_end = e < b + (int32_t)kSeg ? e : b + (int32_t)kSeg;
 }
 
-// Pass 1.  One workgroup = 128 candidates (MFMA rows) x 64 queries (MFMA
-// columns) per candidate tile, looping over a 2048-candidate segment.  Four
-// wavefronts: wavefront w owns candidate rows 32*w.. and all 64 query
-// columns, i.e. two 32x32 output blocks, each with a hi.hi and a cross
-// accumulator.  Per pair: the certified interval of |rho|; its upper end
-// feeds the query's histogram, its lower end, as a squared-distance lower
-// bound, is stored as a one-byte code for the filter.
-// DD > 0 fixes the coordinate dimension (query coordinates then live in
-// registers); DD == 0 reads them from LDS for any d <= kMaxD.  KT is the
-// kernel (0 RBF, 1 Matern-5/2).  Coordinates arrive scaled by 1/lengthscale.
-template <int DD, int KT>
-__global__ void __launch_bounds__(kScreenThreads, 3) screen_kernel(ScreenArgs A)
+// Query rows of a K chunk: 64 rows x kKC bf16, hi and lo, as 16-byte pieces
+// of the fragment layout; thread t moves row t / 4, K16 step t % 4 of the
+// chunk (both 8-wide halves, hi and lo).  Steps past mp read zeros.
+struct BPieces {
+    uint4 h0, h1, l0, l1;
+};
+
+__device__ __forceinline__ BPieces load_b_chunk(const ScreenArgs &A, const int32_t *sQrow,
+                                                int32_t kc, int tid)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t sAh[kCT * kKP];
-    __shared__ __attribute__((aligned(16))) uint16_t sAl[kCT * kKP];
-    __shared__ __attribute__((aligned(16))) uint16_t sBh[kQT * kKP];
-    __shared__ __attribute__((aligned(16))) uint16_t sBl[kQT * kKP];
-    __shared__ __attribute__((aligned(16))) float sCw[kCT];
-    __shared__ __attribute__((aligned(16))) float sCn[kCT];
+    const int32_t steps = A.mp >> 4;
+    const int32_t s = (kc >> 4) + (tid & 3);
+    BPieces p;
+    if (s < steps) {
+        const int64_t row = sQrow[tid >> 2];
+        const int64_t base = (((row >> 5) * steps + s) * 2) * 64 + (row & 31);
+        const uint4 *v = (const uint4 *)A.vsw;
+        p.h0 = v[base];
+        p.h1 = v[base + 32];
+        p.l0 = v[base + 64];
+        p.l1 = v[base + 96];
+    } else {
+        p.h0 = p.h1 = p.l0 = p.l1 = make_uint4(0u, 0u, 0u, 0u);
+    }
+    return p;
+}
+
+__device__ __forceinline__ void store_b_chunk(uint16_t *sBh, uint16_t *sBl, const BPieces &p,
+                                              int tid)
+{
+    const int row = tid >> 2, col = 16 * (tid & 3);
+    *(uint4 *)(sBh + row * kKP + col) = p.h0;
+    *(uint4 *)(sBh + row * kKP + col + 8) = p.h1;
+    *(uint4 *)(sBl + row * kKP + col) = p.l0;
+    *(uint4 *)(sBl + row * kKP + col + 8) = p.l1;
+}
+
+// Pass 1.  A workgroup owns 64 queries (MFMA columns) and loops over a
+// 2048-candidate segment in tiles of 256 candidates; each of its four
+// wavefronts computes a 64 x 64 block (candidates 64 w .. of the tile: four
+// 32 x 32 MFMA blocks, each with a hi.hi and a cross accumulator, 128
+// accumulator registers).  Candidate fragments come straight from global
+// memory in the fragment layout (one 1 KiB load per fragment, two K16 steps
+// ahead); query rows go through LDS in double-buffered K chunks of 64, one
+// barrier per chunk.  Per pair: the certified interval [rlo, rhi] of |rho|;
+// 1 - rlo feeds the query's histogram, 1 - rhi is stored as a one-byte code
+// for the filter.  DD > 0 fixes the coordinate dimension (query coordinates
+// in registers); DD == 0 reads them from LDS for any d <= kMaxD.  KT is the
+// kernel (0 RBF, 1 Matern-5/2).  Coordinates arrive scaled by 1/lengthscale.
+constexpr int kHistRow = kBins / 2 + 1; // words per query column, padded
+
+template <int DD, int KT>
+__global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][kQT * kKP]; // [buf][hi, lo]
     __shared__ int32_t sQrow[kQT];
-    // histogram: two 16-bit bins per word (at most kSeg counts per bin)
-    __shared__ uint32_t sHist[kQT * kBins / 2];
-    // coordinates, structure of arrays: candidates [d][kCT], then queries [d][kQT]
+    // histogram: one 32-bit counter per bin when d is fixed (d <= 5 leaves
+    // room for two workgroups per CU), else two 16-bit bins per word (at
+    // most kSeg counts per bin); rows padded by one word
+    constexpr bool kWide = DD > 0;
+    constexpr int kRow = kWide ? kBins + 1 : kHistRow;
+    __shared__ uint32_t sHist[kQT * kRow];
+    // dynamic: per wavefront its 64 candidates' coordinates [d][64], 1/sqrt(p)
+    // and norms; then the query coordinates [d][kQT]
     extern __shared__ __attribute__((aligned(16))) float sDyn[];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int cb = wid, qh = 0;
-    // blockIdx.x = query tile within the launch, heavy tiles first
-    const int32_t tl = A.n_tiles - 1 - (int32_t)blockIdx.x;
+    const int32_t tl = A.n_tiles - 1 - (int32_t)blockIdx.x; // heavy tiles first
     const int64_t s0 = (int64_t)(A.tile0 + tl) * kQT;
     const int64_t blk = (int64_t)tl * A.n_seg + blockIdx.y;
-    const int32_t mp = A.mp;
+    const int32_t mp = A.mp, steps = mp >> 4;
     const int32_t d = DD > 0 ? DD : A.d;
     int32_t c_begin, c_end;
     tile_range(A, s0, &c_begin, &c_end);
     if (c_begin >= c_end)
         return;
-    float *sCx = sDyn, *sQx = sDyn + d * kCT;
+    float *sCx = sDyn + wid * (d + 2) * kCW, *sCw = sCx + d * kCW, *sCn = sCw + kCW;
+    float *sQx = sDyn + kScreenWaves * (d + 2) * kCW;
 
     for (int i = tid; i < kQT; i += kScreenThreads) {
         const int64_t s = s0 + i;
@@ -252,12 +311,12 @@ __global__ void __launch_bounds__(kScreenThreads, 3) screen_kernel(ScreenArgs A)
         for (int c = 0; c < d; c++)
             sQx[c * kQT + i] = A.x[(int64_t)row * d + c];
     }
-    for (int i = tid; i < kQT * kBins / 2; i += kScreenThreads)
+    for (int i = tid; i < kQT * kRow; i += kScreenThreads)
         sHist[i] = 0u;
+    __syncthreads();
 
-    // this lane's two query columns: mask limit, weight 1/sqrt(p), norm term
     int32_t qcol[2], qlim[2], qself[2], qrowv[2];
-    tile_queries(A, s0, qh, lane, qcol, qlim, qself, qrowv);
+    tile_queries(A, s0, lane, qcol, qlim, qself, qrowv);
     float qw[2], qcn[2];
     float qx[2][DD > 0 ? DD : 1];
 #pragma unroll
@@ -269,182 +328,233 @@ __global__ void __launch_bounds__(kScreenThreads, 3) screen_kernel(ScreenArgs A)
             for (int c = 0; c < (DD > 0 ? DD : 1); c++)
                 qx[b][c] = A.x[(int64_t)qrowv[b] * d + c];
     }
+    // smallest Vecchia limit over the tile's 64 queries (every wavefront
+    // holds all 64 columns): candidate blocks below it are valid for all
+    int32_t lim_min = qlim[0] < qlim[1] ? qlim[0] : qlim[1];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int32_t t = __shfl_xor(lim_min, o, 64);
+        lim_min = t < lim_min ? t : lim_min;
+    }
     const float eKp = A.eK * 1.000002f;
-    // a pair counts when 1 - rlo + 1.5e-7 < kHistCap; screen first on rlo
-    // itself with a slightly lower threshold
-    const float thr1 = 1.0f + 1.5e-7f - kHistCap - 1e-6f;
 
-    const int rbase = 32 * cb + 4 * (lane >> 5);
-    for (int32_t c0 = c_begin; c0 < c_end; c0 += kCT) {
-        __syncthreads(); // the previous tile's epilogue still reads sCx/sCw/sCn
-        for (int i = tid; i < kCT; i += kScreenThreads) {
-            const int32_t j = c0 + i;
-            const int32_t jr = j < A.n ? j : 0;
-            for (int c = 0; c < d; c++)
-                sCx[c * kCT + i] = A.x[(int64_t)jr * d + c];
-            sCw[i] = A.isp[jr];
-            sCn[i] = A.vnorm[jr];
-        }
-        floatx16 acc_h[2], acc_x[2];
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-            acc_h[b] = floatx16{};
-            acc_x[b] = floatx16{};
-        }
-        // stage 128 candidate + 64 query rows x 32 bf16, hi and lo: 1536
-        // segments of 16 B, six per thread (candidate rows tid/4 and 64 +
-        // tid/4, query row tid/4; byte offsets fit 32 bits, which the driver
-        // checks); the next K chunk's loads fly while this one is multiplied
-        const uint32_t seg8 = (uint32_t)(tid & 3) * 8u;
-        const uint32_t rc0 = (uint32_t)(c0 + (tid >> 2)) * (uint32_t)mp + seg8;
-        const uint32_t rc1 = rc0 + 64u * (uint32_t)mp;
-        const uint32_t rq = (uint32_t)sQrow[tid >> 2] * (uint32_t)mp + seg8;
-        const char *vh = (const char *)A.vhi, *vl = (const char *)A.vlo;
-        uint4 v0 = *(const uint4 *)(vh + rc0 * 2u), v1 = *(const uint4 *)(vh + rc1 * 2u);
-        uint4 v2 = *(const uint4 *)(vl + rc0 * 2u), v3 = *(const uint4 *)(vl + rc1 * 2u);
-        uint4 v4 = *(const uint4 *)(vh + rq * 2u), v5 = *(const uint4 *)(vl + rq * 2u);
-        const int srow = tid >> 2, sseg = (tid & 3) * 8;
-        for (int kc = 0; kc < mp; kc += kKC) {
-            __syncthreads();
-            *(uint4 *)(sAh + srow * kKP + sseg) = v0;
-            *(uint4 *)(sAh + (64 + srow) * kKP + sseg) = v1;
-            *(uint4 *)(sAl + srow * kKP + sseg) = v2;
-            *(uint4 *)(sAl + (64 + srow) * kKP + sseg) = v3;
-            *(uint4 *)(sBh + srow * kKP + sseg) = v4;
-            *(uint4 *)(sBl + srow * kKP + sseg) = v5;
-            __syncthreads();
-            if (kc + kKC < mp) {
-                const uint32_t o = (uint32_t)(kc + kKC) * 2u;
-                v0 = *(const uint4 *)(vh + rc0 * 2u + o);
-                v1 = *(const uint4 *)(vh + rc1 * 2u + o);
-                v2 = *(const uint4 *)(vl + rc0 * 2u + o);
-                v3 = *(const uint4 *)(vl + rc1 * 2u + o);
-                v4 = *(const uint4 *)(vh + rq * 2u + o);
-                v5 = *(const uint4 *)(vl + rq * 2u + o);
-            }
-            // MFMA issue ahead of other wavefronts' epilogues on this SIMD
-            __builtin_amdgcn_s_setprio(1);
-#pragma unroll
-            for (int ks = 0; ks < kKC; ks += 16) {
-                const int koff = ks + 8 * (lane >> 5);
-                const int ar = 32 * cb + (lane & 31);
-                const bf16x8 ah = __builtin_bit_cast(bf16x8, *(const uint4 *)(sAh + ar * kKP + koff));
-                const bf16x8 al = __builtin_bit_cast(bf16x8, *(const uint4 *)(sAl + ar * kKP + koff));
-#pragma unroll
-                for (int b = 0; b < 2; b++) {
-                    const int br = qcol[b];
-                    const bf16x8 bh =
-                        __builtin_bit_cast(bf16x8, *(const uint4 *)(sBh + br * kKP + koff));
-                    const bf16x8 bl =
-                        __builtin_bit_cast(bf16x8, *(const uint4 *)(sBl + br * kKP + koff));
-                    acc_h[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc_h[b], 0, 0, 0);
-                    acc_x[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc_x[b], 0, 0, 0);
-                    acc_x[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc_x[b], 0, 0, 0);
-                }
-            }
-            __builtin_amdgcn_s_setprio(0);
-        }
-        // epilogue: output register r of block b is candidate row
-        // rbase + 8*(r/4) + r%4, query column qcol[b]
-        const int ct = (c0 - c_begin) / kCT;
-        uint32_t words[8];
-#pragma unroll
-        for (int g = 0; g < 4; g++) {
-            const int rg = rbase + 8 * g;
-            const float4 cw = *(const float4 *)(sCw + rg);
-            const float4 cn = *(const float4 *)(sCn + rg);
-            float d2[2][4];
-#pragma unroll
-            for (int b = 0; b < 2; b++)
-#pragma unroll
-                for (int rr = 0; rr < 4; rr++)
-                    d2[b][rr] = 0.0f;
-            if (DD > 0) {
-#pragma unroll
-                for (int c = 0; c < (DD > 0 ? DD : 1); c++) {
-                    const float4 cx = *(const float4 *)(sCx + c * kCT + rg);
-#pragma unroll
-                    for (int b = 0; b < 2; b++) {
-                        const float t0 = qx[b][c] - cx.x, t1 = qx[b][c] - cx.y;
-                        const float t2 = qx[b][c] - cx.z, t3 = qx[b][c] - cx.w;
-                        d2[b][0] = __builtin_fmaf(t0, t0, d2[b][0]);
-                        d2[b][1] = __builtin_fmaf(t1, t1, d2[b][1]);
-                        d2[b][2] = __builtin_fmaf(t2, t2, d2[b][2]);
-                        d2[b][3] = __builtin_fmaf(t3, t3, d2[b][3]);
-                    }
-                }
-            } else {
-                for (int c = 0; c < d; c++) {
-                    const float4 cx = *(const float4 *)(sCx + c * kCT + rg);
-#pragma unroll
-                    for (int b = 0; b < 2; b++) {
-                        const float q = sQx[c * kQT + qcol[b]];
-                        const float t0 = q - cx.x, t1 = q - cx.y;
-                        const float t2 = q - cx.z, t3 = q - cx.w;
-                        d2[b][0] = __builtin_fmaf(t0, t0, d2[b][0]);
-                        d2[b][1] = __builtin_fmaf(t1, t1, d2[b][1]);
-                        d2[b][2] = __builtin_fmaf(t2, t2, d2[b][2]);
-                        d2[b][3] = __builtin_fmaf(t3, t3, d2[b][3]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < 2; b++) {
-                uint32_t word = 0u;
-#pragma unroll
-                for (int rr = 0; rr < 4; rr++) {
-                    const int r = 4 * g + rr;
-                    const int32_t j = c0 + rg + rr;
-                    const float cwv = rr == 0 ? cw.x : rr == 1 ? cw.y : rr == 2 ? cw.z : cw.w;
-                    const float cnv = rr == 0 ? cn.x : rr == 1 ? cn.y : rr == 2 ? cn.z : cn.w;
-                    const float kv = fast_kernel<KT>(d2[b][rr], A);
-                    const float num = kv - (acc_h[b][r] + acc_x[b][r]);
-                    // 1/sqrt(max(p_q p_j, 1e-18)) from per-row 1/sqrt(p)
-                    const float w = __builtin_fminf(qw[b] * cwv, 1e9f);
-                    const float ar = __builtin_fabsf(num) * w;
-                    // certified |rho| error: w (eK + cD |V_q||V_j|) (1 + 2e-6)
-                    // for the kernel, the dot and the final sum, plus 7.2e-7 ar
-                    // for the roundings of num, w and ar
-                    const float wE = w * __builtin_fmaf(qcn[b], cnv, eKp);
-                    const bool valid = j < qlim[b];
-                    // histogram of the squared upper bound 1 - (|rho| lower
-                    // bound); the pair's own row may be counted at its
-                    // computed bound or not at all, tau stays an upper bound
-                    const float rlo = __builtin_fmaf(ar, 1.0f - 7.2e-7f, -wE);
-                    if (valid && rlo > thr1) {
-                        // rlo can exceed 1 when var or p do not match V
-                        // (|rho| is clamped to 1 there): bin such pairs at 0
-                        const float ub2 = __builtin_fmaxf((1.0f - rlo) + 1.5e-7f, 0.0f);
-                        if (ub2 < kHistCap) {
-                            const int bin = (int)(ub2 * (float)kBins);
-                            atomicAdd(&sHist[(qcol[b] * kBins + bin) >> 1],
-                                      (bin & 1) ? 65536u : 1u);
-                        }
-                    }
-                    // code = floor(256 * squared lower bound), 255 if invalid
-                    const float rhi = __builtin_fmaf(ar, 1.0f + 7.2e-7f, wE);
-                    const float lb2 = (1.0f - rhi) - 1.5e-7f;
-                    const float cf = __builtin_fminf(__builtin_fmaxf(lb2 * 256.0f, 0.0f), 255.0f);
-                    const uint32_t code = valid ? (uint32_t)cf : 255u;
-                    word |= code << (8 * rr);
-                }
-                words[2 * g + b] = word;
-            }
-        }
-        uint4 *cdst = (uint4 *)(A.codes + code_word(blk, ct, wid, lane));
-        cdst[0] = make_uint4(words[0], words[1], words[2], words[3]);
-        cdst[1] = make_uint4(words[4], words[5], words[6], words[7]);
+    // B chunk 0 into buffer 0; later chunks cycle through the K range (the
+    // query rows are the same for every candidate tile)
+    const int32_t n_chunks = (mp + kKC - 1) / kKC;
+    {
+        const BPieces p = load_b_chunk(A, sQrow, 0, tid);
+        store_b_chunk(sB[0][0], sB[0][1], p, tid);
     }
     __syncthreads();
-    for (int i = tid; i < kQT * kBins / 2; i += kScreenThreads) {
-        const int64_t s = s0 + (2 * i) / kBins;
-        const uint32_t h = sHist[i];
-        if (s < A.nq && h) {
-            const int bin = (2 * i) % kBins;
-            if (h & 0xffffu)
-                atomicAdd(&A.hist[s * kBins + bin], (int32_t)(h & 0xffffu));
-            if (h >> 16)
-                atomicAdd(&A.hist[s * kBins + bin + 1], (int32_t)(h >> 16));
+    int buf = 0;
+    int32_t chunk = 0; // chunk held in sB[buf]
+
+    const uint4 *va = (const uint4 *)A.vsw;
+    // this lane's 16-byte piece of fragment (row block R, step s, part h)
+    auto a_piece = [&](int64_t R, int32_t s, int h) -> uint4 {
+        return va[((R * steps + s) * 2 + h) * 64 + lane];
+    };
+
+    for (int32_t c0 = c_begin; c0 < c_end; c0 += kCT) {
+        const int ct = (c0 - c_begin) / kCT;
+        const int32_t cw0 = c0 + kCW * wid;   // this wavefront's first candidate
+        const bool live = cw0 < c_end;        // uniform per wavefront
+        const int64_t R0 = cw0 >> 5;
+        // this wavefront's candidate data (read after the K loop)
+        {
+            const int32_t j = cw0 + lane < A.n ? cw0 + lane : 0;
+            for (int c = 0; c < d; c++)
+                sCx[c * kCW + lane] = A.x[(int64_t)j * d + c];
+            sCw[lane] = A.isp[j];
+            sCn[lane] = A.vnorm[j];
+        }
+        floatx16 acc_h[2][2], acc_x[2][2]; // [row block][query block]
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                acc_h[rb][b] = floatx16{};
+                acc_x[rb][b] = floatx16{};
+            }
+        // candidate fragments two K16 steps ahead: a[s & 1] = {hi R0, lo R0, hi R0+1, lo R0+1}
+        uint4 a[2][4];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int32_t s = q < steps ? q : 0;
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                a[q][f] = live ? a_piece(R0 + (f >> 1), s, f & 1) : make_uint4(0u, 0u, 0u, 0u);
+        }
+        for (int32_t ci = 0; ci < n_chunks; ci++) {
+            // the next chunk's query pieces fly while this one is multiplied
+            const int32_t nxt = chunk + 1 < n_chunks ? chunk + 1 : 0;
+            const BPieces pb = load_b_chunk(A, sQrow, nxt * kKC, tid);
+            const uint16_t *bh = sB[buf][0], *bl = sB[buf][1];
+            if (live) {
+                __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+                for (int j = 0; j < kKC / 16; j++) {
+                    const int32_t s = chunk * (kKC / 16) + j;
+                    if (s < steps) {
+                        const int koff = 16 * j + 8 * (lane >> 5);
+                        const int q = j & 1;
+                        bf16x8 fb[2][2];
+#pragma unroll
+                        for (int b = 0; b < 2; b++) {
+                            fb[b][0] = __builtin_bit_cast(
+                                bf16x8, *(const uint4 *)(bh + qcol[b] * kKP + koff));
+                            fb[b][1] = __builtin_bit_cast(
+                                bf16x8, *(const uint4 *)(bl + qcol[b] * kKP + koff));
+                        }
+#pragma unroll
+                        for (int rb = 0; rb < 2; rb++) {
+                            const bf16x8 ah = __builtin_bit_cast(bf16x8, a[q][2 * rb]);
+                            const bf16x8 al = __builtin_bit_cast(bf16x8, a[q][2 * rb + 1]);
+#pragma unroll
+                            for (int b = 0; b < 2; b++) {
+                                acc_h[rb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                    ah, fb[b][0], acc_h[rb][b], 0, 0, 0);
+                                acc_x[rb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                    ah, fb[b][1], acc_x[rb][b], 0, 0, 0);
+                                acc_x[rb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                    al, fb[b][0], acc_x[rb][b], 0, 0, 0);
+                            }
+                        }
+                        // refill this slot with step s + 2
+                        if (s + 2 < steps) {
+#pragma unroll
+                            for (int f = 0; f < 4; f++)
+                                a[q][f] = a_piece(R0 + (f >> 1), s + 2, f & 1);
+                        }
+                    }
+                }
+                __builtin_amdgcn_s_setprio(0);
+            }
+            // every wavefront has finished reading sB[buf ^ 1] (the chunk
+            // before this one) at the previous barrier
+            store_b_chunk(sB[buf ^ 1][0], sB[buf ^ 1][1], pb, tid);
+            __syncthreads();
+            buf ^= 1;
+            chunk = nxt;
+        }
+        if (!live)
+            continue;
+        // epilogue: register r of block (rb, b) is candidate cw0 + 32 rb +
+        // 8 (r / 4) + r % 4 + 4 (lane >> 5), query column qcol[b].  FULL:
+        // every pair of the block lies below every query's Vecchia limit.
+        auto epilogue = [&](auto full_tag) {
+            constexpr bool FULL = decltype(full_tag)::value;
+            uint32_t words[16];
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    const int rg = 32 * rb + 8 * g + 4 * (lane >> 5); // within the wavefront
+                    const float4 cw = *(const float4 *)(sCw + rg);
+                    const float4 cn = *(const float4 *)(sCn + rg);
+                    float d2[2][4];
+#pragma unroll
+                    for (int b = 0; b < 2; b++)
+#pragma unroll
+                        for (int rr = 0; rr < 4; rr++)
+                            d2[b][rr] = 0.0f;
+                    for (int c = 0; c < d; c++) {
+                        const float4 cx = *(const float4 *)(sCx + c * kCW + rg);
+#pragma unroll
+                        for (int b = 0; b < 2; b++) {
+                            const float q = DD > 0 ? qx[b][DD > 0 ? c : 0] : sQx[c * kQT + qcol[b]];
+                            const float t0 = q - cx.x, t1 = q - cx.y;
+                            const float t2 = q - cx.z, t3 = q - cx.w;
+                            d2[b][0] = __builtin_fmaf(t0, t0, d2[b][0]);
+                            d2[b][1] = __builtin_fmaf(t1, t1, d2[b][1]);
+                            d2[b][2] = __builtin_fmaf(t2, t2, d2[b][2]);
+                            d2[b][3] = __builtin_fmaf(t3, t3, d2[b][3]);
+                        }
+                    }
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        uint32_t word = 0u;
+#pragma unroll
+                        for (int rr = 0; rr < 4; rr++) {
+                            const int r = 4 * g + rr;
+                            const float cwv = rr == 0 ? cw.x : rr == 1 ? cw.y : rr == 2 ? cw.z : cw.w;
+                            const float cnv = rr == 0 ? cn.x : rr == 1 ? cn.y : rr == 2 ? cn.z : cn.w;
+                            const float kv = fast_kernel<KT>(d2[b][rr], A);
+                            const float an =
+                                __builtin_fabsf(kv - (acc_h[rb][b][r] + acc_x[rb][b][r]));
+                            // 1/sqrt(max(p_q p_j, 1e-18)) from per-row 1/sqrt(p)
+                            const float w = __builtin_fminf(qw[b] * cwv, 1e9f);
+                            // certified |rho| error w (eK + cD |V_q||V_j|) (1 + 2e-6)
+                            // for the kernel, the dot and the final sum, plus
+                            // 9.6e-7 |num| w for the roundings of num, w and
+                            // the products here: rlo <= |rho| <= rhi
+                            const float ew = w * __builtin_fmaf(qcn[b], cnv, eKp);
+                            const float aw = an * w;
+                            const float rlo = __builtin_fmaf(aw, 1.0f - 9.6e-7f, -ew);
+                            const float rhi = __builtin_fmaf(aw, 1.0f + 9.6e-7f, ew);
+                            bool valid = true;
+                            if (!FULL)
+                                valid = cw0 + rg + rr < qlim[b];
+                            // histogram bin of the squared upper bound 1 - rlo:
+                            // 128 (1 + 1.5e-7 - rlo) in one rounding (7.6e-6
+                            // at most, inside the 1.5e-7 margin), counted
+                            // below kHistCap; rlo above 1 (var or p not
+                            // matching V, |rho| clamped to 1) lands in bin 0.
+                            // The pair's own row may be counted at its bound
+                            // or not at all: tau stays an upper bound
+                            const float binf = __builtin_fmaf(rlo, -(float)kBins,
+                                                              (float)kBins * (1.0f + 1.5e-7f));
+                            if (valid && binf < (float)kBins * kHistCap) {
+                                const int bin = __builtin_amdgcn_fmed3f(binf, 0.0f, 127.0f);
+                                if (kWide)
+                                    atomicAdd(&sHist[qcol[b] * kRow + bin], 1u);
+                                else
+                                    atomicAdd(&sHist[qcol[b] * kRow + (bin >> 1)],
+                                              (bin & 1) ? 65536u : 1u);
+                            }
+                            // code = floor(256 * squared lower bound) with the
+                            // bound 1 - rhi - 2.3e-7 (the 2.3e-7 covers the
+                            // rounding of this fma), 255 if invalid
+                            const float cf = __builtin_fmaf(-256.0f, rhi, 255.99994f);
+                            const uint32_t code =
+                                valid ? (uint32_t)__builtin_amdgcn_fmed3f(cf, 0.0f, 255.0f) : 255u;
+                            word |= code << (8 * rr);
+                        }
+                        words[(4 * rb + g) * 2 + b] = word;
+                    }
+                }
+            uint4 *cdst = (uint4 *)(A.codes + code_word(blk, ct, wid, lane));
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                cdst[i] = make_uint4(words[4 * i], words[4 * i + 1], words[4 * i + 2],
+                                     words[4 * i + 3]);
+        };
+        if (cw0 + kCW <= lim_min)
+            epilogue(std::true_type{});
+        else
+            epilogue(std::false_type{});
+    }
+    __syncthreads();
+    if (kWide) {
+        for (int i = tid; i < kQT * kBins; i += kScreenThreads) {
+            const int q = i / kBins, bin = i % kBins;
+            const int64_t s = s0 + q;
+            const uint32_t h = sHist[q * kRow + bin];
+            if (s < A.nq && h)
+                atomicAdd(&A.hist[s * kBins + bin], (int32_t)h);
+        }
+    } else {
+        for (int i = tid; i < kQT * kBins / 2; i += kScreenThreads) {
+            const int q = i / (kBins / 2), wd = i % (kBins / 2);
+            const int64_t s = s0 + q;
+            const uint32_t h = sHist[q * kRow + wd];
+            if (s < A.nq && h) {
+                const int bin = 2 * wd;
+                if (h & 0xffffu)
+                    atomicAdd(&A.hist[s * kBins + bin], (int32_t)(h & 0xffffu));
+                if (h >> 16)
+                    atomicAdd(&A.hist[s * kBins + bin + 1], (int32_t)(h >> 16));
+            }
         }
     }
 }
@@ -457,7 +567,6 @@ __global__ void __launch_bounds__(kScreenThreads, 3) screen_kernel(ScreenArgs A)
 __global__ void __launch_bounds__(kScreenThreads) filter_kernel(ScreenArgs A)
 {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int cb = wid, qh = 0;
     const int32_t tl = A.n_tiles - 1 - (int32_t)blockIdx.x;
     const int64_t s0 = (int64_t)(A.tile0 + tl) * kQT;
     const int64_t blk = (int64_t)tl * A.n_seg + blockIdx.y;
@@ -466,7 +575,7 @@ __global__ void __launch_bounds__(kScreenThreads) filter_kernel(ScreenArgs A)
     if (c_begin >= c_end)
         return;
     int32_t qcol[2], qlim[2], qself[2], qrowv[2];
-    tile_queries(A, s0, qh, lane, qcol, qlim, qself, qrowv);
+    tile_queries(A, s0, lane, qcol, qlim, qself, qrowv);
     uint32_t qt[2];
 #pragma unroll
     for (int b = 0; b < 2; b++) {
@@ -474,32 +583,44 @@ __global__ void __launch_bounds__(kScreenThreads) filter_kernel(ScreenArgs A)
         const float t = s < A.nq ? A.tau[s] : 0.0f;
         qt[b] = t >= 2.0f ? 255u : (uint32_t)(t * 256.0f);
     }
-    const int rbase = 32 * cb + 4 * (lane >> 5);
+    const int rl = 4 * (lane >> 5);
     for (int32_t c0 = c_begin; c0 < c_end; c0 += kCT) {
         const int ct = (c0 - c_begin) / kCT;
-        uint32_t emit[2] = {0u, 0u};
+        const int32_t cw0 = c0 + kCW * wid;
+        if (cw0 >= c_end)
+            continue;
         const uint4 *csrc = (const uint4 *)(A.codes + code_word(blk, ct, wid, lane));
-        const uint4 w0 = csrc[0], w1 = csrc[1];
-        const uint32_t words[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+        uint32_t words[16];
 #pragma unroll
-        for (int g = 0; g < 4; g++)
+        for (int i = 0; i < 4; i++) {
+            const uint4 w = csrc[i];
+            words[4 * i] = w.x;
+            words[4 * i + 1] = w.y;
+            words[4 * i + 2] = w.z;
+            words[4 * i + 3] = w.w;
+        }
+        uint32_t emit[2] = {0u, 0u};
 #pragma unroll
-            for (int b = 0; b < 2; b++) {
-                const uint32_t word = words[2 * g + b];
+        for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-                for (int rr = 0; rr < 4; rr++) {
-                    const int32_t j = c0 + rbase + 8 * g + rr;
-                    const uint32_t code = (word >> (8 * rr)) & 255u;
-                    if (j < qlim[b] && code <= qt[b])
-                        emit[b] |= 1u << (4 * g + rr);
+            for (int g = 0; g < 4; g++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    const uint32_t word = words[(4 * rb + g) * 2 + b];
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) {
+                        const int32_t j = cw0 + 32 * rb + 8 * g + rl + rr;
+                        const uint32_t code = (word >> (8 * rr)) & 255u;
+                        if (j < qlim[b] && code <= qt[b])
+                            emit[b] |= 1u << ((4 * rb + g) * 4 + rr);
+                    }
                 }
-            }
 #pragma unroll
         for (int b = 0; b < 2; b++) {
             // the query's own row (distance 0) is always listed
-            const int32_t dr = qself[b] - c0 - rbase;
-            if (dr >= 0 && dr < 28 && (dr & 7) < 4 && qself[b] < qlim[b])
-                emit[b] |= 1u << (4 * (dr >> 3) + (dr & 3));
+            const int32_t dr = qself[b] - cw0 - rl;
+            if (dr >= 0 && dr < kCW - rl && (dr & 7) < 4 && qself[b] < qlim[b])
+                emit[b] |= 1u << ((dr >> 3) * 4 + (dr & 3));
             const int c_me = __builtin_popcount(emit[b]);
             const int c_other = __shfl_xor(c_me, 32, 64);
             const int64_t s = s0 + qcol[b];
@@ -514,7 +635,7 @@ __global__ void __launch_bounds__(kScreenThreads) filter_kernel(ScreenArgs A)
                 const int r = __builtin_ctz(mk);
                 mk &= mk - 1;
                 if (p < kCap)
-                    dst[p] = c0 + rbase + 8 * (r >> 2) + (r & 3);
+                    dst[p] = cw0 + rl + 8 * (r >> 2) + (r & 3);
                 p++;
             }
         }
@@ -1780,20 +1901,16 @@ static void prepare_mixed(TreeState &t)
         return;
     hipStream_t st = t.stream;
     const int64_t n = t.n;
-    t.mp = ((t.m + kKC - 1) / kKC) * kKC;
+    t.mp = ((t.m + kMpAlign - 1) / kMpAlign) * kMpAlign;
+    // rows padded past the last candidate tile a screen workgroup may read
     const int64_t npad = ((n + kCT - 1) / kCT) * kCT + kCT;
-    // the screen addresses the BF16 rows with 32-bit byte offsets
-    CX_REQUIRE(npad * t.mp * 2 < ((int64_t)1 << 32),
-               "mixed-precision kNN supports N * rank below 2^31 (BF16 row offsets)");
-    t.vhi.alloc((size_t)npad * t.mp);
-    t.vlo.alloc((size_t)npad * t.mp);
-    CX_HIP(hipMemsetAsync(t.vhi.ptr, 0, t.vhi.bytes(), st));
-    CX_HIP(hipMemsetAsync(t.vlo.ptr, 0, t.vlo.bytes(), st));
+    t.vsw.alloc((size_t)npad * t.mp * 2);
+    CX_HIP(hipMemsetAsync(t.vsw.ptr, 0, t.vsw.bytes(), st));
     t.vnorm.alloc(npad);
     CX_HIP(hipMemsetAsync(t.vnorm.ptr, 0, t.vnorm.bytes(), st));
     t.slices_ready = false; // the exact-product screen's slices follow V
-    split_kernel<<<(unsigned)n, 256, 0, st>>>(t.v.ptr, t.inv_order.ptr, n, t.m, t.mp, t.vhi.ptr,
-                                              t.vlo.ptr, t.vnorm.ptr);
+    split_kernel<<<(unsigned)n, 256, 0, st>>>(t.v.ptr, t.inv_order.ptr, n, t.m, t.mp, t.vsw.ptr,
+                                              t.vnorm.ptr);
     t.x_ds.alloc((size_t)npad * t.d);
     t.isp_ds.alloc(npad);
     CX_HIP(hipMemsetAsync(t.x_ds.ptr, 0, t.x_ds.bytes(), st));
@@ -1892,8 +2009,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     CX_HIP(hipMemsetAsync(ctr, 0, 256, st));
 
     ScreenArgs sa{};
-    sa.vhi = t.vhi.ptr;
-    sa.vlo = t.vlo.ptr;
+    sa.vsw = t.vsw.ptr;
     sa.x = t.x_ds.ptr; // scaled by 1/lengthscale
     sa.isp = t.isp_ds.ptr;
     sa.vnorm = t.vnorm.ptr;
@@ -1933,7 +2049,10 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     sa.list = list;
     const int32_t n_tiles = (int32_t)((nq + kQT - 1) / kQT);
     sa.n_seg = (int32_t)((n + kSeg - 1) / kSeg);
-    const size_t dyn_lds = sizeof(float) * (size_t)t.d * (kCT + kQT);
+    // per wavefront: its candidates' coordinates, 1/sqrt(p) and norms; the
+    // query coordinates
+    const size_t dyn_lds =
+        sizeof(float) * ((size_t)kScreenWaves * (t.d + 2) * kCW + (size_t)t.d * kQT);
     // launch groups of query tiles whose pair codes fit the code budget
     const int64_t tile_bytes = (int64_t)sa.n_seg * kSeg * kQT;
     const int32_t group =

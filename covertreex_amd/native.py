@@ -117,7 +117,7 @@ class HipTree:
                 C.c_void_p(d_err) if d_err else None))
 
     # -- telemetry ------------------------------------------------------------------
-    PHASES = {"mixed": ["prep", "query_sort", "screen_pass1", "tau", "screen_pass2", "rescore",
+    PHASES = {"mixed": ["prep", "query_sort", "screen_pass1", "hist_tau", "screen_pass2", "rescore",
                         "hard"],
               "reference": ["traversal"], "reference-budgeted": ["traversal"]}
 
