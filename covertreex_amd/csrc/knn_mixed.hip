@@ -355,6 +355,22 @@ __global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
         return va[((R * steps + s) * 2 + h) * 64 + lane];
     };
 
+    // candidate fragments two K16 steps ahead: a[s & 1] = {hi R0, lo R0, hi
+    // R0+1, lo R0+1}; the first two steps of a tile are loaded before the
+    // previous tile's epilogue
+    uint4 a[2][4];
+    auto load_first_steps = [&](int32_t cw) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int32_t s = q < steps ? q : 0;
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                a[q][f] = a_piece((cw >> 5) + (f >> 1), s, f & 1);
+        }
+    };
+    if (c_begin + kCW * wid < c_end)
+        load_first_steps(c_begin + kCW * wid);
+
     for (int32_t c0 = c_begin; c0 < c_end; c0 += kCT) {
         const int ct = (c0 - c_begin) / kCT;
         const int32_t cw0 = c0 + kCW * wid;   // this wavefront's first candidate
@@ -376,15 +392,6 @@ __global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
                 acc_h[rb][b] = floatx16{};
                 acc_x[rb][b] = floatx16{};
             }
-        // candidate fragments two K16 steps ahead: a[s & 1] = {hi R0, lo R0, hi R0+1, lo R0+1}
-        uint4 a[2][4];
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int32_t s = q < steps ? q : 0;
-#pragma unroll
-            for (int f = 0; f < 4; f++)
-                a[q][f] = live ? a_piece(R0 + (f >> 1), s, f & 1) : make_uint4(0u, 0u, 0u, 0u);
-        }
         for (int32_t ci = 0; ci < n_chunks; ci++) {
             // the next chunk's query pieces fly while this one is multiplied
             const int32_t nxt = chunk + 1 < n_chunks ? chunk + 1 : 0;
@@ -439,6 +446,8 @@ __global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
         }
         if (!live)
             continue;
+        if (cw0 + kCT < c_end)
+            load_first_steps(cw0 + kCT);
         // epilogue: register r of block (rb, b) is candidate cw0 + 32 rb +
         // 8 (r / 4) + r % 4 + 4 (lane >> 5), query column qcol[b].  FULL:
         // every pair of the block lies below every query's Vecchia limit.
