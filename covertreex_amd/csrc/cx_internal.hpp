@@ -192,6 +192,7 @@ struct TreeState {
     bool mixed_ready = false;
     DevBuf<uint16_t> vsw;          // npad x mp bf16 hi and lo, MFMA-fragment layout
     int32_t mp = 0;                // m padded to a multiple of 32
+    int32_t mps = 0;               // m padded to the screen's K chunk (64)
     DevBuf<float> vnorm;           // |V_i| rounded up, dataset order
     DevBuf<float> x_ds, isp_ds;    // raw coords and 1/sqrt(p) in dataset order
     DevBuf<uint8_t> codes;         // one byte per screened pair (a launch group)

@@ -57,6 +57,8 @@ constexpr int kMixMaxK = 256;
 constexpr int64_t kSeg = 8 * kCT; // candidates per workgroup (grid y)
 constexpr int kScreenWaves = 4;
 constexpr int kScreenThreads = 64 * kScreenWaves;
+// Workgroup b of a launch runs on XCD b % kXcds (round-robin dispatch)
+constexpr int kXcds = 8;
 
 // ---- prep ------------------------------------------------------------------
 
@@ -209,9 +211,32 @@ __device__ __forceinline__ void tile_queries(const ScreenArgs &A, int64_t s0, in
     }
 }
 
+// The screen's workgroup -> (query tile, segment) map.  Each XCD takes whole
+// candidate segments (so a segment's 2048 candidate rows, 1 MiB at rank
+// 256, stay in that XCD's L2 while every query tile streams them) and runs
+// all query tiles of one segment, heaviest first, before its next one.
+// Segments go to XCDs in snake order (0..7, then 7..0, ...): under the
+// Vecchia mask segment s is needed by the tiles above it, so pairing early
+// and late segments balances the XCDs.  The grid holds kXcds x
+// ceil(n_seg / kXcds) x n_tiles workgroups; those past the last segment and
+// tiles with no candidate in the segment exit at once.
+__device__ __forceinline__ bool screen_block(const ScreenArgs &A, int32_t *tl, int32_t *seg)
+{
+    const int32_t bid = (int32_t)blockIdx.x;
+    const int32_t xcd = bid % kXcds, k = bid / kXcds;
+    const int32_t round = k / A.n_tiles;
+    const int32_t pos = (round & 1) ? kXcds - 1 - xcd : xcd;
+    const int32_t sg = round * kXcds + pos;
+    if (sg >= A.n_seg)
+        return false;
+    *seg = sg;
+    *tl = A.n_tiles - 1 - k % A.n_tiles;
+    return true;
+}
+
 // candidate range [c_begin, c_end) of this (tile, segment); empty if none
-__device__ __forceinline__ void tile_range(const ScreenArgs &A, int64_t s0, int32_t *c_begin,
-                                           int32_t *c_end)
+__device__ __forceinline__ void tile_range(const ScreenArgs &A, int64_t s0, int32_t seg,
+                                           int32_t *c_begin, int32_t *c_end)
 {
     int32_t e = A.n;
     if (A.pred) {
@@ -220,14 +245,14 @@ __device__ __forceinline__ void tile_range(const ScreenArgs &A, int64_t s0, int3
         const int64_t qmax = A.qsorted[last];
         e = qmax < 0 ? A.n : (qmax < A.n ? (int32_t)qmax : A.n);
     }
-    const int32_t b = (int32_t)blockIdx.y * (int32_t)kSeg;
+    const int32_t b = seg * (int32_t)kSeg;
     *c_begin = b;
     *c_end = e < b + (int32_t)kSeg ? e : b + (int32_t)kSeg;
 }
 
 // Query rows of a K chunk: 64 rows x kKC bf16, hi and lo, as 16-byte pieces
 // of the fragment layout; thread t moves row t / 4, K16 step t % 4 of the
-// chunk (both 8-wide halves, hi and lo).  Steps past mp read zeros.
+// chunk (both 8-wide halves, hi and lo).
 struct BPieces {
     uint4 h0, h1, l0, l1;
 };
@@ -235,20 +260,17 @@ struct BPieces {
 __device__ __forceinline__ BPieces load_b_chunk(const ScreenArgs &A, const int32_t *sQrow,
                                                 int32_t kc, int tid)
 {
+    // the fragment layout's columns are padded to whole chunks
     const int32_t steps = A.mp >> 4;
     const int32_t s = (kc >> 4) + (tid & 3);
+    const int64_t row = sQrow[tid >> 2];
+    const int64_t base = (((row >> 5) * steps + s) * 2) * 64 + (row & 31);
+    const uint4 *v = (const uint4 *)A.vsw;
     BPieces p;
-    if (s < steps) {
-        const int64_t row = sQrow[tid >> 2];
-        const int64_t base = (((row >> 5) * steps + s) * 2) * 64 + (row & 31);
-        const uint4 *v = (const uint4 *)A.vsw;
-        p.h0 = v[base];
-        p.h1 = v[base + 32];
-        p.l0 = v[base + 64];
-        p.l1 = v[base + 96];
-    } else {
-        p.h0 = p.h1 = p.l0 = p.l1 = make_uint4(0u, 0u, 0u, 0u);
-    }
+    p.h0 = v[base];
+    p.h1 = v[base + 32];
+    p.l0 = v[base + 64];
+    p.l1 = v[base + 96];
     return p;
 }
 
@@ -292,13 +314,15 @@ __global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
     extern __shared__ __attribute__((aligned(16))) float sDyn[];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int32_t tl = A.n_tiles - 1 - (int32_t)blockIdx.x; // heavy tiles first
+    int32_t tl, seg;
+    if (!screen_block(A, &tl, &seg))
+        return;
     const int64_t s0 = (int64_t)(A.tile0 + tl) * kQT;
-    const int64_t blk = (int64_t)tl * A.n_seg + blockIdx.y;
+    const int64_t blk = (int64_t)tl * A.n_seg + seg;
     const int32_t mp = A.mp, steps = mp >> 4;
     const int32_t d = DD > 0 ? DD : A.d;
     int32_t c_begin, c_end;
-    tile_range(A, s0, &c_begin, &c_end);
+    tile_range(A, s0, seg, &c_begin, &c_end);
     if (c_begin >= c_end)
         return;
     float *sCx = sDyn + wid * (d + 2) * kCW, *sCw = sCx + d * kCW, *sCn = sCw + kCW;
@@ -340,7 +364,7 @@ __global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
 
     // B chunk 0 into buffer 0; later chunks cycle through the K range (the
     // query rows are the same for every candidate tile)
-    const int32_t n_chunks = (mp + kKC - 1) / kKC;
+    const int32_t n_chunks = mp / kKC;
     {
         const BPieces p = load_b_chunk(A, sQrow, 0, tid);
         store_b_chunk(sB[0][0], sB[0][1], p, tid);
@@ -356,26 +380,27 @@ __global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
     };
 
     // candidate fragments two K16 steps ahead: a[s & 1] = {hi R0, lo R0, hi
-    // R0+1, lo R0+1}; the first two steps of a tile are loaded before the
-    // previous tile's epilogue
+    // R0+1, lo R0+1}.  Every step refills its slot unconditionally (no branch
+    // for the wait counts to merge over): past a tile's last step with the
+    // next tile's first steps (or, after the wavefront's last tile, the same
+    // block again, unused).
     uint4 a[2][4];
-    auto load_first_steps = [&](int32_t cw) {
+    if (c_begin + kCW * wid < c_end) {
+        const int64_t R = (c_begin + kCW * wid) >> 5;
 #pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int32_t s = q < steps ? q : 0;
+        for (int q = 0; q < 2; q++)
 #pragma unroll
             for (int f = 0; f < 4; f++)
-                a[q][f] = a_piece((cw >> 5) + (f >> 1), s, f & 1);
-        }
-    };
-    if (c_begin + kCW * wid < c_end)
-        load_first_steps(c_begin + kCW * wid);
+                a[q][f] = a_piece(R + (f >> 1), q, f & 1);
+    }
 
     for (int32_t c0 = c_begin; c0 < c_end; c0 += kCT) {
         const int ct = (c0 - c_begin) / kCT;
         const int32_t cw0 = c0 + kCW * wid;   // this wavefront's first candidate
         const bool live = cw0 < c_end;        // uniform per wavefront
         const int64_t R0 = cw0 >> 5;
+        // the block whose first steps follow this tile's last ones
+        const int64_t Rn = cw0 + kCT < c_end ? R0 + kCT / 32 : R0;
         // this wavefront's candidate data (read after the K loop)
         {
             const int32_t j = cw0 + lane < A.n ? cw0 + lane : 0;
@@ -384,6 +409,7 @@ __global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
             sCw[lane] = A.isp[j];
             sCn[lane] = A.vnorm[j];
         }
+        // per 32 x 32 block: hi.hi and the cross terms hi.lo + lo.hi
         floatx16 acc_h[2][2], acc_x[2][2]; // [row block][query block]
 #pragma unroll
         for (int rb = 0; rb < 2; rb++)
@@ -399,41 +425,49 @@ __global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
             const uint16_t *bh = sB[buf][0], *bl = sB[buf][1];
             if (live) {
                 __builtin_amdgcn_s_setprio(1);
+                // query fragments one K16 step ahead of the MFMAs
+                bf16x8 fb[2][2][2]; // [slot][query block][hi, lo]
+                auto read_b = [&](int j, int slot) {
+                    const int koff = 16 * j + 8 * (lane >> 5);
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        fb[slot][b][0] = __builtin_bit_cast(
+                            bf16x8, *(const uint4 *)(bh + qcol[b] * kKP + koff));
+                        fb[slot][b][1] = __builtin_bit_cast(
+                            bf16x8, *(const uint4 *)(bl + qcol[b] * kKP + koff));
+                    }
+                };
+                read_b(0, 0);
 #pragma unroll
                 for (int j = 0; j < kKC / 16; j++) {
                     const int32_t s = chunk * (kKC / 16) + j;
-                    if (s < steps) {
-                        const int koff = 16 * j + 8 * (lane >> 5);
-                        const int q = j & 1;
-                        bf16x8 fb[2][2];
+                    const int q = j & 1;
+                    if (j + 1 < kKC / 16)
+                        read_b(j + 1, (j + 1) & 1);
+#pragma unroll
+                    for (int rb = 0; rb < 2; rb++) {
+                        const bf16x8 ah = __builtin_bit_cast(bf16x8, a[q][2 * rb]);
+                        const bf16x8 al = __builtin_bit_cast(bf16x8, a[q][2 * rb + 1]);
 #pragma unroll
                         for (int b = 0; b < 2; b++) {
-                            fb[b][0] = __builtin_bit_cast(
-                                bf16x8, *(const uint4 *)(bh + qcol[b] * kKP + koff));
-                            fb[b][1] = __builtin_bit_cast(
-                                bf16x8, *(const uint4 *)(bl + qcol[b] * kKP + koff));
-                        }
-#pragma unroll
-                        for (int rb = 0; rb < 2; rb++) {
-                            const bf16x8 ah = __builtin_bit_cast(bf16x8, a[q][2 * rb]);
-                            const bf16x8 al = __builtin_bit_cast(bf16x8, a[q][2 * rb + 1]);
-#pragma unroll
-                            for (int b = 0; b < 2; b++) {
-                                acc_h[rb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                    ah, fb[b][0], acc_h[rb][b], 0, 0, 0);
-                                acc_x[rb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                    ah, fb[b][1], acc_x[rb][b], 0, 0, 0);
-                                acc_x[rb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                    al, fb[b][0], acc_x[rb][b], 0, 0, 0);
-                            }
-                        }
-                        // refill this slot with step s + 2
-                        if (s + 2 < steps) {
-#pragma unroll
-                            for (int f = 0; f < 4; f++)
-                                a[q][f] = a_piece(R0 + (f >> 1), s + 2, f & 1);
+                            acc_h[rb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                ah, fb[j & 1][b][0], acc_h[rb][b], 0, 0, 0);
+                            acc_x[rb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                ah, fb[j & 1][b][1], acc_x[rb][b], 0, 0, 0);
+                            acc_x[rb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                al, fb[j & 1][b][0], acc_x[rb][b], 0, 0, 0);
                         }
                     }
+                    // refill this slot with step s + 2 (of this tile or the next)
+                    const bool wrap = s + 2 >= steps;
+                    const int64_t Rf = wrap ? Rn : R0;
+                    const int32_t sf = wrap ? s + 2 - steps : s + 2;
+#pragma unroll
+                    for (int f = 0; f < 4; f++)
+                        a[q][f] = a_piece(Rf + (f >> 1), sf, f & 1);
+                    // keep the refill here: the scheduler would otherwise sink
+                    // it next to its use two steps on, exposing its latency
+                    __builtin_amdgcn_sched_barrier(0);
                 }
                 __builtin_amdgcn_s_setprio(0);
             }
@@ -446,8 +480,6 @@ __global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
         }
         if (!live)
             continue;
-        if (cw0 + kCT < c_end)
-            load_first_steps(cw0 + kCT);
         // epilogue: register r of block (rb, b) is candidate cw0 + 32 rb +
         // 8 (r / 4) + r % 4 + 4 (lane >> 5), query column qcol[b].  FULL:
         // every pair of the block lies below every query's Vecchia limit.
@@ -580,7 +612,7 @@ __global__ void __launch_bounds__(kScreenThreads) filter_kernel(ScreenArgs A)
     const int64_t s0 = (int64_t)(A.tile0 + tl) * kQT;
     const int64_t blk = (int64_t)tl * A.n_seg + blockIdx.y;
     int32_t c_begin, c_end;
-    tile_range(A, s0, &c_begin, &c_end);
+    tile_range(A, s0, (int32_t)blockIdx.y, &c_begin, &c_end);
     if (c_begin >= c_end)
         return;
     int32_t qcol[2], qlim[2], qself[2], qrowv[2];
@@ -653,7 +685,10 @@ __global__ void __launch_bounds__(kScreenThreads) filter_kernel(ScreenArgs A)
 
 template <int KT> static void launch_screen_kt(const ScreenArgs &A, size_t lds, hipStream_t st)
 {
-    const dim3 grid((unsigned)A.n_tiles, (unsigned)A.n_seg);
+    const int64_t rounds = (A.n_seg + kXcds - 1) / kXcds;
+    const int64_t blocks = (int64_t)kXcds * rounds * A.n_tiles;
+    CX_REQUIRE(blocks < ((int64_t)1 << 31), "screen grid exceeds 2^31 workgroups");
+    const dim3 grid((unsigned)blocks);
     switch (A.d) {
     case 2:
         screen_kernel<2, KT><<<grid, kScreenThreads, lds, st>>>(A);
@@ -710,9 +745,7 @@ __global__ void tau_kernel(const int32_t *__restrict__ hist, int64_t s_begin, in
 constexpr int kWaves = 4;                       // wavefronts per workgroup
 constexpr int kRsThreads = 64 * kWaves;
 
-// Workgroup b of a launch runs on XCD b % kXcds (round-robin dispatch); the
-// scoring kernels give each XCD its own range of work.
-constexpr int kXcds = 8;
+// the scoring kernels give each XCD its own range of work
 constexpr int64_t kXcdQueries = 256; // consecutive sorted queries per XCD chunk
 
 
@@ -1913,12 +1946,14 @@ static void prepare_mixed(TreeState &t)
     t.mp = ((t.m + kMpAlign - 1) / kMpAlign) * kMpAlign;
     // rows padded past the last candidate tile a screen workgroup may read
     const int64_t npad = ((n + kCT - 1) / kCT) * kCT + kCT;
-    t.vsw.alloc((size_t)npad * t.mp * 2);
+    // the screen's fragment layout pads the columns to whole K chunks
+    t.mps = ((t.m + kKC - 1) / kKC) * kKC;
+    t.vsw.alloc((size_t)npad * t.mps * 2);
     CX_HIP(hipMemsetAsync(t.vsw.ptr, 0, t.vsw.bytes(), st));
     t.vnorm.alloc(npad);
     CX_HIP(hipMemsetAsync(t.vnorm.ptr, 0, t.vnorm.bytes(), st));
     t.slices_ready = false; // the exact-product screen's slices follow V
-    split_kernel<<<(unsigned)n, 256, 0, st>>>(t.v.ptr, t.inv_order.ptr, n, t.m, t.mp, t.vsw.ptr,
+    split_kernel<<<(unsigned)n, 256, 0, st>>>(t.v.ptr, t.inv_order.ptr, n, t.m, t.mps, t.vsw.ptr,
                                               t.vnorm.ptr);
     t.x_ds.alloc((size_t)npad * t.d);
     t.isp_ds.alloc(npad);
@@ -2027,17 +2062,17 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     sa.n = (int32_t)n;
     sa.nq = nq;
     sa.d = t.d;
-    sa.mp = t.mp;
+    sa.mp = t.mps;
     sa.kernel = t.kernel;
     sa.pred = predecessor ? 1 : 0;
     sa.k = k;
     sa.var = t.var;
     // |computed V.V - exact| <= cD |V_q||V_j|: the hi/lo split drops at most
-    // 3.1 * 2^-18 of each product, the hi.hi FP32 accumulation adds gamma_mp,
-    // the cross terms (2^-8 smaller) gamma_2mp, and the final sum one rounding.
+    // 3.1 * 2^-18 of each product, the hi.hi FP32 accumulation adds gamma_mps,
+    // the cross terms (2^-8 smaller) gamma_2mps, and the final sum one rounding.
     const double u = 1.0 / 16777216.0;
-    const double g1 = (t.mp + 16) * u / (1.0 - (t.mp + 16) * u);
-    const double g2 = (2 * t.mp + 16) * u / (1.0 - (2 * t.mp + 16) * u);
+    const double g1 = (t.mps + 16) * u / (1.0 - (t.mps + 16) * u);
+    const double g2 = (2 * t.mps + 16) * u / (1.0 - (2 * t.mps + 16) * u);
     sa.cD = (float)((3.1 / 262144.0 + g1 * 1.01 + g2 * 0.01 + 2 * u) * 1.05);
     // kernel value error: the FP32 kernel (exp2/sqrt, folded constants) stays
     // within 32 u var; scaled coordinates |x/l| <= R add at most
