@@ -34,4 +34,5 @@ from .lib import (  # noqa: F401
     morton_order,
     si_cache,
     subtree_bounds,
+    tree_from_arrays,
 )

@@ -432,13 +432,20 @@ def build_residual_tree_with_insert(coords, v, p, coords_new, v_new, p_new, *, v
     return _build(metric, n1, order, segs, c_ord, grid_scale, 1)
 
 
-def build_euclidean_tree(points, *, chunk=512) -> OracleTree:
-    """Chunked batch_insert_collect with the Euclidean metric in natural order."""
+def build_euclidean_tree(points, *, chunk=512, grid_scale=None) -> OracleTree:
+    """Chunked batch_insert_collect with the Euclidean metric in natural order.
+
+    With ``grid_scale`` the batch insert also adds grid conflict edges over
+    the points themselves (``coords`` of batch_insert_collect,
+    src/algo/batch/mod.rs:660-668), as the residual build does over its raw
+    coordinates; the reference's Euclidean API path passes no coordinates."""
     pts = _c(points, np.float32)
     n = pts.shape[0]
     order = np.arange(n, dtype=np.int64)
     metric = euclidean_metric(pts)
-    return _build(metric, n, order, [(0, n, chunk)], None, 1.0, 0)
+    if grid_scale is None:
+        return _build(metric, n, order, [(0, n, chunk)], None, 1.0, 0)
+    return _build(metric, n, order, [(0, n, chunk)], pts, float(grid_scale), 1)
 
 
 def build_euclidean_tree_with_insert(points, new_points, *, chunk=512,
@@ -459,7 +466,40 @@ def si_cache(tree: OracleTree) -> np.ndarray:
     return si[: tree.parents.size]
 
 
-def _opts(budget, threads):
+def tree_from_arrays(coords, v, p, *, parents, children, next_node, levels, si, var=1.0, ls=1.0,
+                     kernel_type=0, exp_mode=EXP_LIBM) -> OracleTree:
+    """A residual tree given node by node (natural order: node i is dataset
+    row i), for hand-built query cases: flat arrays as CoverTreeData keeps
+    them (src/tree.rs:6-125), si the cover radii the query reads."""
+    coords = _c(coords, np.float32)
+    n = coords.shape[0]
+    metric = residual_metric(coords, np.asarray(v, np.float32), np.asarray(p, np.float32), var,
+                             ls, kernel_type, exp_mode)
+    L = lib()
+    t = L.ox_tree_new(max(n, 1))
+    tt = t.contents
+    for name, arr, dt in (("parents", parents, np.int64), ("children", children, np.int64),
+                          ("next", next_node, np.int64), ("levels", levels, np.int32),
+                          ("si", si, np.float32)):
+        a = _c(arr, dt)
+        assert a.shape == (n,)
+        C.memmove(getattr(tt, name), a.ctypes.data, a.nbytes)
+    tt.n = n
+    lv = np.asarray(levels, np.int64)
+    set_lv = lv[lv != np.iinfo(np.int32).min]
+    tt.min_level = int(set_lv.min()) if set_lv.size else 0
+    tt.max_level = int(set_lv.max()) if set_lv.size else 0
+    order = np.arange(n, dtype=np.int64)
+    par = _c(parents, np.int64)
+    smin, smax = subtree_bounds(par, order)
+    return OracleTree(order=order, inv_order=order.copy(), parents=par.copy(),
+                      children=_c(children, np.int64).copy(), next=_c(next_node, np.int64).copy(),
+                      levels=_c(levels, np.int32).copy(), si=_c(si, np.float32).copy(),
+                      min_level=tt.min_level, max_level=tt.max_level, subtree_min=smin,
+                      subtree_max=smax, _tree=t, _metric=metric)
+
+
+def _opts(budget, threads, budget_up=None, budget_down=None):
     o = _QueryOpts()
     lib().ox_query_opts_default(C.byref(o))
     if budget is not None and budget != "default":
@@ -467,16 +507,22 @@ def _opts(budget, threads):
         o.n_budget = len(sched)
         for i, b in enumerate(sched[:8]):
             o.budget[i] = int(b)
+    if budget_up is not None:
+        o.budget_up = float(budget_up)
+    if budget_down is not None:
+        o.budget_down = float(budget_down)
     o.threads = int(threads)
     return o
 
 
 def knn_residual(tree: OracleTree, queries, k: int, *, predecessor=False, budget=(),
-                 threads: int = 0, use_bounds=True):
+                 threads: int = 0, use_bounds=True, budget_up=None, budget_down=None):
     """CoverTreeWrapper.knn_query_residual cached path (src/lib.rs:369-507).
 
     ``budget=()`` is the exact search (empty schedule); ``budget="default"``
-    the reference default ladder [32,64,96]."""
+    the reference default ladder [32,64,96].  ``budget_up`` / ``budget_down``
+    replace the yield thresholds 0.015 / 0.002 (the reference reads them from
+    COVERTREEX_RESIDUAL_BUDGET_UP / _DOWN, src/algo.rs:661-676)."""
     q = _c(np.asarray(queries).reshape(-1), np.int64)
     n = tree.order.size
     q_tree = np.where((q >= 0) & (q < n), tree.inv_order[np.clip(q, 0, max(n - 1, 0))], 0)
@@ -485,7 +531,7 @@ def knn_residual(tree: OracleTree, queries, k: int, *, predecessor=False, budget
     out_i = np.empty((nq, k), np.int64)
     out_d = np.empty((nq, k), np.float32)
     evals = np.zeros(nq, np.int64)
-    o = _opts(budget, threads)
+    o = _opts(budget, threads, budget_up, budget_down)
     smin = tree.subtree_min if (predecessor and use_bounds) else None
     lib().ox_knn_residual(tree._tree, tree._metric.h, _ptr(tree.order), _ptr(q_tree), _ptr(q),
                           nq, int(k), int(bool(predecessor)), _ptr(smin), C.byref(o),
