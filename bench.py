@@ -45,11 +45,31 @@ def make_inputs():
     return x, v, p
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(x, v, p, sample_q: int):
     """The oracle's restatement of the reference search (exact semantics) on
-    the host cores over a bounded sample of queries; build excluded."""
-    import oracle
+    the host cores over a bounded sample of queries; build excluded.  The
+    timed copy is compiled here for this host (-O3 -march=native, the same
+    floating-point flags); the in-tree build is used if that fails."""
+    import tempfile
 
+    import oracle
+    import oracle.lib as ol
+
+    native = os.path.join(tempfile.mkdtemp(prefix="cx_oracle_"), "liboracle_native.so")
+    built = ol.build_native(native)
+    if built:
+        ol.use_library(native)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     t = oracle.build_residual_tree(x, v, p, kernel_type=1)
     q = np.random.default_rng(3).choice(N, sample_q, replace=False)
@@ -59,10 +79,19 @@ def cpu_baseline(x, v, p, sample_q: int):
     t0 = time.perf_counter()
     oracle.knn_residual(t, q, K, predecessor=True, budget="default", threads=threads)
     el_b = time.perf_counter() - t0
+    q1 = q[: max(1, sample_q // 8)]
+    t0 = time.perf_counter()
+    oracle.knn_residual(t, q1, K, predecessor=True, budget=(), threads=1)
+    el_1 = time.perf_counter() - t0
     return {"value": round(sample_q / el, 1), "unit": "queries/s", "cores": threads,
             "kind": "port",
             "sample": f"{sample_q} random queries of the same workload, exact search "
                       f"(empty budget schedule), {threads} OpenMP threads",
+            "cpu_model": _cpu_model(),
+            "compiler_flags": ("-O3 -march=native -fopenmp -ffp-contract=off" if built
+                               else "-O3 -fopenmp -ffp-contract=off (in-tree build)"),
+            "one_thread_qps": round(q1.size / el_1, 1),
+            "one_thread_sample": f"{q1.size} of the same queries, exact search, 1 thread",
             "budgeted_default_qps": round(sample_q / el_b, 1),
             "mean_evals_per_query": float(evals.mean())}
 

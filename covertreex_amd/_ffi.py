@@ -86,6 +86,7 @@ SIGNATURES = {
     "cx_tree_delete": (_i32, [_p, _p, _i64]),
     "cx_tree_build_telemetry": (_i32, [_p, _p, _i64, _p]),
     "cx_tree_check_invariants": (_i32, [_p, _p]),
+    "cx_tree_debug_set_node": (_i32, [_p, _i64, _i64, _i32]),
     "cx_tree_set_query_telemetry": (_i32, [_p, _i32]),
     "cx_tree_last_query_telemetry": (_i32, [_p, _p, _i32, _p]),
     "cx_tree_clear_last_query_telemetry": (_i32, [_p]),

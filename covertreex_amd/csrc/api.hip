@@ -633,6 +633,24 @@ cx_status cx_tree_check_invariants(cx_tree *t, int64_t *out)
     });
 }
 
+cx_status cx_tree_debug_set_node(cx_tree *t, int64_t node, int64_t parent, int32_t level)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        TreeState &s = t->s;
+        CX_REQUIRE(node >= 0 && node < s.n, "node out of range");
+        CX_REQUIRE(parent >= -1 && parent < s.n, "parent out of range");
+        std::lock_guard<std::mutex> lock(s.mu);
+        StreamScope ss_(s.stream);
+        CX_HIP(hipSetDevice(s.device));
+        CX_HIP(hipMemcpyAsync(s.parents.ptr + node, &parent, sizeof(int64_t),
+                              hipMemcpyHostToDevice, s.stream));
+        CX_HIP(hipMemcpyAsync(s.levels.ptr + node, &level, sizeof(int32_t),
+                              hipMemcpyHostToDevice, s.stream));
+        CX_HIP(hipStreamSynchronize(s.stream));
+    });
+}
+
 cx_status cx_tree_build_telemetry(const cx_tree *t, int64_t *out, int64_t cap_batches,
                                   int64_t *n_batches)
 {

@@ -233,6 +233,11 @@ class HipTree:
             raise CoverTreeInvariantError(f"cover-tree invariants violated: {bad} ({res})")
         return res
 
+    def debug_set_node(self, node: int, parent: int, level: int) -> None:
+        """Test support: overwrite one node's parent pointer and level on the
+        device without relinking (cx_tree_debug_set_node)."""
+        check(_ffi.load().cx_tree_debug_set_node(self._h, int(node), int(parent), int(level)))
+
     def _after_insert(self) -> None:
         if self.check_after_insert:
             self.check_invariants()

@@ -279,6 +279,13 @@ cx_status cx_tree_set_query_telemetry(cx_tree *t, int32_t enable);
 #define CX_INV_BATCHES 6    /* separation groups */
 cx_status cx_tree_check_invariants(cx_tree *t, int64_t *out);
 
+/* Test support, no counterpart in the reference: overwrite one node's parent
+ * pointer and level on the device, leaving every child list as it is, so a
+ * test can hand cx_tree_check_invariants a tree that breaks the covering,
+ * nesting, separation or forest invariants.  Queries on such a tree are
+ * meaningless. */
+cx_status cx_tree_debug_set_node(cx_tree *t, int64_t node, int64_t parent, int32_t level);
+
 /* *n_out = CX_QT_SIZE when the last call recorded telemetry, else 0. */
 cx_status cx_tree_last_query_telemetry(const cx_tree *t, int64_t *out, int32_t cap,
                                        int32_t *n_out);

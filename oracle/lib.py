@@ -58,10 +58,32 @@ class _Tree(C.Structure):
     ]
 
 
+def build_native(out_path: str) -> bool:
+    """Compile a copy tuned for the host it runs on (-O3 -march=native, same
+    floating-point flags, so identical results) at ``out_path``; the CPU
+    baseline times that copy.  Returns False when the compiler is missing or
+    fails."""
+    flags = ("-O3 -march=native -fPIC -std=c11 -fopenmp -ffp-contract=off -fno-fast-math "
+             "-Wall -Wextra -Wno-unused-parameter")
+    r = subprocess.run(["make", "-s", "-B", "-C", _HERE, f"OUT={out_path}", f"CFLAGS={flags}"],
+                       capture_output=True)
+    return r.returncode == 0 and os.path.exists(out_path)
+
+
+def use_library(path: str) -> None:
+    """Load the oracle from ``path`` instead of the in-tree build (before any
+    other use in this process)."""
+    global _LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    _LIB_PATH = path
+
+
 def lib():
     global _lib
     if _lib is None:
-        build_library()
+        if _LIB_PATH == os.path.join(_HERE, "liboracle.so"):
+            build_library()
         L = C.CDLL(_LIB_PATH)
         L.ox_metric_residual.restype = _p
         L.ox_metric_residual.argtypes = [_i64, _i32, _i32, _p, _p, _p, _f32, _p, _i32, _i32, _i32]

@@ -71,3 +71,36 @@ def test_euclidean_tree_invariants(hip):
     assert inv["nodes"] == int(np.sum(t.get_parents() >= 0))
     batches = [(b["start"], b["size"]) for b in t.build_telemetry()]
     assert inv["pairs"] == _same_level_pairs(t.get_levels(), batches)
+
+
+def test_checker_reports_a_corrupted_tree(hip):
+    # the hand-traced residual tree of tests/test_known_answers.py (1-D, V = 0,
+    # p = 1, RBF, chunk 4, grid scale 2): clean it has no violation; then one
+    # node at a time is rewritten on the device without relinking
+    from covertreex_amd.native import CoverTreeInvariantError, HipTree
+
+    y = np.array([0.0, 0.25, 0.25, 0.6, 0.125, 0.2, -0.1, 0.2], np.float32).reshape(-1, 1)
+    v, p = np.zeros((8, 1), np.float32), np.ones(8, np.float32)
+
+    def build():
+        return HipTree.build_residual(v, p, y, 1.0, np.ones(1, np.float32), chunk_size=4,
+                                      grid_whiten_scale=2.0, kernel_type=0)
+
+    t = build()
+    inv = t.check_invariants()
+    assert all(inv[k] == 0 for k in ("covering", "nesting", "separation", "forest"))
+    # node 7 (level -10, d^2 = 0 to its parent 5) re-pointed at node 3: d^2 =
+    # 1 - exp(-0.4^2/2) = 0.077 > 4^-9 breaks covering, and 5's child list
+    # still holds 7 while 3's does not: the forest is inconsistent
+    t.debug_set_node(7, 3, -10)
+    inv = t.check_invariants(raise_on_violation=False)
+    assert inv["covering"] >= 1 and inv["forest"] >= 1
+    with pytest.raises(CoverTreeInvariantError):
+        t.check_invariants()
+    # node 5 moved to level -4 beside node 4, both of batch 2: d^2 = 0.0028 <=
+    # 4^-4 and the same grid cell break separation
+    t = build()
+    t.debug_set_node(5, 1, -4)
+    inv = t.check_invariants(raise_on_violation=False)
+    assert inv["separation"] >= 1
+    assert inv["covering"] == 0 and inv["forest"] == 0
