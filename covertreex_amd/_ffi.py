@@ -76,6 +76,7 @@ SIGNATURES = {
     "cx_subtree_bounds": (_i32, [_p, _p, _i64, _p, _p]),
     "cx_morton_order": (_i32, [_p, _i64, _i32, _p]),
     "cx_hilbert_order": (_i32, [_p, _i64, _i32, _p]),
+    "cx_hilbert_order_noalloc": (_i32, [_p, _i64, _i32, _p, _p]),
     "cx_luby_mis": (_i32, [_p, _p, _i64, C.c_uint32, _p, _p]),
     "cx_nystrom_residual": (_i32, [_p, _i64, _i32, _p, _i32, C.c_double, _p, _i32, _i32,
                                    C.c_double, _p, _p]),

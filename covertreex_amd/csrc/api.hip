@@ -719,6 +719,20 @@ cx_status cx_hilbert_order(const double *points, int64_t n, int32_t d, int64_t *
     });
 }
 
+cx_status cx_hilbert_order_noalloc(const double *d_points, int64_t n, int32_t d,
+                                   int64_t *d_order_out, void *stream)
+{
+    return guarded([&] {
+        CX_REQUIRE(n >= 0, "n must be >= 0");
+        CX_REQUIRE(d >= 1 && d <= 64, "need 1 <= d <= 64 (codes must fit 64 bits)");
+        if (n == 0)
+            return;
+        CX_REQUIRE(d_points && d_order_out, "NULL pointer");
+        require_device();
+        hilbert_order_noalloc(d_points, n, d, d_order_out, (hipStream_t)stream);
+    });
+}
+
 cx_status cx_luby_mis(const int64_t *indptr, const int64_t *indices, int64_t n, uint32_t seed,
                       uint8_t *selected_out, int64_t *iterations_out)
 {

@@ -278,6 +278,8 @@ void knn_euclidean(TreeState &t, const float *d_queries, int64_t nq, int32_t k, 
 
 // graph.hip
 void hilbert_order_device(const double *points_host, int64_t n, int32_t d, int64_t *order_host);
+void hilbert_order_noalloc(const double *d_points, int64_t n, int32_t d, int64_t *d_order,
+                           hipStream_t st);
 int64_t luby_mis_device(const int64_t *indptr_host, const int64_t *indices_host, int64_t n,
                         uint32_t seed, uint8_t *sel_host);
 

@@ -4,6 +4,8 @@
 // Both are integer work: bit-exact with the reference's semantics.
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include "cx_internal.hpp"
 
 namespace cx {
@@ -113,6 +115,69 @@ void hilbert_order_device(const double *points_host, int64_t n, int32_t d, int64
     CX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, codes.ptr, sorted.ptr, idx.ptr,
                                               order.ptr, (int)n, 0, 64, st));
     CX_HIP(hipMemcpy(order_host, order.ptr, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+}
+
+// ---- Hilbert order without allocation ---------------------------------------
+//
+// The same codes and stable sort on device-resident points, with every work
+// buffer carved from one file-scope device array instead of device
+// allocations, for callers that cannot allocate (graph capture, tight
+// serving loops).  Layout: per-axis bounds (2 x 64 doubles), codes, sorted
+// codes and indices (kNoAllocMaxPoints each), then the radix sort's
+// temporary storage.  Every GPU holds its own copy of the array until the
+// process ends; calls that overlap on one GPU share it and race.
+
+constexpr int64_t kNoAllocMaxPoints = (int64_t)1 << 18;
+constexpr size_t kNoAllocSortBytes = (size_t)8 << 20;
+constexpr size_t kNoAllocFixedBytes =
+    2 * 64 * sizeof(double) + 3 * (size_t)kNoAllocMaxPoints * sizeof(uint64_t);
+constexpr size_t kNoAllocScratchBytes = kNoAllocFixedBytes + kNoAllocSortBytes;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's address of `scratch` (looked up once per device)
+static unsigned char *scratch_for_current_device()
+{
+    int dev = 0;
+    CX_HIP(hipGetDevice(&dev));
+    CX_REQUIRE(dev >= 0 && dev < kMaxDevices, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *ptr = nullptr;
+        CX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        CX_REQUIRE(ptr != nullptr, "hipGetSymbolAddress returned no address for scratch");
+        g_scratch_ptr[dev] = ptr;
+    }
+    return (unsigned char *)g_scratch_ptr[dev];
+}
+
+void hilbert_order_noalloc(const double *d_points, int64_t n, int32_t d, int64_t *d_order,
+                           hipStream_t st)
+{
+    CX_REQUIRE(n <= kNoAllocMaxPoints, "allocation-free Hilbert order supports at most 262144 "
+                                       "points (its fixed scratch array)");
+    unsigned char *base = scratch_for_current_device();
+    double *lo = (double *)base, *hi = lo + 64;
+    uint64_t *codes = (uint64_t *)(hi + 64);
+    uint64_t *sorted = codes + kNoAllocMaxPoints;
+    int64_t *idx = (int64_t *)(sorted + kNoAllocMaxPoints);
+    void *tmp = base + kNoAllocFixedBytes;
+    size_t tmp_bytes = 0;
+    CX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, codes, sorted, idx, d_order,
+                                              (int)n, 0, 64, st));
+    CX_REQUIRE(tmp_bytes <= kNoAllocSortBytes, "radix sort storage exceeds the scratch array");
+    int32_t bits = 64 / d;
+    bits = bits > 16 ? 16 : (bits < 1 ? 1 : bits);
+    minmax_f64_kernel<<<(unsigned)d, 256, 0, st>>>(d_points, n, d, lo, hi);
+    hilbert_codes_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(d_points, n, d, bits, lo,
+                                                                      hi, codes, idx);
+    CX_HIP(hipGetLastError());
+    CX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, codes, sorted, idx, d_order, (int)n,
+                                              0, 64, st));
 }
 
 // ---- Luby MIS ----------------------------------------------------------------

@@ -149,6 +149,17 @@ cx_status cx_morton_order(const float *coords, int64_t n, int32_t d, int64_t *or
  * stably (ties by index).  points: n x d float64; d <= 64. */
 cx_status cx_hilbert_order(const double *points, int64_t n, int32_t d, int64_t *order_out);
 
+/* No counterpart in the reference: cx_hilbert_order for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  d_points (n x d float64) and d_order_out (n int64) are
+ * device pointers; the work is enqueued on `stream` (a hipStream_t, NULL for
+ * the default stream) and the order is complete when that stream's work is.
+ * It allocates nothing: its work buffers are one fixed device array per GPU
+ * (about 14 MiB, held until the process ends), so n is at most 262144
+ * (CX_ERR_INVALID beyond) and calls that overlap on one GPU race. */
+cx_status cx_hilbert_order_noalloc(const double *d_points, int64_t n, int32_t d,
+                                   int64_t *d_order_out, void *stream);
+
 /* Replaces run_mis_numba / _run_mis_numba_impl (covertreex/algo/mis.py:55-139,
  * covertreex/algo/_mis_numba.py:64-131): greedy in index order when the
  * max degree is <= 8, else Luby rounds with xorshift priorities (32-bit
