@@ -1,8 +1,9 @@
 // Nystrom factor on the device in FP64 (build_residual_backend,
 // covertreex/metrics/residual/host_backend.py:165-246):
 //   K_mm = k(U, U) + jitter * var * I           (208-210)
-//   L    = cholesky(K_mm)                       (211)
-//   V    = (L^-1 K_xm^T)^T, one row per thread  (213-215)
+//   L    = cholesky(K_mm)                       (211), blocked, one workgroup
+//   V    = (L^-1 K_xm^T)^T                      (213-215), blocked forward
+//          substitution in column blocks of 32, one row per thread
 //   p    = max(var - |V_i|^2, 1e-9)             (217-218)
 // then FP32 storage of V and p (236-240).  Lengthscales may be per dimension
 // (ARD); the reference supports a scalar only.
@@ -42,65 +43,178 @@ __global__ void kmm_kernel(const double *__restrict__ pts, const int64_t *__rest
     kmm[t] = v;
 }
 
-// Left-looking Cholesky of an m x m SPD matrix in place (lower triangle), one
-// workgroup.  status[0] = 1 when a pivot is not positive.
-__global__ void __launch_bounds__(1024) cholesky_kernel(double *__restrict__ a, int32_t m,
-                                                        int32_t *status)
+// Blocked left-looking Cholesky of an m x m SPD matrix in place (lower
+// triangle), one workgroup of 1024 threads, panels of kNb columns:
+//   1. the panel's top rows L[jb : jb+nb, 0 : jb] (final) go to LDS;
+//   2. panel update A[jb:m, jb:jb+nb] -= L[jb:m, 0:jb] L[jb:jb+nb, 0:jb]^T,
+//      one thread per (row, column) entry, rows read from global memory (the
+//      matrix stays in L2);
+//   3. the nb x nb diagonal block is factored in LDS (one column at a time);
+//   4. the rows below are solved against it by forward substitution, one
+//      thread per row.
+// The sums are those of column-by-column Cholesky in another order.
+// status[0] = 1 when a pivot is not positive.
+constexpr int kNb = 32;
+constexpr int kCholThreads = 1024;
+constexpr int kCholMaxM = 512;
+
+__global__ void __launch_bounds__(kCholThreads) cholesky_kernel(double *__restrict__ a, int32_t m,
+                                                                int32_t *status)
 {
-    __shared__ double s_piv;
-    for (int32_t j = 0; j < m; j++) {
-        if (threadIdx.x == 0) {
-            double s = a[(int64_t)j * m + j];
-            for (int32_t k = 0; k < j; k++) {
-                double l = a[(int64_t)j * m + k];
-                s -= l * l;
-            }
-            if (!(s > 0.0)) {
-                status[0] = 1;
-                s = 1.0;
-            }
-            s = __builtin_sqrt(s);
-            a[(int64_t)j * m + j] = s;
-            s_piv = s;
+    // top rows of the panel, padded by one double against bank conflicts
+    __shared__ double s_top[kNb * (kCholMaxM + 1)];
+    __shared__ double s_diag[kNb * (kNb + 1)];
+    const int tid = threadIdx.x;
+    const int ld = kCholMaxM + 1;
+    for (int32_t jb = 0; jb < m; jb += kNb) {
+        const int nb = m - jb < kNb ? m - jb : kNb;
+        for (int t = tid; t < nb * jb; t += kCholThreads) {
+            const int r = t / jb, k = t % jb;
+            s_top[r * ld + k] = a[(int64_t)(jb + r) * m + k];
         }
         __syncthreads();
-        const double piv = s_piv;
-        for (int32_t i = j + 1 + threadIdx.x; i < m; i += blockDim.x) {
-            double s = a[(int64_t)i * m + j];
-            for (int32_t k = 0; k < j; k++)
-                s -= a[(int64_t)i * m + k] * a[(int64_t)j * m + k];
-            a[(int64_t)i * m + j] = s / piv;
+        // 2. panel update over rows jb .. m-1, columns jb .. jb+nb-1
+        const int64_t rows = m - jb;
+        for (int64_t t = tid; t < rows * nb; t += kCholThreads) {
+            const int32_t i = jb + (int32_t)(t / nb), c = (int32_t)(t % nb);
+            if (i < jb + c)
+                continue; // above the diagonal of the block
+            const double *li = a + (int64_t)i * m;
+            double s = li[jb + c];
+            for (int32_t k = 0; k < jb; k++)
+                s -= li[k] * s_top[c * ld + k];
+            a[(int64_t)i * m + jb + c] = s;
+        }
+        __syncthreads();
+        // 3. factor the diagonal block in LDS
+        for (int t = tid; t < nb * nb; t += kCholThreads) {
+            const int r = t / nb, c = t % nb;
+            s_diag[r * (kNb + 1) + c] = a[(int64_t)(jb + r) * m + jb + c];
+        }
+        __syncthreads();
+        if (tid < 64) {
+            for (int j = 0; j < nb; j++) {
+                if (tid == 0) {
+                    double s = s_diag[j * (kNb + 1) + j];
+                    for (int k = 0; k < j; k++) {
+                        const double l = s_diag[j * (kNb + 1) + k];
+                        s -= l * l;
+                    }
+                    if (!(s > 0.0)) {
+                        status[0] = 1;
+                        s = 1.0;
+                    }
+                    s_diag[j * (kNb + 1) + j] = __builtin_sqrt(s);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                const double piv = s_diag[j * (kNb + 1) + j];
+                const int i = j + 1 + tid;
+                if (i < nb) {
+                    double s = s_diag[i * (kNb + 1) + j];
+                    for (int k = 0; k < j; k++)
+                        s -= s_diag[i * (kNb + 1) + k] * s_diag[j * (kNb + 1) + k];
+                    s_diag[i * (kNb + 1) + j] = s / piv;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            }
+        }
+        __syncthreads();
+        for (int t = tid; t < nb * nb; t += kCholThreads) {
+            const int r = t / nb, c = t % nb;
+            if (c <= r)
+                a[(int64_t)(jb + r) * m + jb + c] = s_diag[r * (kNb + 1) + c];
+        }
+        // 4. rows below the block: x L_bb^T = a  (forward substitution)
+        for (int32_t i = jb + nb + tid; i < m; i += kCholThreads) {
+            double *ri = a + (int64_t)i * m + jb;
+            double x[kNb];
+#pragma unroll
+            for (int c = 0; c < kNb; c++)
+                x[c] = c < nb ? ri[c] : 0.0;
+#pragma unroll
+            for (int c = 0; c < kNb; c++) {
+                if (c < nb) {
+                    double s = x[c];
+                    for (int k = 0; k < c; k++)
+                        s -= x[k] * s_diag[c * (kNb + 1) + k];
+                    x[c] = s / s_diag[c * (kNb + 1) + c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kNb; c++)
+                if (c < nb)
+                    ri[c] = x[c];
         }
         __syncthreads();
     }
 }
 
-// Forward substitution v = L^-1 k(x_i, U) for one row per thread; v lives in
-// a column-major scratch (vt[j * n + i]) so a wavefront's accesses coalesce,
-// while every thread of a wavefront reads the same L entry.
-__global__ void __launch_bounds__(256)
-    trsv_rows_kernel(const double *__restrict__ pts, int64_t n, int32_t d,
-                     const int64_t *__restrict__ ind, int32_t m, const double *__restrict__ inv_ls,
-                     double var, int32_t kernel, const double *__restrict__ L,
-                     double *__restrict__ vt, float *__restrict__ v_out, float *__restrict__ p_out)
+// lt[k * m + j] = L[j][k] for k <= j (L^T, upper triangle), so that a column
+// block's entries of one row k of L^T are contiguous
+__global__ void transpose_lower_kernel(const double *__restrict__ l, int32_t m,
+                                       double *__restrict__ lt)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m * m)
+        return;
+    const int32_t j = (int32_t)(t / m), k = (int32_t)(t % m);
+    lt[(int64_t)k * m + j] = k <= j ? l[t] : 0.0;
+}
+
+// One column block of V = K_xm L^-T, one row per thread: the block's kNb
+// columns c of row i start at k(x_i, u_c), take the earlier columns' terms
+// sum_{l < jb} V[i][l] L[c][l] (V read column-major from vt, L^T rows
+// contiguous and the same for the whole wavefront), then the block is solved
+// by forward substitution against L's diagonal block.  The row's squared norm
+// accumulates in FP64 across blocks; the last block writes p.
+__global__ void __launch_bounds__(64)
+    trsm_block_kernel(const double *__restrict__ pts, int64_t n, int32_t d,
+                      const int64_t *__restrict__ ind, int32_t m,
+                      const double *__restrict__ inv_ls, double var, int32_t kernel,
+                      const double *__restrict__ lt, int32_t jb, double *__restrict__ vt,
+                      double *__restrict__ norm2, float *__restrict__ v_out,
+                      float *__restrict__ p_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
+    const int nb = m - jb < kNb ? m - jb : kNb;
     const double *x = pts + i * d;
-    double norm2 = 0.0;
-    for (int32_t j = 0; j < m; j++) {
-        double s = kernel_f64(x, pts + ind[j] * d, inv_ls, d, var, kernel);
-        const double *lj = L + (int64_t)j * m;
-        for (int32_t l = 0; l < j; l++)
-            s -= lj[l] * vt[(int64_t)l * n + i];
-        s /= lj[j];
-        vt[(int64_t)j * n + i] = s;
-        v_out[i * m + j] = (float)s;
-        norm2 += s * s;
+    double acc[kNb];
+#pragma unroll
+    for (int c = 0; c < kNb; c++)
+        acc[c] = c < nb ? kernel_f64(x, pts + ind[jb + c] * d, inv_ls, d, var, kernel) : 0.0;
+    for (int32_t l = 0; l < jb; l++) {
+        const double v = vt[(int64_t)l * n + i];
+        const double *row = lt + (int64_t)l * m + jb;
+#pragma unroll
+        for (int c = 0; c < kNb; c++)
+            acc[c] -= v * (c < nb ? row[c] : 0.0);
     }
-    double p = var - norm2;
-    p_out[i] = (float)(p > 1e-9 ? p : 1e-9);
+    double s2 = jb > 0 ? norm2[i] : 0.0;
+#pragma unroll
+    for (int c = 0; c < kNb; c++) {
+        if (c < nb) {
+            double s = acc[c];
+            for (int k = 0; k < c; k++)
+                s -= acc[k] * lt[(int64_t)(jb + k) * m + jb + c];
+            s /= lt[(int64_t)(jb + c) * m + jb + c];
+            acc[c] = s;
+            vt[(int64_t)(jb + c) * n + i] = s;
+            v_out[i * m + jb + c] = (float)s;
+            s2 += s * s;
+        }
+    }
+    if (jb + nb < m) {
+        norm2[i] = s2;
+    } else {
+        const double p = var - s2;
+        p_out[i] = (float)(p > 1e-9 ? p : 1e-9);
+    }
 }
 
 void nystrom_device(const double *pts_host, int64_t n, int32_t d, const int64_t *ind_host,
@@ -128,7 +242,9 @@ void nystrom_device(const double *pts_host, int64_t n, int32_t d, const int64_t 
         double l = (ls_host && n_ls > 0) ? ls_host[c < n_ls ? c : 0] : 1.0;
         inv_ls[c] = 1.0 / (l > 1e-12 ? l : 1e-12); // max(lengthscale, 1e-12), 23-24
     }
-    DevBuf<double> pts((size_t)n * d), il(d), kmm((size_t)m * m), vt((size_t)n * m);
+    CX_REQUIRE(m <= kCholMaxM, "the Nystrom factor supports at most 512 inducing points");
+    DevBuf<double> pts((size_t)n * d), il(d), kmm((size_t)m * m), lt((size_t)m * m),
+        vt((size_t)n * m), norm2(n);
     DevBuf<int64_t> ind(m);
     DevBuf<float> v((size_t)n * m), p(n);
     DevBuf<int32_t> status(1);
@@ -139,9 +255,12 @@ void nystrom_device(const double *pts_host, int64_t n, int32_t d, const int64_t 
     const int64_t mm = (int64_t)m * m;
     kmm_kernel<<<(unsigned)((mm + 255) / 256), 256, 0, st>>>(pts.ptr, ind.ptr, m, d, il.ptr, var,
                                                             kernel, jitter * var, kmm.ptr);
-    cholesky_kernel<<<1, 1024, 0, st>>>(kmm.ptr, m, status.ptr);
-    trsv_rows_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
-        pts.ptr, n, d, ind.ptr, m, il.ptr, var, kernel, kmm.ptr, vt.ptr, v.ptr, p.ptr);
+    cholesky_kernel<<<1, kCholThreads, 0, st>>>(kmm.ptr, m, status.ptr);
+    transpose_lower_kernel<<<(unsigned)((mm + 255) / 256), 256, 0, st>>>(kmm.ptr, m, lt.ptr);
+    for (int32_t jb = 0; jb < m; jb += kNb)
+        trsm_block_kernel<<<(unsigned)((n + 63) / 64), 64, 0, st>>>(
+            pts.ptr, n, d, ind.ptr, m, il.ptr, var, kernel, lt.ptr, jb, vt.ptr, norm2.ptr, v.ptr,
+            p.ptr);
     CX_HIP(hipGetLastError());
     int32_t bad = 0;
     CX_HIP(hipMemcpyAsync(&bad, status.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, st));
