@@ -1333,6 +1333,7 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
 template <int KIND> static void finalize_tree(TreeState &t, const MetricView &mv, hipStream_t st)
 {
     const int64_t n = t.n;
+    t.eu_ready = false; // the exact Euclidean blocks follow the point set
     t.min_level = n > 0 ? -10 : -20;
     t.max_level = n > 0 ? 10 : 20;
     build_child_csr(t, st);

@@ -198,6 +198,17 @@ struct TreeState {
     DevBuf<uint8_t> codes;         // one byte per screened pair (a launch group)
     DevBuf<uint8_t> part;          // per-segment top-k lists of overflowed queries
 
+    // exact Euclidean kNN: the points in Hilbert order cut into blocks of 64
+    // with their bounding boxes (built on the first exact query)
+    bool eu_ready = false;
+    int64_t eu_nb = 0;
+    DevBuf<float> eu_pts;       // eu_nb * 64 x d, Hilbert order
+    DevBuf<int32_t> eu_idx;     // tree index of each sorted row, -1 past n
+    DevBuf<float> eu_lo, eu_hi; // block then superblock boxes, axis-major
+    DevBuf<int32_t> eu_min;     // smallest index per block then per superblock
+    DevBuf<uint64_t> eu_codes;  // sorted Hilbert codes (the seed search)
+    DevBuf<double> eu_bounds;   // per-axis quantisation bounds: lo[d], hi[d]
+
     // query telemetry: when enabled, kNN kernels add their counters into
     // telem (CX_QT_SIZE int64, zeroed per call; null when disabled) and the
     // entry point copies them to last_telem
@@ -280,6 +291,10 @@ void knn_euclidean(TreeState &t, const float *d_queries, int64_t nq, int32_t k, 
 void hilbert_order_device(const double *points_host, int64_t n, int32_t d, int64_t *order_host);
 void hilbert_order_noalloc(const double *d_points, int64_t n, int32_t d, int64_t *d_order,
                            hipStream_t st);
+// Hilbert codes of n FP32 points (sorted) and the stable sorting permutation;
+// d_lo / d_hi receive the per-axis bounds used for quantisation
+void hilbert_sort_points_f32(const float *d_pts, int64_t n, int32_t d, uint64_t *d_sorted_codes,
+                             int64_t *d_perm, double *d_lo, double *d_hi, hipStream_t st);
 int64_t luby_mis_device(const int64_t *indptr_host, const int64_t *indices_host, int64_t n,
                         uint32_t seed, uint8_t *sel_host);
 

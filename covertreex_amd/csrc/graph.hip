@@ -7,6 +7,7 @@
 #include <mutex>
 
 #include "cx_internal.hpp"
+#include "hilbert.hpp"
 
 namespace cx {
 
@@ -50,55 +51,45 @@ __global__ void hilbert_codes_kernel(const double *__restrict__ x, int64_t n, in
     if (i >= n)
         return;
     uint64_t v[64];
-    const uint64_t max_val = (1ull << bits) - 1;
-    for (int32_t c = 0; c < d; c++) {
-        double span = hi[c] - lo[c];
-        span = span > 1e-9 ? span : 1e-9; // np.maximum(hi - lo, 1e-9)
-        double s = (x[i * d + c] - lo[c]) / span;
-        double r = __builtin_rint(s * (double)max_val); // np.round: half to even
-        r = r < 0.0 ? 0.0 : r;
-        r = r > (double)max_val ? (double)max_val : r;
-        v[c] = (uint64_t)r;
-    }
-    // Skilling AxesToTranspose (strategy.py:63-88)
-    const uint64_t M = 1ull << (bits - 1);
-    for (uint64_t Q = M; Q > 1; Q >>= 1) {
-        const uint64_t P = Q - 1;
-        for (int32_t c = 0; c < d; c++) {
-            if (v[c] & Q) {
-                v[0] ^= P;
-            } else {
-                uint64_t t = (v[0] ^ v[c]) & P;
-                v[0] ^= t;
-                v[c] ^= t;
-            }
-        }
-    }
-    for (int32_t c = 1; c < d; c++)
-        v[c] ^= v[c - 1];
-    uint64_t t = 0;
-    for (uint64_t Q = M; Q > 1; Q >>= 1)
-        if (v[d - 1] & Q)
-            t ^= Q - 1;
     for (int32_t c = 0; c < d; c++)
-        v[c] ^= t;
-    // interleave, most significant bit first (strategy.py:91-99)
-    uint64_t code = 0;
-    for (int32_t b = bits - 1; b >= 0; b--) {
-        uint64_t word = 0;
-        for (int32_t c = 0; c < d; c++)
-            word = (word << 1) | ((v[c] >> b) & 1ull);
-        code = (code << d) | word;
-    }
-    codes[i] = code;
+        v[c] = hilbert_quantize(x[i * d + c], lo[c], hi[c], bits);
+    codes[i] = hilbert_from_quantized(v, d, bits);
     idx[i] = i;
+}
+
+__global__ void f32_to_f64_kernel(const float *__restrict__ a, int64_t count,
+                                  double *__restrict__ b)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        b[i] = (double)a[i];
+}
+
+void hilbert_sort_points_f32(const float *d_pts, int64_t n, int32_t d, uint64_t *d_sorted_codes,
+                             int64_t *d_perm, double *d_lo, double *d_hi, hipStream_t st)
+{
+    const int32_t bits = hilbert_bits(d);
+    DevBuf<double> x((size_t)n * d);
+    DevBuf<uint64_t> codes(n);
+    DevBuf<int64_t> idx(n);
+    const int64_t cnt = n * d;
+    f32_to_f64_kernel<<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(d_pts, cnt, x.ptr);
+    minmax_f64_kernel<<<(unsigned)d, 256, 0, st>>>(x.ptr, n, d, d_lo, d_hi);
+    hilbert_codes_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(x.ptr, n, d, bits, d_lo,
+                                                                      d_hi, codes.ptr, idx.ptr);
+    CX_HIP(hipGetLastError());
+    size_t tmp_bytes = 0;
+    CX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, codes.ptr, d_sorted_codes,
+                                              idx.ptr, d_perm, (int)n, 0, 64, st));
+    DevBuf<uint8_t> tmp(tmp_bytes);
+    CX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, codes.ptr, d_sorted_codes,
+                                              idx.ptr, d_perm, (int)n, 0, 64, st));
 }
 
 void hilbert_order_device(const double *points_host, int64_t n, int32_t d, int64_t *order_host)
 {
     hipStream_t st = nullptr;
-    int32_t bits = 64 / d;
-    bits = bits > 16 ? 16 : (bits < 1 ? 1 : bits);
+    const int32_t bits = hilbert_bits(d);
     DevBuf<double> x((size_t)n * d), lo(d), hi(d);
     DevBuf<uint64_t> codes(n), sorted(n);
     DevBuf<int64_t> idx(n), order(n);
@@ -170,8 +161,7 @@ void hilbert_order_noalloc(const double *d_points, int64_t n, int32_t d, int64_t
     CX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, codes, sorted, idx, d_order,
                                               (int)n, 0, 64, st));
     CX_REQUIRE(tmp_bytes <= kNoAllocSortBytes, "radix sort storage exceeds the scratch array");
-    int32_t bits = 64 / d;
-    bits = bits > 16 ? 16 : (bits < 1 ? 1 : bits);
+    const int32_t bits = hilbert_bits(d);
     minmax_f64_kernel<<<(unsigned)d, 256, 0, st>>>(d_points, n, d, lo, hi);
     hilbert_codes_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(d_points, n, d, bits, lo,
                                                                       hi, codes, idx);

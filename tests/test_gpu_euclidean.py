@@ -91,3 +91,88 @@ def test_factory_euclidean(hip):
     idx, dist = tree.knn(k=5, return_distances=True)
     assert idx.shape == (3000, 5)
     assert np.all(idx[:, 0] == np.arange(3000)) and np.all(dist[:, 0] == 0)
+
+
+def _fp32_fold_knn(pts, qs, k, pred=False):
+    """k smallest (d^2, index) pairs of the FP32 in-order fold
+    d^2 = (((q0-p0)^2 + (q1-p1)^2) + ...) by brute force; sqrt of d^2 out,
+    -1 / FLT_MAX past the available candidates."""
+    n = pts.shape[0]
+    acc = np.zeros((qs.shape[0], n), np.float32)
+    for c in range(pts.shape[1]):
+        df = qs[:, None, c] - pts[None, :, c]
+        acc = acc + df * df
+    idx = np.full((qs.shape[0], k), -1, np.int64)
+    dist = np.full((qs.shape[0], k), np.finfo(np.float32).max, np.float32)
+    for r in range(qs.shape[0]):
+        m = r if pred else n
+        order = np.lexsort((np.arange(m), acc[r, :m]))[:k]
+        idx[r, :order.size] = order
+        dist[r, :order.size] = np.sqrt(acc[r, order])
+    return idx, dist
+
+
+def _grid_points(n, d, seed):
+    # small integer grid: many equal distances, so ties decide the lists
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 6, (n, d)).astype(np.float32)
+
+
+@pytest.mark.parametrize("n,d,k,pred,kind", [
+    (20000, 8, 32, False, "gmm"),
+    (5000, 3, 50, True, "gmm"),
+    (37, 2, 64, False, "gmm"),        # one partial block, k above n
+    (4097, 5, 8, True, "grid"),       # ties everywhere, ragged last block
+    (3000, 16, 64, False, "uniform"),
+    (6000, 1, 10, False, "grid"),
+])
+def test_euclidean_exact_bitwise_vs_fp32_bruteforce(hip, n, d, k, pred, kind):
+    # the box-pruned search returns exactly the brute-force list of FP32 fold
+    # distances ordered by (d^2, index), bit for bit
+    from covertreex_amd.euclidean import EuclideanHipTree
+
+    rng = np.random.default_rng(n + d)
+    if kind == "gmm":
+        pts = _gmm(n, d, 16, seed=n)[0]
+    elif kind == "grid":
+        pts = _grid_points(n, d, seed=n)
+    else:
+        pts = rng.random((n, d)).astype(np.float32)
+    if pred:
+        qs = pts[:min(n, 700)]
+    else:
+        extra = (pts[rng.choice(n, 50)] if kind != "grid"
+                 else _grid_points(50, d, seed=n + 1))
+        far = (rng.standard_normal((20, d)) * 3).astype(np.float32)  # outside every box
+        qs = np.concatenate([_gmm(300, d, 16, seed=n + 2)[0], extra, far])
+    print(f"exact bitwise n={n} d={d} k={k} pred={pred} kind={kind}", flush=True)
+    t = EuclideanHipTree.build(pts)
+    idx, dist = t.knn_points(qs, k, predecessor_mode=pred, mode="exact")
+    ri, rd = _fp32_fold_knn(pts, qs, k, pred=pred)
+    assert np.array_equal(dist.view(np.uint32), rd.view(np.uint32))
+    assert np.array_equal(idx, ri)
+
+
+def test_euclidean_exact_follows_insert_and_delete(hip):
+    # the Hilbert blocks are rebuilt after each change of the point set
+    from covertreex_amd.euclidean import EuclideanHipTree
+
+    rng = np.random.default_rng(11)
+    pts = _gmm(3000, 4, 8, seed=11)[0]
+    more = _gmm(1500, 4, 8, seed=12)[0]
+    qs = _gmm(200, 4, 8, seed=13)[0]
+    t = EuclideanHipTree.build(pts, chunk_size=512)
+    idx, dist = t.knn_points(qs, 12, mode="exact")
+    ri, rd = _fp32_fold_knn(pts, qs, 12)
+    assert np.array_equal(idx, ri) and np.array_equal(dist.view(np.uint32), rd.view(np.uint32))
+    t.insert(more, chunk_size=512)
+    allp = np.concatenate([pts, more])
+    idx, dist = t.knn_points(qs, 12, mode="exact")
+    ri, rd = _fp32_fold_knn(allp, qs, 12)
+    assert np.array_equal(idx, ri) and np.array_equal(dist.view(np.uint32), rd.view(np.uint32))
+    rem = rng.choice(allp.shape[0], 400, replace=False)
+    t.delete(rem)
+    node_pts = np.delete(allp, rem, axis=0)[t.get_order()]  # node i's coordinates
+    idx, dist = t.knn_points(qs, 12, mode="exact")
+    ri, rd = _fp32_fold_knn(node_pts, qs, 12)
+    assert np.array_equal(idx, ri) and np.array_equal(dist.view(np.uint32), rd.view(np.uint32))
