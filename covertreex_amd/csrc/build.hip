@@ -15,6 +15,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "cx_internal.hpp"
 #include "device_common.hpp"
@@ -267,10 +268,12 @@ __global__ void __launch_bounds__(kBiResThreads)
 // the cover's child list, 64 positions per pass, and stores the result.  A
 // candidate that stayed at its cover scores only the children linked since
 // its previous scan (BiState::stay).
+// wave0 / nw: this wavefront's first work item and the stride (the number of
+// wavefronts sharing the scan).
 template <int KIND>
-__global__ void __launch_bounds__(kBiParThreads)
-    bi_scan_kernel(MetricView mv, TreeLinks t, BiState *S, const int32_t *__restrict__ rk,
-                   float r2, int64_t n_nodes)
+__device__ __forceinline__ void bi_scan_body(const MetricView &mv, const TreeLinks &t, BiState *S,
+                                             const int32_t *__restrict__ rk, float r2,
+                                             int64_t n_nodes, int64_t wave0, int64_t nw)
 {
     const int nc = S->nc;
     if (nc == 0)
@@ -279,9 +282,7 @@ __global__ void __launch_bounds__(kBiParThreads)
     const int nrk = S->nrk;
     const int R = (nrk + 64) / 64;
     const int64_t items = (int64_t)nc * R;
-    const int64_t nw = (int64_t)gridDim.x * kBiParWaves;
-    for (int64_t it = (int64_t)blockIdx.x * kBiParWaves + (threadIdx.x >> 6); it < items;
-         it += nw) {
+    for (int64_t it = wave0; it < items; it += nw) {
         const int i = (int)(it / R), g = (int)(it % R);
         const int64_t q = S->cand[i], p = S->cover[i];
         const bool st = S->stay[i] != 0;
@@ -359,6 +360,16 @@ __global__ void __launch_bounds__(kBiParThreads)
             S->shead[i] = (int32_t)first;
         }
     }
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kBiParThreads)
+    bi_scan_kernel(MetricView mv, TreeLinks t, BiState *S, const int32_t *__restrict__ rk,
+                   float r2, int64_t n_nodes)
+{
+    bi_scan_body<KIND>(mv, t, S, rk, r2, n_nodes,
+                       (int64_t)blockIdx.x * kBiParWaves + (threadIdx.x >> 6),
+                       (int64_t)gridDim.x * kBiParWaves);
 }
 
 // Level 9 of a residual batch: every candidate starts at the root, so the
@@ -475,19 +486,24 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Levels 9..0 under the residual metric.  d^2 = 1 - |clamp(rho)| lies in
-// [0, 1] <= 4^level for level >= 0, so at these levels every candidate is
-// near, no leader is linked and each candidate's descent depends only on the
-// tree as the batch found it: one workgroup per candidate runs the scans
-// back to back (the same positions, keys and first-minimum choice as
-// bi_scan_kernel), taking level 9 from bi_rootscan_kernel when the root's
-// list is long.  Once a scan chooses the cover itself, every later scan
-// down to level 0 would repeat it (same list, every d^2 within r^2), so they
-// are only counted.  The candidate is left at its level-0 cover with its
-// evaluations in ev_pre and its incremental-scan state for level -1.
+// Levels 9..bottom where every candidate is near: under the residual metric
+// d^2 = 1 - |clamp(rho)| lies in [0, 1] <= 4^level for level >= 0 (bottom 0);
+// under the Euclidean metric the FP32 d^2 of any two points is at most the
+// FP32 fold of the squared bounding-box edges (subtraction, squaring and
+// addition of non-negative terms are monotone under rounding), so every
+// level whose r^2 covers that bound qualifies.  At these levels no leader is
+// linked and each candidate's descent depends only on the tree as the batch
+// found it: one workgroup per candidate runs the scans back to back (the
+// same positions, keys and first-minimum choice as bi_scan_kernel), taking
+// level 9 from bi_rootscan_kernel when the root's list is long (residual).
+// Once a scan chooses the cover itself, every later scan down to bottom
+// would repeat it (same list, every d^2 within r^2), so they are only
+// counted.  The candidate is left at its cover for level bottom - 1 with its
+// evaluations in ev_pre and its incremental-scan state.
+template <int KIND>
 __global__ void __launch_bounds__(kBiParThreads)
     bi_descend_kernel(MetricView mv, TreeLinks t, BiState *S, const int32_t *__restrict__ rk,
-                      int64_t n_nodes)
+                      int64_t n_nodes, int32_t bottom)
 {
     __shared__ unsigned long long s_wmin[kBiParWaves];
     __shared__ int64_t s_cover, s_len, s_head;
@@ -501,7 +517,7 @@ __global__ void __launch_bounds__(kBiParThreads)
     const int nrk = S->nrk;
     int64_t p = 0, ev = 0;
     int32_t top = 9;
-    if (nrk >= kRtMin) {
+    if (KIND == kMetricResidual && nrk >= kRtMin) {
         // level 9 from bi_rootscan_kernel: a root child, or the root itself
         const int pos = (int)(uint32_t)S->key[i];
         ev = 1 + nrk;
@@ -509,7 +525,7 @@ __global__ void __launch_bounds__(kBiParThreads)
             if (tid == 0) {
                 S->cover[i] = 0;
                 S->key[i] = kBiFar;
-                S->ev_pre[i] = 10 * (int64_t)(1 + nrk);
+                S->ev_pre[i] = (int64_t)(10 - bottom) * (1 + nrk);
                 S->stay[i] = 1;
                 S->d0[i] = S->sd0[i];
                 S->plen[i] = nrk;
@@ -520,13 +536,13 @@ __global__ void __launch_bounds__(kBiParThreads)
         p = rk[nrk - pos];
         top = 8;
     }
-    for (int32_t level = top; level >= 0; --level) {
+    for (int32_t level = top; level >= bottom; --level) {
         const float r = ldexpf(1.0f, level), r2 = r * r;
         if (p == 0) {
             uint64_t best = kBiFar;
             for (int pos = tid; pos <= nrk; pos += kBiParThreads) {
                 const int64_t c = pos == 0 ? 0 : rk[nrk - pos];
-                const float dc = dist_sq_ref(mv, q, c);
+                const float dc = build_dist_sq<KIND>(mv, q, c);
                 if (pos == 0)
                     s_d0 = dc;
                 if (dc <= r2) {
@@ -568,7 +584,7 @@ __global__ void __launch_bounds__(kBiParThreads)
                 }
                 uint64_t key = kBiFar;
                 if (mine >= 0) {
-                    const float dc = dist_sq_ref(mv, q, mine);
+                    const float dc = build_dist_sq<KIND>(mv, q, mine);
                     if (base == 0 && lane == 0)
                         s_d0 = dc;
                     if (dc <= r2)
@@ -595,9 +611,9 @@ __global__ void __launch_bounds__(kBiParThreads)
         p = s_cover;
         const bool stayed = s_pos == 0;
         __syncthreads();
-        // this level and, when the cover stayed, levels level-1 .. 0 as well
-        ev += (stayed ? (int64_t)level + 1 : 1) * (1 + len);
-        if (stayed || level == 0) {
+        // this level and, when the cover stayed, levels level-1 .. bottom as well
+        ev += (stayed ? (int64_t)(level - bottom) + 1 : 1) * (1 + len);
+        if (stayed || level == bottom) {
             if (tid == 0) {
                 S->cover[i] = (int32_t)p;
                 S->key[i] = kBiFar;
@@ -612,15 +628,17 @@ __global__ void __launch_bounds__(kBiParThreads)
     }
 }
 
-// The far candidates, in candidate order, into s_far; returns their count.
-__device__ __forceinline__ int far_list_256(const BiState *S, int nc, int32_t *s_far, int *s_cnt)
+// The far candidates, in candidate order, into s_far; returns their count
+// (a workgroup of WAVES wavefronts).
+template <int WAVES>
+__device__ __forceinline__ int far_list(const BiState *S, int nc, int32_t *s_far, int *s_cnt)
 {
     int nf = 0;
-    for (int base = 0; base < nc; base += kBiParThreads) {
+    for (int base = 0; base < nc; base += WAVES * 64) {
         const int i = base + (int)threadIdx.x;
         const bool f = i < nc && S->key[i] == kBiFar;
         int tot;
-        const int pos = wg_prefix<kBiParWaves>(f, s_cnt, &tot);
+        const int pos = wg_prefix<WAVES>(f, s_cnt, &tot);
         if (f)
             s_far[nf + pos] = S->cand[i];
         nf += tot;
@@ -632,27 +650,29 @@ __device__ __forceinline__ int far_list_256(const BiState *S, int nc, int32_t *s
 // Conflicts among far candidates: for far indices i < j, dist[i][j] = d^2
 // and bit j of row i's conflict word = grid-adjacent || d^2 <= r^2, with the
 // grid bits alone in gadj.  Work items are (i, word w >= i / 64).
-template <int KIND>
-__global__ void __launch_bounds__(kBiParThreads)
-    bi_pairs_kernel(MetricView mv, BiState *S, const float *__restrict__ raw, int32_t gd,
-                    float gs, int32_t use_grid, float r2, float *__restrict__ dist,
-                    uint64_t *__restrict__ adj, uint64_t *__restrict__ gadj)
+// A workgroup of WAVES wavefronts; wave0 / nw as in bi_scan_body; stamp: this
+// workgroup records the pass's start tick.
+template <int KIND, int WAVES>
+__device__ __forceinline__ void bi_pairs_body(const MetricView &mv, BiState *S,
+                                              const float *__restrict__ raw, int32_t gd,
+                                              float gs, int32_t use_grid, float r2,
+                                              float *__restrict__ dist,
+                                              uint64_t *__restrict__ adj,
+                                              uint64_t *__restrict__ gadj, int32_t *s_far,
+                                              int *s_cnt, int wave0, int nw, bool stamp)
 {
-    __shared__ int32_t s_far[kBiMaxChunk];
-    __shared__ int s_cnt[kBiParWaves];
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (stamp && threadIdx.x == 0)
         S->tk_pairs = wall_clock64();
     const int nc = S->nc;
     if (nc < 2)
         return;
-    const int nf = far_list_256(S, nc, s_far, s_cnt);
+    const int nf = far_list<WAVES>(S, nc, s_far, s_cnt);
     if (nf < 2)
         return;
     const int lane = threadIdx.x & 63;
     const int W = (nf + 63) >> 6;
     const int items = nf * W;
-    const int nw = (int)gridDim.x * kBiParWaves;
-    for (int it = (int)blockIdx.x * kBiParWaves + (int)(threadIdx.x >> 6); it < items; it += nw) {
+    for (int it = wave0; it < items; it += nw) {
         const int i = it / W, w = it - i * W;
         if (w < (i >> 6))
             continue;
@@ -672,6 +692,19 @@ __global__ void __launch_bounds__(kBiParThreads)
             gadj[i * kBiWords + w] = gb;
         }
     }
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kBiParThreads)
+    bi_pairs_kernel(MetricView mv, BiState *S, const float *__restrict__ raw, int32_t gd,
+                    float gs, int32_t use_grid, float r2, float *__restrict__ dist,
+                    uint64_t *__restrict__ adj, uint64_t *__restrict__ gadj)
+{
+    __shared__ int32_t s_far[kBiMaxChunk];
+    __shared__ int s_cnt[kBiParWaves];
+    bi_pairs_body<KIND, kBiParWaves>(mv, S, raw, gd, gs, use_grid, r2, dist, adj, gadj, s_far,
+                                     s_cnt, (int)blockIdx.x * kBiParWaves + (int)(threadIdx.x >> 6),
+                                     (int)gridDim.x * kBiParWaves, blockIdx.x == 0);
 }
 
 // *acc (LDS) += the sum of v over the workgroup.
@@ -760,11 +793,12 @@ __device__ void link_in_order(TreeLinks t, int32_t *__restrict__ rk, const int32
 // -10 also the bottom survivors and the batch's counters (bs: its row of
 // kBatchFields, or null).
 template <int KIND>
-__global__ void __launch_bounds__(kBiResThreads)
-    bi_resolve_kernel(TreeLinks t, BiState *S, int32_t *__restrict__ rk,
-                      const float *__restrict__ dist, const uint64_t *__restrict__ adj,
-                      const uint64_t *__restrict__ gadj, int32_t level, float r2,
-                      int64_t *__restrict__ bs)
+__device__ __forceinline__ void bi_resolve_body(const TreeLinks &t, BiState *S,
+                                                int32_t *__restrict__ rk,
+                                                const float *__restrict__ dist,
+                                                const uint64_t *__restrict__ adj,
+                                                const uint64_t *__restrict__ gadj, int32_t level,
+                                                float r2, int64_t *__restrict__ bs)
 {
     __shared__ int32_t s_nxc[kBiMaxChunk], s_nxv[kBiMaxChunk]; // next level
     __shared__ int32_t s_nxst[kBiMaxChunk], s_nxpl[kBiMaxChunk], s_nxph[kBiMaxChunk];
@@ -977,6 +1011,51 @@ __global__ void __launch_bounds__(kBiResThreads)
             bs[13] = (int64_t)S->tk_nl;
             bs[14] = (int64_t)(now - S->tk_begin);
         }
+    }
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kBiResThreads)
+    bi_resolve_kernel(TreeLinks t, BiState *S, int32_t *__restrict__ rk,
+                      const float *__restrict__ dist, const uint64_t *__restrict__ adj,
+                      const uint64_t *__restrict__ gadj, int32_t level, float r2,
+                      int64_t *__restrict__ bs)
+{
+    bi_resolve_body<KIND>(t, S, rk, dist, adj, gadj, level, r2, bs);
+}
+
+// Levels top .. -10 of a batch in one workgroup: the scan, the conflict pass
+// and the resolve of each level back to back, separated by workgroup
+// barriers.  Launching three kernels per level costs more than the work of a
+// level once few candidates remain (and for the Euclidean metric at every
+// level).  The scan's atomic minima are made at the L2 cache, so after the
+// scan the vector L1 is invalidated (an agent-scope acquire) before the keys
+// are read; everything else stays within this workgroup's CU.  Levels with
+// no candidates left are skipped, except level -10, whose resolve records
+// the batch's counters.
+template <int KIND>
+__global__ void __launch_bounds__(kBiResThreads)
+    bi_levels_kernel(MetricView mv, TreeLinks t, BiState *S, int32_t *__restrict__ rk,
+                     const float *__restrict__ raw, int32_t gd, float gs, int32_t use_grid,
+                     float *__restrict__ dist, uint64_t *__restrict__ adj,
+                     uint64_t *__restrict__ gadj, int32_t top, int64_t *__restrict__ bs,
+                     int64_t n_nodes)
+{
+    __shared__ int32_t s_pfar[kBiMaxChunk];
+    __shared__ int s_pcnt[kBiResWaves];
+    const int wave = (int)(threadIdx.x >> 6);
+    for (int32_t level = top; level >= -10; level--) {
+        if (level > -10 && S->nc == 0)
+            continue;
+        const float r = ldexpf(1.0f, level), r2 = r * r;
+        bi_scan_body<KIND>(mv, t, S, rk, r2, n_nodes, wave, kBiResWaves);
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        bi_pairs_body<KIND, kBiResWaves>(mv, S, raw, gd, gs, use_grid, r2, dist, adj, gadj,
+                                         s_pfar, s_pcnt, wave, kBiResWaves, true);
+        __syncthreads();
+        bi_resolve_body<KIND>(t, S, rk, dist, adj, gadj, level, r2, bs);
+        __syncthreads();
     }
 }
 
@@ -1274,6 +1353,108 @@ static int32_t compute_depths(const int64_t *parents, int64_t n, int32_t *depth,
 
 // Runs the batch insert of rows [s_begin, s_end) (default: to t.n) and
 // appends the per-batch telemetry to t.batch_stats.
+// Highest level that a batch runs in one workgroup (bi_levels_kernel).  Its
+// scan walks the candidates' child lists with 16 wavefronts instead of one
+// wavefront per candidate, so it pays only where few candidates remain.
+// Under the residual metric levels 9..0 run in the descent kernels and most
+// far candidates (hundreds per batch) appear at levels -1 and -2, whose
+// conflict passes of rank-m dot products need the whole GPU; deeper levels
+// hold a few candidates each.  A Euclidean batch of clustered points keeps
+// most candidates down to the cluster scale and settles them there (levels
+// -4 .. -6 for clusters of width 0.02); a handful reach level -7.
+// COVERTREEX_HIP_FUSED_LEVEL overrides the choice (any level in [-11, 9];
+// -11 runs no level fused).
+static int32_t fused_build_level(bool euclidean)
+{
+    if (const char *e = getenv("COVERTREEX_HIP_FUSED_LEVEL")) {
+        const int v = atoi(e);
+        if (v >= -11 && v <= 9)
+            return v;
+    }
+    return euclidean ? -7 : -3;
+}
+
+// Per-axis minimum and maximum of the rows [blockIdx.x * rows_per, ...) of
+// pts (n x d): part[(c * 2 + 0) * gridDim.x + blockIdx.x] = min, +1 = max.
+__global__ void __launch_bounds__(256)
+    box_partial_kernel(const float *__restrict__ pts, int64_t n, int32_t d, int64_t rows_per,
+                       float *__restrict__ part)
+{
+    __shared__ float s_lo[256], s_hi[256];
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per;
+    const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
+    for (int32_t c = 0; c < d; c++) {
+        float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) {
+            const float v = pts[r * d + c];
+            lo = fminf(lo, v);
+            hi = fmaxf(hi, v);
+        }
+        s_lo[threadIdx.x] = lo;
+        s_hi[threadIdx.x] = hi;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) {
+                s_lo[threadIdx.x] = fminf(s_lo[threadIdx.x], s_lo[threadIdx.x + o]);
+                s_hi[threadIdx.x] = fmaxf(s_hi[threadIdx.x], s_hi[threadIdx.x + o]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            part[(2 * (int64_t)c) * gridDim.x + blockIdx.x] = s_lo[0];
+            part[(2 * (int64_t)c + 1) * gridDim.x + blockIdx.x] = s_hi[0];
+        }
+        __syncthreads();
+    }
+}
+
+// The FP32 fold over the axes of (hi - lo)^2 of the box of all points: no
+// FP32 squared distance between two of the points exceeds it.
+__global__ void box_diag_kernel(const float *__restrict__ part, int32_t nparts, int32_t d,
+                                float *__restrict__ out)
+{
+    float acc = 0.0f;
+    for (int32_t c = 0; c < d; c++) {
+        float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
+        for (int32_t b = 0; b < nparts; b++) {
+            lo = fminf(lo, part[(2 * (int64_t)c) * nparts + b]);
+            hi = fmaxf(hi, part[(2 * (int64_t)c + 1) * nparts + b]);
+        }
+        const float g = hi - lo;
+        acc = acc + g * g;
+    }
+    *out = acc;
+}
+
+// Lowest level in [-9, 9] whose r^2 = (2^level)^2 covers the FP32 squared
+// diameter bound of the tree's points, with every level above it covering it
+// too; 10 when level 9 does not.  Level -10 always runs the full level (its
+// resolve links the bottom survivors).
+static int32_t euclid_near_bottom(TreeState &t, hipStream_t st)
+{
+    const int64_t n = t.n;
+    if (n <= 0)
+        return 10;
+    const int64_t rows_per = 4096;
+    const int32_t nparts = (int32_t)((n + rows_per - 1) / rows_per);
+    DevBuf<float> part((size_t)nparts * 2 * t.d + 1);
+    box_partial_kernel<<<(unsigned)nparts, 256, 0, st>>>(t.raw.ptr, n, t.d, rows_per, part.ptr);
+    box_diag_kernel<<<1, 1, 0, st>>>(part.ptr, nparts, t.d, part.ptr + (size_t)nparts * 2 * t.d);
+    CX_HIP(hipGetLastError());
+    float d2 = 0.0f;
+    CX_HIP(hipMemcpyAsync(&d2, part.ptr + (size_t)nparts * 2 * t.d, sizeof(float),
+                          hipMemcpyDeviceToHost, st));
+    CX_HIP(hipStreamSynchronize(st));
+    int32_t bottom = 10;
+    for (int32_t level = 9; level >= -9; level--) {
+        const float r = ldexpf(1.0f, level), r2 = r * r;
+        if (!(d2 <= r2))
+            break;
+        bottom = level;
+    }
+    return bottom;
+}
+
 template <int KIND>
 static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, float grid_scale,
                        int64_t s_begin, int32_t chunk, hipStream_t st, int64_t s_end = -1)
@@ -1292,6 +1473,8 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
     DevBuf<uint64_t> adj((size_t)kBiMaxChunk * kBiWords), gadj((size_t)kBiMaxChunk * kBiWords);
     if (s_begin > 0)
         bi_root_kids_kernel<<<1, 1, 0, st>>>(links, state.ptr, rk.ptr, t.n);
+    const int32_t fused_from = fused_build_level(KIND == kMetricEuclidean);
+    const int32_t near_bottom = KIND == kMetricEuclidean ? euclid_near_bottom(t, st) : 10;
     for (int64_t b = 0; b < nb; b++) {
         const int64_t b0 = s_begin + b * chunk;
         const int64_t bn = n - b0 < chunk ? n - b0 : chunk;
@@ -1301,11 +1484,18 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
             const float r9 = ldexpf(1.0f, 9), r9sq = r9 * r9;
             bi_rootscan_kernel<<<dim3(128, (unsigned)((bn + kRtTile - 1) / kRtTile)), 256, 0,
                                  st>>>(mv, state.ptr, rk.ptr, r9sq);
-            bi_descend_kernel<<<(unsigned)bn, kBiParThreads, 0, st>>>(mv, links, state.ptr,
-                                                                     rk.ptr, t.n);
+            bi_descend_kernel<KIND><<<(unsigned)bn, kBiParThreads, 0, st>>>(
+                mv, links, state.ptr, rk.ptr, t.n, 0);
             top = -1;
+        } else if (near_bottom <= 9) {
+            bi_descend_kernel<KIND><<<(unsigned)bn, kBiParThreads, 0, st>>>(
+                mv, links, state.ptr, rk.ptr, t.n, near_bottom);
+            top = near_bottom - 1;
         }
-        for (int32_t level = top; level >= -10; level--) {
+        // levels above `fused` run as three launches each (all CUs), the
+        // rest in one workgroup
+        const int32_t fused = fused_from < top ? fused_from : top;
+        for (int32_t level = top; level > fused; level--) {
             const float r = ldexpf(1.0f, level), r2 = r * r;
             bi_scan_kernel<KIND>
                 <<<kBiParBlocks, kBiParThreads, 0, st>>>(mv, links, state.ptr, rk.ptr, r2, t.n);
@@ -1316,6 +1506,10 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
                 links, state.ptr, rk.ptr, dist.ptr, adj.ptr, gadj.ptr, level, r2,
                 bst.ptr + b * kBatchFields);
         }
+        if (fused >= -10)
+            bi_levels_kernel<KIND><<<1, kBiResThreads, 0, st>>>(
+                mv, links, state.ptr, rk.ptr, t.raw.ptr, t.d, grid_scale, use_grid, dist.ptr,
+                adj.ptr, gadj.ptr, fused, bst.ptr + b * kBatchFields, t.n);
         CX_HIP(hipGetLastError());
         t.live_batches.push_back(b0);
         t.live_batches.push_back(bn);

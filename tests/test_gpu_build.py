@@ -145,3 +145,25 @@ def test_c2_shape_build_and_reference_queries(hip):
     hi, hd = ht.knn_residual(q, 50, predecessor_mode=True, mode="reference")
     assert np.array_equal(hd.view(np.uint32), od.view(np.uint32))
     _same_sets_modulo_cut_ties(hi, hd, oi, od)
+
+
+@pytest.mark.parametrize("fused", ["-11", "-1", "-5"])
+@pytest.mark.parametrize("case", [BUILD_CASES[0], BUILD_CASES[1], BUILD_CASES[6]])
+def test_build_split_between_wide_and_fused_levels(hip, monkeypatch, case, fused):
+    # where the batch loop switches from three launches per level to the
+    # one-workgroup kernel does not change the tree (-11: no level fused,
+    # -1: every level below the descent fused)
+    monkeypatch.setenv("COVERTREEX_HIP_FUSED_LEVEL", fused)
+    print(f"case {case} fused from {fused}", flush=True)
+    x, v, p, ot, ht = _tree_pair(hip, case)
+    assert np.array_equal(ht.get_parents(), ot.parents)
+    assert np.array_equal(ht.get_children(), ot.children)
+    assert np.array_equal(ht.get_next_node(), ot.next)
+    assert np.array_equal(ht.get_levels(), ot.levels)
+    tel = ht.build_telemetry()
+    assert len(tel) == len(ot.batch_stats)
+    for h, o in zip(tel, ot.batch_stats):
+        assert (h["scan_evals"], h["mis_evals"], h["nonleader_evals"], h["leaders"],
+                h["orphans"], h["far_levels"]) == (
+            o["filter_evals"], o["conflict_pairs"], o["nonleader_evals"], o["leaders"],
+            o["orphans_dropped"], o["far_levels"])

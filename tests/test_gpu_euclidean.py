@@ -176,3 +176,38 @@ def test_euclidean_exact_follows_insert_and_delete(hip):
     idx, dist = t.knn_points(qs, 12, mode="exact")
     ri, rd = _fp32_fold_knn(node_pts, qs, 12)
     assert np.array_equal(idx, ri) and np.array_equal(dist.view(np.uint32), rd.view(np.uint32))
+
+
+@pytest.mark.parametrize("scale", [1e-3, 1.0, 300.0])
+def test_euclidean_tree_near_level_descent(hip, scale):
+    # levels whose radius covers the points' bounding box run in the descent
+    # kernel: many of them for a small box (down to level -9), none above 9
+    from covertreex_amd.euclidean import EuclideanHipTree
+
+    pts = (_gmm(2500, 4, 8, seed=9)[0] * np.float32(scale)).astype(np.float32)
+    pts[7] = pts[3]  # a duplicate point
+    ot = oracle.build_euclidean_tree(pts, chunk=256)
+    ht = EuclideanHipTree.build(pts, chunk_size=256)
+    assert np.array_equal(ht.get_parents(), ot.parents)
+    assert np.array_equal(ht.get_children(), ot.children)
+    assert np.array_equal(ht.get_next_node(), ot.next)
+    assert np.array_equal(ht.get_levels(), ot.levels)
+    tel = ht.build_telemetry()
+    assert [b["scan_evals"] for b in tel] == [b["filter_evals"] for b in ot.batch_stats]
+    assert [b["leaders"] for b in tel] == [b["leaders"] for b in ot.batch_stats]
+
+
+@pytest.mark.parametrize("fused", ["-11", "-3"])
+def test_euclidean_tree_wide_levels(hip, monkeypatch, fused):
+    # three launches per level for the levels above `fused` (all of them at
+    # -11), one workgroup below: the same tree as the oracle's
+    from covertreex_amd.euclidean import EuclideanHipTree
+
+    monkeypatch.setenv("COVERTREEX_HIP_FUSED_LEVEL", fused)
+    pts, _ = _gmm(3000, 8, 16, seed=3000)
+    ot = oracle.build_euclidean_tree(pts, chunk=512)
+    ht = EuclideanHipTree.build(pts, chunk_size=512)
+    assert np.array_equal(ht.get_parents(), ot.parents)
+    assert np.array_equal(ht.get_children(), ot.children)
+    assert np.array_equal(ht.get_next_node(), ot.next)
+    assert np.array_equal(ht.get_levels(), ot.levels)
