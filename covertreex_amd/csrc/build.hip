@@ -166,6 +166,7 @@ constexpr uint64_t kBiFar = ~0ull; // scan key of a candidate with no cover with
 
 struct BiState {
     int64_t s, nb;  // first row and size of the batch
+    int64_t bidx;   // batch index within the insert run (advanced at level -10)
     int32_t nc;     // candidates at the current level
     int32_t nrk;    // entries of the root's child array (oldest link first)
     int64_t ev_scan, ev_mis, ev_nl, orphans, e_grid, e_dist, leaders, far_levels;
@@ -225,10 +226,13 @@ __global__ void bi_root_kids_kernel(TreeLinks t, BiState *S, int32_t *__restrict
 // add_point(row, i32::MIN, -1) for the batch's rows (511-513); the root
 // bootstrap at level 10 when the batch starts at row 0 (521-531); every
 // candidate starts at cover 0 (539-540).
+// The batch's first row is s_base + bidx * chunk, so one recorded launch
+// sequence serves every full batch of the run.
 __global__ void __launch_bounds__(kBiResThreads)
-    bi_begin_kernel(TreeLinks t, BiState *S, int64_t s, int64_t nb)
+    bi_begin_kernel(TreeLinks t, BiState *S, int64_t s_base, int64_t chunk, int64_t nb)
 {
     const int tid = threadIdx.x;
+    const int64_t s = s_base + S->bidx * chunk;
     for (int i = tid; i < nb; i += kBiResThreads) {
         st_rx(&t.parents[s + i], (int64_t)-1);
         st_rx(&t.children[s + i], (int64_t)-1);
@@ -995,6 +999,7 @@ __device__ __forceinline__ void bi_resolve_body(const TreeLinks &t, BiState *S,
         const uint64_t now = wall_clock64();
         S->tk_mark = now;
         if (level == -10 && bs) {
+            bs += S->bidx * kBatchFields;
             bs[0] = S->s;
             bs[1] = S->nb;
             bs[2] = S->leaders;
@@ -1011,6 +1016,8 @@ __device__ __forceinline__ void bi_resolve_body(const TreeLinks &t, BiState *S,
             bs[13] = (int64_t)S->tk_nl;
             bs[14] = (int64_t)(now - S->tk_begin);
         }
+        if (level == -10)
+            S->bidx += 1;
     }
 }
 
@@ -1353,6 +1360,16 @@ static int32_t compute_depths(const int64_t *parents, int64_t n, int32_t *depth,
 
 // Runs the batch insert of rows [s_begin, s_end) (default: to t.n) and
 // appends the per-batch telemetry to t.batch_stats.
+// Runs of at least this many full batches replay a recorded graph.
+constexpr int64_t kGraphMinBatches = 3;
+
+// COVERTREEX_HIP_BUILD_GRAPHS=0 launches every batch directly.
+static bool build_graphs_enabled()
+{
+    const char *e = getenv("COVERTREEX_HIP_BUILD_GRAPHS");
+    return !(e && e[0] == '0');
+}
+
 // Highest level that a batch runs in one workgroup (bi_levels_kernel).  Its
 // scan walks the candidates' child lists with 16 wavefronts instead of one
 // wavefront per candidate, so it pays only where few candidates remain.
@@ -1475,10 +1492,10 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
         bi_root_kids_kernel<<<1, 1, 0, st>>>(links, state.ptr, rk.ptr, t.n);
     const int32_t fused_from = fused_build_level(KIND == kMetricEuclidean);
     const int32_t near_bottom = KIND == kMetricEuclidean ? euclid_near_bottom(t, st) : 10;
-    for (int64_t b = 0; b < nb; b++) {
-        const int64_t b0 = s_begin + b * chunk;
-        const int64_t bn = n - b0 < chunk ? n - b0 : chunk;
-        bi_begin_kernel<<<1, kBiResThreads, 0, st>>>(links, state.ptr, b0, bn);
+    CX_HIP(hipMemsetAsync(&state.ptr->bidx, 0, sizeof(int64_t), st));
+    // one batch of bn rows: every launch of levels 9 .. -10
+    auto launch_batch = [&](int64_t bn) {
+        bi_begin_kernel<<<1, kBiResThreads, 0, st>>>(links, state.ptr, s_begin, chunk, bn);
         int32_t top = 9;
         if constexpr (KIND == kMetricResidual) {
             const float r9 = ldexpf(1.0f, 9), r9sq = r9 * r9;
@@ -1503,16 +1520,51 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
                 mv, state.ptr, t.raw.ptr, t.d, grid_scale, use_grid, r2, dist.ptr, adj.ptr,
                 gadj.ptr);
             bi_resolve_kernel<KIND><<<1, kBiResThreads, 0, st>>>(
-                links, state.ptr, rk.ptr, dist.ptr, adj.ptr, gadj.ptr, level, r2,
-                bst.ptr + b * kBatchFields);
+                links, state.ptr, rk.ptr, dist.ptr, adj.ptr, gadj.ptr, level, r2, bst.ptr);
         }
         if (fused >= -10)
             bi_levels_kernel<KIND><<<1, kBiResThreads, 0, st>>>(
                 mv, links, state.ptr, rk.ptr, t.raw.ptr, t.d, grid_scale, use_grid, dist.ptr,
-                adj.ptr, gadj.ptr, fused, bst.ptr + b * kBatchFields, t.n);
+                adj.ptr, gadj.ptr, fused, bst.ptr, t.n);
+    };
+    // Full batches replay one recorded graph of the batch's launches (the
+    // host then submits a batch at a time instead of tens of kernels); the
+    // graph holds kernel nodes only.  The last, partial batch is launched
+    // directly.
+    const int64_t n_full = (n - s_begin) / chunk;
+    int64_t b = 0;
+    if (n_full >= kGraphMinBatches && build_graphs_enabled()) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        CX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        launch_batch(chunk);
+        const hipError_t launch_err = hipGetLastError();
+        const hipError_t cap_err = hipStreamEndCapture(st, &graph);
+        CX_HIP(launch_err);
+        CX_HIP(cap_err);
+        const hipError_t inst_err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        CX_HIP(inst_err);
+        for (; b < n_full; b++) {
+            const hipError_t e = hipGraphLaunch(exec, st);
+            if (e != hipSuccess) {
+                (void)hipStreamSynchronize(st);
+                (void)hipGraphExecDestroy(exec);
+                CX_HIP(e);
+            }
+        }
+        CX_HIP(hipStreamSynchronize(st)); // the graph's last replay has finished
+        CX_HIP(hipGraphExecDestroy(exec));
+    }
+    for (; b < nb; b++) {
+        const int64_t b0 = s_begin + b * chunk;
+        launch_batch(n - b0 < chunk ? n - b0 : chunk);
         CX_HIP(hipGetLastError());
+    }
+    for (b = 0; b < nb; b++) {
+        const int64_t b0 = s_begin + b * chunk;
         t.live_batches.push_back(b0);
-        t.live_batches.push_back(bn);
+        t.live_batches.push_back(n - b0 < chunk ? n - b0 : chunk);
     }
     const size_t old = t.batch_stats.size();
     t.batch_stats.resize(old + (size_t)nb * kBatchFields);

@@ -167,3 +167,21 @@ def test_build_split_between_wide_and_fused_levels(hip, monkeypatch, case, fused
                 h["orphans"], h["far_levels"]) == (
             o["filter_evals"], o["conflict_pairs"], o["nonleader_evals"], o["leaders"],
             o["orphans_dropped"], o["far_levels"])
+
+
+@pytest.mark.parametrize("graphs", ["0", "1"])
+def test_build_with_and_without_recorded_batches(hip, monkeypatch, graphs):
+    # full batches replay one recorded launch graph (or are launched one by
+    # one): the same tree and the same per-batch counters either way
+    monkeypatch.setenv("COVERTREEX_HIP_BUILD_GRAPHS", graphs)
+    case = dict(n=5000, d=3, m=64, kt=1, chunk=256, gs=1.0)
+    x, v, p, ot, ht = _tree_pair(hip, case)
+    assert np.array_equal(ht.get_parents(), ot.parents)
+    assert np.array_equal(ht.get_children(), ot.children)
+    assert np.array_equal(ht.get_next_node(), ot.next)
+    assert np.array_equal(ht.get_levels(), ot.levels)
+    tel = ht.build_telemetry()
+    assert [(b["start"], b["size"]) for b in tel] == [(o["start"], o["size"])
+                                                      for o in ot.batch_stats]
+    assert [b["scan_evals"] for b in tel] == [o["filter_evals"] for o in ot.batch_stats]
+    assert [b["leaders"] for b in tel] == [o["leaders"] for o in ot.batch_stats]
