@@ -167,6 +167,9 @@ constexpr uint64_t kBiFar = ~0ull; // scan key of a candidate with no cover with
 struct BiState {
     int64_t s, nb;  // first row and size of the batch
     int64_t bidx;   // batch index within the insert run (advanced at level -10)
+    // scan evaluations made inside bi_levels_kernel over the insert run (the
+    // host reads it to choose between the fused and the wide levels)
+    int64_t ev_fused, ev_fused_pairs;
     int32_t nc;     // candidates at the current level
     int32_t nrk;    // entries of the root's child array (oldest link first)
     int64_t ev_scan, ev_mis, ev_nl, orphans, e_grid, e_dist, leaders, far_levels;
@@ -1026,9 +1029,18 @@ __global__ void __launch_bounds__(kBiResThreads)
     bi_resolve_kernel(TreeLinks t, BiState *S, int32_t *__restrict__ rk,
                       const float *__restrict__ dist, const uint64_t *__restrict__ adj,
                       const uint64_t *__restrict__ gadj, int32_t level, float r2,
-                      int64_t *__restrict__ bs)
+                      int64_t *__restrict__ bs, int32_t track_from)
 {
+    // levels at or below track_from count towards the work the fused
+    // kernel would have had (thread 0 makes every update of these counters)
+    const bool track = level <= track_from && threadIdx.x == 0;
+    const int64_t ev0 = track ? S->ev_scan : 0;
+    const int64_t evp0 = track ? S->ev_mis + S->ev_nl : 0;
     bi_resolve_body<KIND>(t, S, rk, dist, adj, gadj, level, r2, bs);
+    if (track) {
+        S->ev_fused += S->ev_scan - ev0;
+        S->ev_fused_pairs += S->ev_mis + S->ev_nl - evp0;
+    }
 }
 
 // Levels top .. -10 of a batch in one workgroup: the scan, the conflict pass
@@ -1051,6 +1063,9 @@ __global__ void __launch_bounds__(kBiResThreads)
     __shared__ int32_t s_pfar[kBiMaxChunk];
     __shared__ int s_pcnt[kBiResWaves];
     const int wave = (int)(threadIdx.x >> 6);
+    // thread 0 updates ev_scan in every resolve, so its own read is current
+    const int64_t ev0 = threadIdx.x == 0 ? S->ev_scan : 0;
+    const int64_t evp0 = threadIdx.x == 0 ? S->ev_mis + S->ev_nl : 0;
     for (int32_t level = top; level >= -10; level--) {
         if (level > -10 && S->nc == 0)
             continue;
@@ -1063,6 +1078,10 @@ __global__ void __launch_bounds__(kBiResThreads)
         __syncthreads();
         bi_resolve_body<KIND>(t, S, rk, dist, adj, gadj, level, r2, bs);
         __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        S->ev_fused += S->ev_scan - ev0;
+        S->ev_fused_pairs += S->ev_mis + S->ev_nl - evp0;
     }
 }
 
@@ -1362,6 +1381,9 @@ static int32_t compute_depths(const int64_t *parents, int64_t n, int32_t *depth,
 // appends the per-batch telemetry to t.batch_stats.
 // Runs of at least this many full batches replay a recorded graph.
 constexpr int64_t kGraphMinBatches = 3;
+// Batches per segment between the host's reads of the fused levels' work
+// (the first segment is shorter and read at once).
+constexpr int64_t kSegBatches = 8, kFirstSegBatches = 2;
 
 // COVERTREEX_HIP_BUILD_GRAPHS=0 launches every batch directly.
 static bool build_graphs_enabled()
@@ -1389,6 +1411,27 @@ static int32_t fused_build_level(bool euclidean)
             return v;
     }
     return euclidean ? -7 : -3;
+}
+
+// Work of the fused kernel per batch, in row elements read (evaluations of
+// the scans, conflict pairs and non-leader checks, times the row length),
+// above which the rest of an insert runs every level wide.
+// COVERTREEX_HIP_FUSED_MAX_WORK overrides it (0: wide after the first
+// segments whose fused levels did any work).
+static int64_t fused_max_work()
+{
+    if (const char *e = getenv("COVERTREEX_HIP_FUSED_MAX_WORK")) {
+        const long long v = atoll(e);
+        if (v >= 0)
+            return (int64_t)v;
+    }
+    return 65536;
+}
+
+static bool fused_too_slow(int64_t d_scan, int64_t d_pairs, int64_t d_batches, int32_t dim,
+                           int64_t max_work)
+{
+    return (d_scan + d_pairs) * (int64_t)(dim + 8) > max_work * d_batches;
 }
 
 // Per-axis minimum and maximum of the rows [blockIdx.x * rows_per, ...) of
@@ -1493,8 +1536,10 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
     const int32_t fused_from = fused_build_level(KIND == kMetricEuclidean);
     const int32_t near_bottom = KIND == kMetricEuclidean ? euclid_near_bottom(t, st) : 10;
     CX_HIP(hipMemsetAsync(&state.ptr->bidx, 0, sizeof(int64_t), st));
-    // one batch of bn rows: every launch of levels 9 .. -10
-    auto launch_batch = [&](int64_t bn) {
+    CX_HIP(hipMemsetAsync(&state.ptr->ev_fused, 0, 2 * sizeof(int64_t), st));
+    // one batch of bn rows: every launch of levels 9 .. -10; with `fused`
+    // false every level below the descent runs as three launches
+    auto launch_batch = [&](int64_t bn, bool fused_mode) {
         bi_begin_kernel<<<1, kBiResThreads, 0, st>>>(links, state.ptr, s_begin, chunk, bn);
         int32_t top = 9;
         if constexpr (KIND == kMetricResidual) {
@@ -1511,7 +1556,7 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
         }
         // levels above `fused` run as three launches each (all CUs), the
         // rest in one workgroup
-        const int32_t fused = fused_from < top ? fused_from : top;
+        const int32_t fused = !fused_mode ? -11 : fused_from < top ? fused_from : top;
         for (int32_t level = top; level > fused; level--) {
             const float r = ldexpf(1.0f, level), r2 = r * r;
             bi_scan_kernel<KIND>
@@ -1520,47 +1565,124 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
                 mv, state.ptr, t.raw.ptr, t.d, grid_scale, use_grid, r2, dist.ptr, adj.ptr,
                 gadj.ptr);
             bi_resolve_kernel<KIND><<<1, kBiResThreads, 0, st>>>(
-                links, state.ptr, rk.ptr, dist.ptr, adj.ptr, gadj.ptr, level, r2, bst.ptr);
+                links, state.ptr, rk.ptr, dist.ptr, adj.ptr, gadj.ptr, level, r2, bst.ptr,
+                fused_from);
         }
         if (fused >= -10)
             bi_levels_kernel<KIND><<<1, kBiResThreads, 0, st>>>(
                 mv, links, state.ptr, rk.ptr, t.raw.ptr, t.d, grid_scale, use_grid, dist.ptr,
                 adj.ptr, gadj.ptr, fused, bst.ptr, t.n);
     };
-    // Full batches replay one recorded graph of the batch's launches (the
-    // host then submits a batch at a time instead of tens of kernels); the
-    // graph holds kernel nodes only.  The last, partial batch is launched
-    // directly.
+    // Full batches replay one recorded graph of the batch's launches per
+    // mode (the host then submits a batch at a time instead of tens of
+    // kernels); the graphs hold kernel nodes only.  The last, partial batch
+    // is launched directly.
     const int64_t n_full = (n - s_begin) / chunk;
-    int64_t b = 0;
-    if (n_full >= kGraphMinBatches && build_graphs_enabled()) {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        CX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        launch_batch(chunk);
-        const hipError_t launch_err = hipGetLastError();
-        const hipError_t cap_err = hipStreamEndCapture(st, &graph);
-        CX_HIP(launch_err);
-        CX_HIP(cap_err);
-        const hipError_t inst_err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        CX_HIP(inst_err);
-        for (; b < n_full; b++) {
-            const hipError_t e = hipGraphLaunch(exec, st);
-            if (e != hipSuccess) {
-                (void)hipStreamSynchronize(st);
-                (void)hipGraphExecDestroy(exec);
-                CX_HIP(e);
+    const bool graphs = n_full >= kGraphMinBatches && build_graphs_enabled();
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    auto drop_graphs = [&]() {
+        for (hipGraphExec_t &e : exec)
+            if (e) {
+                (void)hipGraphExecDestroy(e);
+                e = nullptr;
             }
+    };
+    auto launch_range = [&](int64_t b0, int64_t b1, bool fused_mode) {
+        for (int64_t b = b0; b < b1; b++) {
+            const int64_t r0 = s_begin + b * chunk;
+            if (!graphs || b >= n_full) {
+                launch_batch(n - r0 < chunk ? n - r0 : chunk, fused_mode);
+                CX_HIP(hipGetLastError());
+                continue;
+            }
+            hipGraphExec_t &e = exec[fused_mode ? 1 : 0];
+            if (!e) {
+                hipGraph_t graph = nullptr;
+                CX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                launch_batch(chunk, fused_mode);
+                const hipError_t launch_err = hipGetLastError();
+                const hipError_t cap_err = hipStreamEndCapture(st, &graph);
+                CX_HIP(launch_err);
+                CX_HIP(cap_err);
+                const hipError_t inst_err = hipGraphInstantiate(&e, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                CX_HIP(inst_err);
+            }
+            CX_HIP(hipGraphLaunch(e, st));
         }
-        CX_HIP(hipStreamSynchronize(st)); // the graph's last replay has finished
-        CX_HIP(hipGraphExecDestroy(exec));
+    };
+    // The fused levels pay while the candidates that reach them scan short
+    // lists.  Where they stay under covers with long child lists (smooth
+    // fields: correlations at rounding level keep most candidates at the
+    // root, whose list grows to tens of thousands), one workgroup scanning
+    // them is many times slower than the wide levels.  Batches therefore
+    // run in segments, each in the mode the work measured so far asks for:
+    // both modes count the work of the levels at or below fused_from, and
+    // after each segment the host reads it (one segment behind, so the
+    // device stays busy; the first segment of two batches is read at once).
+    // The tree is the same in either mode.
+    int64_t b = 0;
+    bool fused_mode = fused_from >= -10;
+    try {
+        if (!fused_mode) {
+            launch_range(0, nb, false);
+        } else {
+            const int64_t max_work = fused_max_work();
+            const int32_t dim = mv.m > 0 ? mv.m : mv.d;
+            int64_t *word[2] = {(int64_t *)acquire_host_word(), (int64_t *)acquire_host_word()};
+            hipEvent_t ev[2] = {acquire_event(), acquire_event()};
+            int64_t seg_end[2] = {-1, -1};
+            int64_t seen_scan = 0, seen_pairs = 0, seen_b = 0;
+            int slot = 0;
+            auto give_back = [&]() {
+                for (int i = 0; i < 2; i++) {
+                    (void)hipEventSynchronize(ev[i]);
+                    release_event(ev[i]);
+                    release_host_word((int32_t *)word[i]);
+                }
+            };
+            auto decide = [&](int s) {
+                CX_HIP(hipEventSynchronize(ev[s]));
+                const int64_t d_scan = word[s][0] - seen_scan, d_pairs = word[s][1] - seen_pairs;
+                const int64_t d_b = seg_end[s] - seen_b;
+                seen_scan = word[s][0];
+                seen_pairs = word[s][1];
+                seen_b = seg_end[s];
+                seg_end[s] = -1;
+                fused_mode = !fused_too_slow(d_scan, d_pairs, d_b, dim, max_work);
+            };
+            try {
+                while (b < nb) {
+                    const int64_t len = b == 0 ? kFirstSegBatches : kSegBatches;
+                    const int64_t e = b + len < nb ? b + len : nb;
+                    launch_range(b, e, fused_mode);
+                    b = e;
+                    if (b >= nb)
+                        break;
+                    CX_HIP(hipMemcpyAsync(word[slot], &state.ptr->ev_fused, 2 * sizeof(int64_t),
+                                          hipMemcpyDeviceToHost, st));
+                    CX_HIP(hipEventRecord(ev[slot], st));
+                    seg_end[slot] = b;
+                    if (b == len) {
+                        decide(slot);
+                    } else if (seg_end[slot ^ 1] >= 0) {
+                        decide(slot ^ 1); // the segment before: finished or about to
+                    }
+                    slot ^= 1;
+                }
+            } catch (...) {
+                give_back();
+                throw;
+            }
+            give_back();
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        drop_graphs();
+        throw;
     }
-    for (; b < nb; b++) {
-        const int64_t b0 = s_begin + b * chunk;
-        launch_batch(n - b0 < chunk ? n - b0 : chunk);
-        CX_HIP(hipGetLastError());
-    }
+    CX_HIP(hipStreamSynchronize(st)); // the graphs' last replays have finished
+    drop_graphs();
     for (b = 0; b < nb; b++) {
         const int64_t b0 = s_begin + b * chunk;
         t.live_batches.push_back(b0);

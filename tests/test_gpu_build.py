@@ -185,3 +185,24 @@ def test_build_with_and_without_recorded_batches(hip, monkeypatch, graphs):
                                                       for o in ot.batch_stats]
     assert [b["scan_evals"] for b in tel] == [o["filter_evals"] for o in ot.batch_stats]
     assert [b["leaders"] for b in tel] == [o["leaders"] for o in ot.batch_stats]
+
+
+@pytest.mark.parametrize("max_work", ["0", "3000", "1000000000000000"])
+@pytest.mark.parametrize("case", [BUILD_CASES[0], BUILD_CASES[6],
+                                  dict(n=6000, d=3, m=32, kt=0, chunk=128, gs=1.0)])
+def test_build_fused_and_wide_segments(hip, monkeypatch, case, max_work):
+    # batches run in segments whose mode (fused or wide levels) follows the
+    # work measured so far; the threshold moves the switches (0: wide after
+    # the first segment that did any fused-level work; huge: always fused),
+    # never the tree or the per-batch counters
+    monkeypatch.setenv("COVERTREEX_HIP_FUSED_MAX_WORK", max_work)
+    print(f"case {case} max work {max_work}", flush=True)
+    x, v, p, ot, ht = _tree_pair(hip, case)
+    assert np.array_equal(ht.get_parents(), ot.parents)
+    assert np.array_equal(ht.get_children(), ot.children)
+    assert np.array_equal(ht.get_next_node(), ot.next)
+    assert np.array_equal(ht.get_levels(), ot.levels)
+    tel = ht.build_telemetry()
+    assert [(b["scan_evals"], b["mis_evals"], b["nonleader_evals"], b["leaders"])
+            for b in tel] == [(o["filter_evals"], o["conflict_pairs"], o["nonleader_evals"],
+                               o["leaders"]) for o in ot.batch_stats]

@@ -211,3 +211,22 @@ def test_euclidean_tree_wide_levels(hip, monkeypatch, fused):
     assert np.array_equal(ht.get_children(), ot.children)
     assert np.array_equal(ht.get_next_node(), ot.next)
     assert np.array_equal(ht.get_levels(), ot.levels)
+
+
+@pytest.mark.parametrize("max_work", ["0", "1000", "1000000000000000"])
+def test_euclidean_tree_fused_and_wide_segments(hip, monkeypatch, max_work):
+    # segments of batches switch between fused and wide levels on the work
+    # measured so far: the same tree and counters whatever the threshold
+    from covertreex_amd.euclidean import EuclideanHipTree
+
+    monkeypatch.setenv("COVERTREEX_HIP_FUSED_MAX_WORK", max_work)
+    pts, _ = _gmm(12000, 8, 16, seed=12)
+    ot = oracle.build_euclidean_tree(pts, chunk=256)
+    ht = EuclideanHipTree.build(pts, chunk_size=256)
+    assert np.array_equal(ht.get_parents(), ot.parents)
+    assert np.array_equal(ht.get_children(), ot.children)
+    assert np.array_equal(ht.get_next_node(), ot.next)
+    assert np.array_equal(ht.get_levels(), ot.levels)
+    tel = ht.build_telemetry()
+    assert [b["scan_evals"] for b in tel] == [b["filter_evals"] for b in ot.batch_stats]
+    assert [b["leaders"] for b in tel] == [b["leaders"] for b in ot.batch_stats]
