@@ -10,6 +10,13 @@ wavefront per query) and the C oracle's restatement of the reference search on
 the host cores over a bounded query sample.
 
 Prints ONE JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+
+`--workload c1|c3|c4` measures one of the other workloads instead (its own
+JSON line, no CPU leg), for profiling: c1 the 4,096-point, rank-64, k=16 query
+of every point (not predecessor); c3 the Euclidean build of 1,048,576 points
+in D=8 from a 64-component Gaussian mixture and the exact kNN (k=32) of 65,536
+separate queries; c4 the 262,144-point, D=5, anisotropic-RBF, rank-512, k=30
+predecessor query of every point (with its Nystrom factor and build timed).
 """
 
 from __future__ import annotations
@@ -127,6 +134,93 @@ def measure_peaks(torch, dev) -> dict:
     return out
 
 
+def _timed(fn, steps, torch):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def other_workload(name: str, steps: int, warmup: int) -> dict:
+    """The c1, c3 and c4 lines (see the module docstring)."""
+    import torch
+
+    from covertreex_amd.native import HipTree
+
+    dev = torch.device("cuda:0")
+    if name == "c3":
+        from covertreex_amd.euclidean import EuclideanHipTree
+
+        centres = np.random.default_rng(0).random((64, 8))
+
+        def mixture(n, seed):
+            rng = np.random.default_rng(seed)
+            lab = rng.integers(0, 64, n)
+            return (centres[lab] + 0.02 * rng.standard_normal((n, 8))).astype(np.float32)
+
+        pts, qs = mixture(1 << 20, 0), mixture(65536, 1)
+        t0 = time.perf_counter()
+        tree = EuclideanHipTree.build(pts, chunk_size=512)
+        build_s = time.perf_counter() - t0
+        for _ in range(warmup):
+            tree.knn_points(qs, 32, mode="exact")
+        el = _timed(lambda: tree.knn_points(qs, 32, mode="exact"), steps, torch)
+        return {"metric": "euclidean exact kNN queries/s @N=1M,D=8,k=32 (host buffers)",
+                "value": round(qs.shape[0] / el, 1), "unit": "queries/s",
+                "ms_per_step": round(el * 1e3, 3), "steps": steps, "warmup": warmup,
+                "build_s": round(build_s, 3), "dtype": "f32",
+                "config": {"workload": "euclidean GMM 64 components sigma 0.02: N=1048576 "
+                                       "D=8, 65536 separate queries, k=32, batches of 512"}}
+    from covertreex_amd.residual import build_residual_backend
+
+    if name == "c1":
+        n, d, m, k, pred, kt = 4096, 3, 64, 16, False, 1
+        x = np.random.default_rng(0).standard_normal((n, d)).astype(np.float32)
+        ls = np.ones(1, np.float32)
+    else:
+        n, d, m, k, pred, kt = 262144, 5, 512, 30, True, 0
+        x = np.random.default_rng(0).uniform(-3, 3, (n, d)).astype(np.float32)
+        ls = np.exp(np.random.default_rng(0).uniform(np.log(0.5), np.log(2.0), d))
+    t0 = time.perf_counter()
+    b = build_residual_backend(x, seed=0, inducing_count=m, variance=1.0, lengthscale=ls,
+                               kernel_type=kt)
+    nystrom_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tree = HipTree.build_residual(b.v_matrix, b.p_diag, x, 1.0, np.asarray(ls, np.float32),
+                                  kernel_type=kt)
+    build_s = time.perf_counter() - t0
+    q = torch.arange(n, dtype=torch.int64, device=dev)
+    o_idx = torch.empty((n, k), dtype=torch.int64, device=dev)
+    o_dst = torch.empty((n, k), dtype=torch.float32, device=dev)
+    o_err = torch.empty((n, k), dtype=torch.float32, device=dev)
+
+    def step():
+        tree.knn_residual_device(q.data_ptr(), n, k, o_idx.data_ptr(), o_dst.data_ptr(),
+                                 o_err.data_ptr(), predecessor_mode=pred, mode="mixed")
+
+    for _ in range(warmup):
+        step()
+    el = _timed(step, steps, torch)
+    valid = o_idx >= 0
+    tree.set_profiling(True)
+    step()
+    phases = dict(zip(HipTree.PHASES["mixed"], tree.last_profile()))
+    tree.set_profiling(False)
+    label = ("residual kNN queries/s @N=4096,D=3,k=16" if name == "c1"
+             else "residual kNN queries/s @N=262144,D=5,k=30 predecessor")
+    return {"metric": label, "value": round(n / el, 1), "unit": "queries/s",
+            "ms_per_step": round(el * 1e3, 3), "steps": steps, "warmup": warmup,
+            "nystrom_s": round(nystrom_s, 3), "build_s": round(build_s, 3),
+            "max_err_bound": float(o_err[valid].max().item()),
+            "phases_ms": {k_: round(v_, 4) for k_, v_ in phases.items()},
+            "dtype": "bf16-hilo-mfma/f32-compensated",
+            "config": {"workload": (f"residual N={n} D={d} m={m} k={k} "
+                                    f"{'predecessor' if pred else 'all neighbours'} "
+                                    f"{'Matern52 l=1' if kt == 1 else 'ARD RBF'}")}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +232,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=15.0)
     ap.add_argument("--serve-inflight", type=int, default=4)
+    ap.add_argument("--workload", choices=("c2", "c1", "c3", "c4"), default="c2",
+                    help="c2 (default) is the headline line; c1/c3/c4 print their own")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -159,6 +255,10 @@ def main() -> int:
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}))
+        return 0
+
+    if args.workload != "c2":
+        print(json.dumps(other_workload(args.workload, args.steps, args.warmup)))
         return 0
 
     from covertreex_amd.native import HipTree

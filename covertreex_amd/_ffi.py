@@ -81,6 +81,8 @@ SIGNATURES = {
     "cx_nystrom_residual": (_i32, [_p, _i64, _i32, _p, _i32, C.c_double, _p, _i32, _i32,
                                    C.c_double, _p, _p]),
     "cx_tree_set_profiling": (_i32, [_p, _i32]),
+    "cx_tree_set_screen_segment": (_i32, [_p, _i32]),
+    "cx_tree_get_screen_segment": (_i32, [_p, _p]),
     "cx_tree_last_profile": (_i32, [_p, _p, _i32, _p]),
     "cx_tree_insert_residual": (_i32, [_p, _p, _p, _p, _i64, _i32]),
     "cx_tree_insert_euclidean": (_i32, [_p, _p, _i64, _i32]),

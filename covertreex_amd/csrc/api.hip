@@ -534,6 +534,26 @@ cx_status cx_nystrom_residual(const double *points, int64_t n, int32_t d,
     });
 }
 
+cx_status cx_tree_set_screen_segment(cx_tree *t, int32_t seg)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr, "tree is NULL");
+        CX_REQUIRE(screen_segment_valid(seg),
+                   "screen segment must be 0 or a multiple of 256 in [256, 16384]");
+        std::lock_guard<std::mutex> lock(t->s.mu);
+        t->s.screen_seg = seg;
+    });
+}
+
+cx_status cx_tree_get_screen_segment(cx_tree *t, int32_t *seg_out)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr && seg_out != nullptr, "NULL argument");
+        std::lock_guard<std::mutex> lock(t->s.mu);
+        *seg_out = t->s.last_seg;
+    });
+}
+
 cx_status cx_tree_set_profiling(cx_tree *t, int32_t enable)
 {
     return guarded([&] {

@@ -193,6 +193,8 @@ struct TreeState {
     DevBuf<uint16_t> vsw;          // npad x mp bf16 hi and lo, MFMA-fragment layout
     int32_t mp = 0;                // m padded to a multiple of 32
     int32_t mps = 0;               // m padded to the screen's K chunk (64)
+    int32_t screen_seg = 0;        // candidates per screen segment (0: default)
+    int32_t last_seg = 0;          // the segment length of the last mixed query
     DevBuf<float> vnorm;           // |V_i| rounded up, dataset order
     DevBuf<float> x_ds, isp_ds;    // raw coords and 1/sqrt(p) in dataset order
     DevBuf<uint8_t> codes;         // one byte per screened pair (a launch group)
@@ -280,6 +282,11 @@ void knn_reference_f32(TreeState &t, const int64_t *d_queries, int64_t nq, int32
                        float *d_err);
 
 // knn_mixed.hip
+// Candidates per screen segment of the mixed engine for nq queries on this
+// tree, and whether a requested value is acceptable (0 or a multiple of 256
+// in [256, 16384]).
+int32_t screen_segment(const TreeState &t, int64_t nq);
+bool screen_segment_valid(int32_t seg);
 void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k,
                int32_t predecessor, int64_t *d_idx, float *d_dist, float *d_err);
 

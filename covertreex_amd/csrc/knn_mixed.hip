@@ -54,7 +54,9 @@ constexpr int kCapBig = 8192;
 constexpr int64_t kBigListBudget = (int64_t)1 << 30; // bytes of those lists per call
 constexpr int kMaxD = 16;
 constexpr int kMixMaxK = 256;
-constexpr int64_t kSeg = 8 * kCT; // candidates per workgroup (grid y)
+// candidates per workgroup of the screen (a segment; grid y of the filter):
+// a multiple of kCT in [kSegMin, kSegMax], chosen per tree (see screen_segment)
+constexpr int32_t kSegMin = kCT, kSegMax = 64 * kCT;
 constexpr int kScreenWaves = 4;
 constexpr int kScreenThreads = 64 * kScreenWaves;
 // Workgroup b of a launch runs on XCD b % kXcds (round-robin dispatch)
@@ -166,6 +168,7 @@ struct ScreenArgs {
     // this launch covers query tiles [tile0, tile0 + n_tiles) x all segments;
     // codes hold one byte per (candidate, query) pair of those tiles
     int32_t tile0, n_tiles, n_seg;
+    int32_t seg; // candidates per segment
     uint32_t *codes;
 };
 
@@ -187,9 +190,10 @@ template <int KT> __device__ __forceinline__ float fast_kernel(float d2, const S
 // the wavefront's candidate rows 32 rb + 8 g + 4 (lane >> 5) .. + 3 (one byte
 // each) against query column 32 b + (lane & 31), so a wavefront stores and
 // reloads 4 KiB contiguous per candidate tile.
-__device__ __forceinline__ int64_t code_word(int64_t blk, int ct, int wid, int lane)
+__device__ __forceinline__ int64_t code_word(const ScreenArgs &A, int64_t blk, int ct, int wid,
+                                             int lane)
 {
-    return (((blk * (kSeg / kCT) + ct) * kScreenWaves + wid) * 64 + lane) * 16;
+    return (((blk * (A.seg / kCT) + ct) * kScreenWaves + wid) * 64 + lane) * 16;
 }
 
 // Per-lane query columns of a query tile (shared by the screen and the
@@ -245,9 +249,9 @@ __device__ __forceinline__ void tile_range(const ScreenArgs &A, int64_t s0, int3
         const int64_t qmax = A.qsorted[last];
         e = qmax < 0 ? A.n : (qmax < A.n ? (int32_t)qmax : A.n);
     }
-    const int32_t b = seg * (int32_t)kSeg;
+    const int32_t b = seg * A.seg;
     *c_begin = b;
-    *c_end = e < b + (int32_t)kSeg ? e : b + (int32_t)kSeg;
+    *c_end = e < b + A.seg ? e : b + A.seg;
 }
 
 // Query rows of a K chunk: 64 rows x kKC bf16, hi and lo, as 16-byte pieces
@@ -305,7 +309,7 @@ __global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
     __shared__ int32_t sQrow[kQT];
     // histogram: one 32-bit counter per bin when d is fixed (d <= 5 leaves
     // room for two workgroups per CU), else two 16-bit bins per word (at
-    // most kSeg counts per bin); rows padded by one word
+    // most kSegMax counts per bin); rows padded by one word
     constexpr bool kWide = DD > 0;
     constexpr int kRow = kWide ? kBins + 1 : kHistRow;
     __shared__ uint32_t sHist[kQT * kRow];
@@ -564,7 +568,7 @@ __global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
                         words[(4 * rb + g) * 2 + b] = word;
                     }
                 }
-            uint4 *cdst = (uint4 *)(A.codes + code_word(blk, ct, wid, lane));
+            uint4 *cdst = (uint4 *)(A.codes + code_word(A, blk, ct, wid, lane));
 #pragma unroll
             for (int i = 0; i < 4; i++)
                 cdst[i] = make_uint4(words[4 * i], words[4 * i + 1], words[4 * i + 2],
@@ -630,7 +634,7 @@ __global__ void __launch_bounds__(kScreenThreads) filter_kernel(ScreenArgs A)
         const int32_t cw0 = c0 + kCW * wid;
         if (cw0 >= c_end)
             continue;
-        const uint4 *csrc = (const uint4 *)(A.codes + code_word(blk, ct, wid, lane));
+        const uint4 *csrc = (const uint4 *)(A.codes + code_word(A, blk, ct, wid, lane));
         uint32_t words[16];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -1974,16 +1978,46 @@ constexpr int64_t kCodeBudget = (int64_t)4 << 30;
 // segment top-k lists of the hard queries per launch batch (bytes)
 constexpr int64_t kPartBudget = (int64_t)512 << 20;
 
+// Candidates per screen segment for nq queries.  Each XCD streams a
+// segment's candidate fragments (hi and lo, 4 * mp bytes per row) for every
+// query tile while they sit in its L2 (4 MiB), so a segment holds about
+// 2 MiB of them: 2 MiB / (4 mp) rows, a power of two in [kSegMin, 4096]
+// (mp 64 and 128: 4096, 256: 2048, 512: 1024); it is halved (down to
+// kSegMin) while the grid would hold fewer than 512 workgroups, so small
+// calls still spread over the CUs.  These are the picks of the tuner
+// (covertreex_amd/tuner.py) on the workload shapes.  A tree's own choice
+// (cx_tree_set_screen_segment) overrides the rule.
+int32_t screen_segment(const TreeState &t, int64_t nq)
+{
+    if (t.screen_seg > 0)
+        return t.screen_seg;
+    const int64_t rows = ((int64_t)2 << 20) / (4 * (int64_t)std::max(t.mps, 1));
+    int32_t seg = kSegMin;
+    while (seg * 2 <= rows && seg * 2 <= 4096)
+        seg *= 2;
+    const int64_t tiles = (nq + kQT - 1) / kQT;
+    while (seg > kSegMin && tiles * ((t.n + seg - 1) / seg) < 512)
+        seg /= 2;
+    return seg;
+}
+
+bool screen_segment_valid(int32_t seg)
+{
+    return seg == 0 || (seg >= kSegMin && seg <= kSegMax && seg % kCT == 0);
+}
+
 void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, int32_t predecessor,
                int64_t *d_idx, float *d_dist, float *d_err)
 {
     CX_REQUIRE(k >= 1 && k <= kMixMaxK, "mixed-precision kNN supports 1 <= k <= 256");
     CX_REQUIRE(t.d <= kMaxD, "mixed-precision kNN supports d <= 16");
-    CX_REQUIRE(t.n < ((int64_t)1 << 31) - 4 * kSeg,
-               "mixed-precision kNN supports fewer than 2^31 - 8192 points");
-    // the screen's grid y counts 2048-candidate segments (at most 65535)
-    CX_REQUIRE(t.n <= (int64_t)65535 * kSeg,
-               "mixed-precision kNN supports at most 65535 * 2048 points");
+    const int32_t seg_len = screen_segment(t, nq);
+    t.last_seg = seg_len;
+    CX_REQUIRE(t.n < ((int64_t)1 << 31) - 4 * (int64_t)kSegMax,
+               "mixed-precision kNN supports fewer than 2^31 - 65536 points");
+    // the filter's grid y counts segments (at most 65535)
+    CX_REQUIRE(t.n <= (int64_t)65535 * seg_len,
+               "mixed-precision kNN supports at most 65535 segments of candidates");
     if (nq <= 0)
         return;
     prof_mark(t, 0);
@@ -2092,13 +2126,14 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     sa.cnt = cnt;
     sa.list = list;
     const int32_t n_tiles = (int32_t)((nq + kQT - 1) / kQT);
-    sa.n_seg = (int32_t)((n + kSeg - 1) / kSeg);
+    sa.seg = seg_len;
+    sa.n_seg = (int32_t)((n + seg_len - 1) / seg_len);
     // per wavefront: its candidates' coordinates, 1/sqrt(p) and norms; the
     // query coordinates
     const size_t dyn_lds =
         sizeof(float) * ((size_t)kScreenWaves * (t.d + 2) * kCW + (size_t)t.d * kQT);
     // launch groups of query tiles whose pair codes fit the code budget
-    const int64_t tile_bytes = (int64_t)sa.n_seg * kSeg * kQT;
+    const int64_t tile_bytes = (int64_t)sa.n_seg * seg_len * kQT;
     const int32_t group =
         (int32_t)std::max<int64_t>(1, std::min<int64_t>(n_tiles, kCodeBudget / tile_bytes));
     if (t.codes.count < (size_t)(group * tile_bytes)) {

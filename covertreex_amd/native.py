@@ -121,6 +121,19 @@ class HipTree:
                         "hard"],
               "reference": ["traversal"], "reference-budgeted": ["traversal"]}
 
+    def set_screen_segment(self, seg: int = 0) -> None:
+        """Candidates per screen segment of the mixed engine (0: default).
+        Results do not depend on it; it is the tile shape the tuner sets."""
+        with self._lock:
+            check(_ffi.load().cx_tree_set_screen_segment(self._h, int(seg)))
+
+    def screen_segment(self) -> int:
+        """The segment length the last mixed query used (0 before the first)."""
+        out = C.c_int32(0)
+        with self._lock:
+            check(_ffi.load().cx_tree_get_screen_segment(self._h, C.byref(out)))
+        return int(out.value)
+
     def set_profiling(self, enable: bool = True) -> None:
         check(_ffi.load().cx_tree_set_profiling(self._h, int(bool(enable))))
 

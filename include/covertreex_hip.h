@@ -187,6 +187,17 @@ cx_status cx_nystrom_residual(const double *points, int64_t n, int32_t d,
  *                          rescore, overflowed queries past the rescore's end
  *                          (they run on a second stream beside it)]
  *   CX_KNN_REFERENCE_F32: [traversal]                                         */
+/* Candidates per screen segment of the mixed engine (CX_KNN_MIXED): each
+ * workgroup of the screen streams one segment's V rows against 64 queries.
+ * 0 restores the default (the power of two of rows, at most 4096, whose
+ * BF16 hi/lo fragments fill at most 2 MiB, halved while a call's grid would
+ * hold fewer than 512 workgroups); other values must be multiples of 256 in
+ * [256, 16384].  Results do not depend on it.  No counterpart in the
+ * reference: a tile shape for the tuner (covertreex_amd/tuner.py).  The
+ * getter reports the segment length the last mixed query used (0 before the
+ * first). */
+cx_status cx_tree_set_screen_segment(cx_tree *t, int32_t seg);
+cx_status cx_tree_get_screen_segment(cx_tree *t, int32_t *seg_out);
 cx_status cx_tree_set_profiling(cx_tree *t, int32_t enable);
 cx_status cx_tree_last_profile(cx_tree *t, float *ms_out, int32_t cap, int32_t *n_out);
 

@@ -55,6 +55,8 @@ def test_status_codes_without_device():
         _ffi.check(st)
     st = lib.cx_tree_knn_residual(None, None, 0, 5, 0, 1, None, None, None)
     assert st == _ffi.CX_ERR_INVALID
+    assert lib.cx_tree_set_screen_segment(None, 1024) == _ffi.CX_ERR_INVALID
+    assert lib.cx_tree_get_screen_segment(None, None) == _ffi.CX_ERR_INVALID
 
 
 def test_missing_library_fails_loudly(tmp_path, monkeypatch):

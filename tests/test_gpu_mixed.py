@@ -288,3 +288,32 @@ def test_mixed_out_of_range_queries(hip):
                                 mode="mixed")
         for r in (0, 1):
             assert np.array_equal(gi[r], want_i[0]) and np.array_equal(gd[r], want_d[0])
+
+
+@pytest.mark.parametrize("pred", [True, False])
+def test_screen_segment_lengths_same_results(hip, pred):
+    # the screen's segment length (its tile shape, set by the tuner) changes
+    # which workgroup scores a pair, never the neighbours, distances or bounds
+    from covertreex_amd.native import HipTree
+
+    n, d, m, k = 5000, 3, 128, 20
+    x = np.random.default_rng(4).standard_normal((n, d)).astype(np.float32)
+    v, p, *_ = fp64.nystrom(x, inducing_count=m, seed=0, kernel_type=1)
+    tree = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
+    q = np.arange(n)
+    outs = []
+    for seg in (0, 256, 768, 4096, 16384):
+        tree.set_screen_segment(seg)
+        outs.append(tree.knn_residual(q, k, predecessor_mode=pred, mode="mixed",
+                                      return_error=True))
+        # default for mp = 128: 4096, halved while the grid holds fewer than
+        # 512 workgroups (79 query tiles: down to 512)
+        assert tree.screen_segment() == (seg or 512)
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0])
+        assert np.array_equal(o[1].view(np.uint32), outs[0][1].view(np.uint32))
+        assert np.array_equal(o[2].view(np.uint32), outs[0][2].view(np.uint32))
+    qs = q[::37]
+    _check(outs[0][0][qs], outs[0][1][qs], outs[0][2][qs], x, v, p, qs, k, pred, 1)
+    with pytest.raises(ValueError):
+        tree.set_screen_segment(300)
