@@ -33,7 +33,7 @@ CX_KNN_EXACT = 1
 CX_BATCH_FIELDS = 16
 BATCH_FIELDS = ("start", "size", "leaders", "survivors", "orphans", "far_levels",
                 "scan_evals", "mis_evals", "nonleader_evals", "grid_edges", "distance_edges",
-                "scan_ns", "mis_ns", "nonleader_ns", "total_ns")
+                "scan_ns", "mis_ns", "nonleader_ns", "total_ns", "conflict_ns")
 # query telemetry layout (CX_QT_*)
 CX_QT_SIZE = 160
 CX_QT_LEVELS = 64
@@ -82,6 +82,8 @@ SIGNATURES = {
                                    C.c_double, _p, _p]),
     "cx_tree_set_profiling": (_i32, [_p, _i32]),
     "cx_tree_set_screen_segment": (_i32, [_p, _i32]),
+    "cx_set_conflict_log": (_i32, [_i32]),
+    "cx_tree_conflict_log": (_i32, [_p, _p, _i64, _p]),
     "cx_tree_get_screen_segment": (_i32, [_p, _p]),
     "cx_tree_last_profile": (_i32, [_p, _p, _i32, _p]),
     "cx_tree_insert_residual": (_i32, [_p, _p, _p, _p, _i64, _i32]),

@@ -687,10 +687,27 @@ cx_status cx_tree_build_telemetry(const cx_tree *t, int64_t *out, int64_t cap_ba
         for (int64_t b = 0; b < nb && b < cap_batches; b++)
             for (int f = 0; f < CX_BATCH_FIELDS; f++) {
                 int64_t v = s.batch_stats[(size_t)(b * CX_BATCH_FIELDS + f)];
-                if (f >= CX_BF_NS_SCAN && f <= CX_BF_NS_TOTAL)
+                if (f >= CX_BF_NS_SCAN && f <= CX_BF_NS_CONFLICT)
                     v = (int64_t)((double)v * ns_per_tick);
                 out[b * CX_BATCH_FIELDS + f] = v;
             }
+    });
+}
+
+cx_status cx_set_conflict_log(int32_t enable)
+{
+    return guarded([&] { set_conflict_log(enable != 0); });
+}
+
+cx_status cx_tree_conflict_log(const cx_tree *t, int32_t *out, int64_t cap, int64_t *n_words)
+{
+    return guarded([&] {
+        CX_REQUIRE(t != nullptr && n_words != nullptr, "NULL pointer");
+        const std::vector<int32_t> &log = t->s.conflict_log;
+        *n_words = (int64_t)log.size();
+        if (out && cap > 0)
+            std::memcpy(out, log.data(),
+                        sizeof(int32_t) * (size_t)std::min<int64_t>(cap, (int64_t)log.size()));
     });
 }
 

@@ -14,6 +14,9 @@
 // bit for bit.
 #include <hipcub/hipcub.hpp>
 
+#include <atomic>
+#include <string>
+
 #include <algorithm>
 #include <cstdlib>
 
@@ -173,7 +176,7 @@ struct BiState {
     int32_t nc;     // candidates at the current level
     int32_t nrk;    // entries of the root's child array (oldest link first)
     int64_t ev_scan, ev_mis, ev_nl, orphans, e_grid, e_dist, leaders, far_levels;
-    uint64_t tk_begin, tk_mark, tk_pairs, tk_scan, tk_mis, tk_nl;
+    uint64_t tk_begin, tk_mark, tk_pairs, tk_scan, tk_conf, tk_mis, tk_nl;
     int32_t cand[kBiMaxChunk], cover[kBiMaxChunk];
     // scan result: (d^2 bits << 32) | position in [cover, children...], or kBiFar
     uint64_t key[kBiMaxChunk];
@@ -263,7 +266,7 @@ __global__ void __launch_bounds__(kBiResThreads)
         S->e_grid = S->e_dist = S->leaders = S->far_levels = 0;
         const uint64_t now = wall_clock64();
         S->tk_begin = S->tk_mark = S->tk_pairs = now;
-        S->tk_scan = S->tk_mis = S->tk_nl = 0;
+        S->tk_scan = S->tk_conf = S->tk_mis = S->tk_nl = 0;
     }
 }
 
@@ -795,6 +798,23 @@ __device__ void link_in_order(TreeLinks t, int32_t *__restrict__ rk, const int32
     __syncthreads();
 }
 
+// Optional per-level record of a batch's conflict graph, for the reference's
+// BatchInsertTelemetry payload (src/algo/batch/mod.rs:802-956).  buf[0..1]
+// is a 64-bit cursor counting the words used after that header; entries are
+//   level:  1, batch, level, nf, far node ids [nf], leader bits [W],
+//           covered-non-leader bits [W], per far row i: distance-conflict
+//           bits [W] (d^2 <= r^2, columns j > i) for every row, then grid
+//           bits [W] for every row (W = ceil(nf / 32) words; bit j of row i
+//           in word j / 32)
+//   bottom: 2, batch, count, survivor node ids [count]
+// A run that needs more than cap words leaves the cursor past cap and
+// writes nothing past it (the host reports the overflow).
+struct ConflictLog {
+    int32_t *buf; // null: no record
+    int64_t cap;  // words after the header
+    int32_t batch0;
+};
+
 // The rest of a level after the scan and the conflict pass: near/far split,
 // greedy MIS, leaders, non-leaders, the next level's candidates; at level
 // -10 also the bottom survivors and the batch's counters (bs: its row of
@@ -805,7 +825,8 @@ __device__ __forceinline__ void bi_resolve_body(const TreeLinks &t, BiState *S,
                                                 const float *__restrict__ dist,
                                                 const uint64_t *__restrict__ adj,
                                                 const uint64_t *__restrict__ gadj, int32_t level,
-                                                float r2, int64_t *__restrict__ bs)
+                                                float r2, int64_t *__restrict__ bs,
+                                                const ConflictLog &CL)
 {
     __shared__ int32_t s_nxc[kBiMaxChunk], s_nxv[kBiMaxChunk]; // next level
     __shared__ int32_t s_nxst[kBiMaxChunk], s_nxpl[kBiMaxChunk], s_nxph[kBiMaxChunk];
@@ -819,6 +840,7 @@ __device__ __forceinline__ void bi_resolve_body(const TreeLinks &t, BiState *S,
     __shared__ int64_t s_red[kBiResWaves];
     __shared__ int64_t s_acc[3]; // scan evaluations, grid edges, distance-only edges
     __shared__ LinkScratch L;
+    __shared__ int64_t s_log; // this level's entry in the conflict record (-1: none)
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint64_t tk0 = wall_clock64();
@@ -918,6 +940,51 @@ __device__ __forceinline__ void bi_resolve_body(const TreeLinks &t, BiState *S,
                 s_sel[lane] = sel;
         }
         __syncthreads();
+        // the level's conflict record: far set, leaders, conflict bits
+        const int W32 = (nf + 31) >> 5;
+        if (CL.buf) {
+            if (tid == 0) {
+                const int64_t words = 4 + nf + 2 * W32 + 2 * (int64_t)nf * W32;
+                const int64_t base =
+                    (int64_t)atomicAdd((unsigned long long *)CL.buf, (unsigned long long)words);
+                s_log = base + words <= CL.cap ? base : -1;
+            }
+            __syncthreads();
+            if (s_log >= 0) {
+                int32_t *e = CL.buf + 2 + s_log;
+                if (tid == 0) {
+                    e[0] = 1;
+                    e[1] = CL.batch0 + (int32_t)S->bidx;
+                    e[2] = level;
+                    e[3] = nf;
+                }
+                for (int i = tid; i < nf; i += kBiResThreads)
+                    e[4 + i] = s_far[i];
+                for (int w = tid; w < W32; w += kBiResThreads) {
+                    e[4 + nf + w] = (int32_t)(uint32_t)(s_sel[w >> 1] >> (32 * (w & 1)));
+                    e[4 + nf + W32 + w] = 0; // covered bits, set below
+                }
+                int32_t *db = e + 4 + nf + 2 * W32, *gb = db + (int64_t)nf * W32;
+                for (int it = tid; it < nf * W32; it += kBiResThreads) {
+                    const int i = it / W32, w = it - i * W32;
+                    uint32_t dw = 0u, gw = 0u;
+                    if (nf > 1) {
+#pragma unroll 4
+                        for (int b = 0; b < 32; b++) {
+                            const int j = 32 * w + b;
+                            if (j > i && j < nf) {
+                                if (dist[(size_t)i * kBiMaxChunk + j] <= r2)
+                                    dw |= 1u << b;
+                                if ((gadj[i * kBiWords + (j >> 6)] >> (j & 63)) & 1)
+                                    gw |= 1u << b;
+                            }
+                        }
+                    }
+                    db[it] = (int32_t)dw;
+                    gb[it] = (int32_t)gw;
+                }
+            }
+        }
         const bool isl = tid < nf && ((s_sel[tid >> 6] >> (tid & 63)) & 1);
         int nl;
         const int pl = wg_prefix<kBiResWaves>(isl, L.cnt, &nl);
@@ -961,6 +1028,11 @@ __device__ __forceinline__ void bi_resolve_body(const TreeLinks &t, BiState *S,
         }
         __syncthreads();
         const bool keep = tid < nnl && s_nlkey[tid] != kBiFar;
+        if (CL.buf && s_log >= 0 && keep) {
+            const int f = s_nlf[tid];
+            atomicOr((unsigned int *)(CL.buf + 2 + s_log + 4 + nf + W32 + (f >> 5)),
+                     1u << (f & 31));
+        }
         int nk;
         const int pk = wg_prefix<kBiResWaves>(keep, L.cnt, &nk);
         if (keep) {
@@ -985,6 +1057,25 @@ __device__ __forceinline__ void bi_resolve_body(const TreeLinks &t, BiState *S,
     }
     if (level == -10)
         link_in_order(t, rk, s_nxc, s_nxv, nn, level, L);
+    if (level == -10 && CL.buf) {
+        if (tid == 0) {
+            const int64_t words = 3 + nn;
+            const int64_t base =
+                (int64_t)atomicAdd((unsigned long long *)CL.buf, (unsigned long long)words);
+            s_log = base + words <= CL.cap ? base : -1;
+        }
+        __syncthreads();
+        if (s_log >= 0) {
+            int32_t *e = CL.buf + 2 + s_log;
+            if (tid == 0) {
+                e[0] = 2;
+                e[1] = CL.batch0 + (int32_t)S->bidx;
+                e[2] = nn;
+            }
+            for (int i = tid; i < nn; i += kBiResThreads)
+                e[3 + i] = s_nxc[i];
+        }
+    }
     if (tid == 0) {
         S->nc = nn;
         S->nrk = L.nrk;
@@ -997,7 +1088,8 @@ __device__ __forceinline__ void bi_resolve_body(const TreeLinks &t, BiState *S,
         S->leaders += leaders;
         S->far_levels += nf > 0 ? 1 : 0;
         S->tk_scan += S->tk_pairs > S->tk_mark ? S->tk_pairs - S->tk_mark : 0;
-        S->tk_mis += (tk0 > S->tk_pairs ? tk0 - S->tk_pairs : 0) + tk_mis;
+        S->tk_conf += tk0 > S->tk_pairs ? tk0 - S->tk_pairs : 0;
+        S->tk_mis += tk_mis;
         S->tk_nl += tk_nl;
         const uint64_t now = wall_clock64();
         S->tk_mark = now;
@@ -1018,6 +1110,7 @@ __device__ __forceinline__ void bi_resolve_body(const TreeLinks &t, BiState *S,
             bs[12] = (int64_t)S->tk_mis;
             bs[13] = (int64_t)S->tk_nl;
             bs[14] = (int64_t)(now - S->tk_begin);
+            bs[15] = (int64_t)S->tk_conf;
         }
         if (level == -10)
             S->bidx += 1;
@@ -1029,14 +1122,14 @@ __global__ void __launch_bounds__(kBiResThreads)
     bi_resolve_kernel(TreeLinks t, BiState *S, int32_t *__restrict__ rk,
                       const float *__restrict__ dist, const uint64_t *__restrict__ adj,
                       const uint64_t *__restrict__ gadj, int32_t level, float r2,
-                      int64_t *__restrict__ bs, int32_t track_from)
+                      int64_t *__restrict__ bs, int32_t track_from, ConflictLog CL)
 {
     // levels at or below track_from count towards the work the fused
     // kernel would have had (thread 0 makes every update of these counters)
     const bool track = level <= track_from && threadIdx.x == 0;
     const int64_t ev0 = track ? S->ev_scan : 0;
     const int64_t evp0 = track ? S->ev_mis + S->ev_nl : 0;
-    bi_resolve_body<KIND>(t, S, rk, dist, adj, gadj, level, r2, bs);
+    bi_resolve_body<KIND>(t, S, rk, dist, adj, gadj, level, r2, bs, CL);
     if (track) {
         S->ev_fused += S->ev_scan - ev0;
         S->ev_fused_pairs += S->ev_mis + S->ev_nl - evp0;
@@ -1058,7 +1151,7 @@ __global__ void __launch_bounds__(kBiResThreads)
                      const float *__restrict__ raw, int32_t gd, float gs, int32_t use_grid,
                      float *__restrict__ dist, uint64_t *__restrict__ adj,
                      uint64_t *__restrict__ gadj, int32_t top, int64_t *__restrict__ bs,
-                     int64_t n_nodes)
+                     int64_t n_nodes, ConflictLog CL)
 {
     __shared__ int32_t s_pfar[kBiMaxChunk];
     __shared__ int s_pcnt[kBiResWaves];
@@ -1076,7 +1169,7 @@ __global__ void __launch_bounds__(kBiResThreads)
         bi_pairs_body<KIND, kBiResWaves>(mv, S, raw, gd, gs, use_grid, r2, dist, adj, gadj,
                                          s_pfar, s_pcnt, wave, kBiResWaves, true);
         __syncthreads();
-        bi_resolve_body<KIND>(t, S, rk, dist, adj, gadj, level, r2, bs);
+        bi_resolve_body<KIND>(t, S, rk, dist, adj, gadj, level, r2, bs, CL);
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -1377,6 +1470,24 @@ static int32_t compute_depths(const int64_t *parents, int64_t n, int32_t *depth,
     return h;
 }
 
+// Conflict records of batch inserts (off by default): set_conflict_log
+// turns them on for every later insert in the process.
+static std::atomic<int> g_conflict_log{0};
+
+void set_conflict_log(bool enable) { g_conflict_log.store(enable ? 1 : 0); }
+
+// Words of conflict record one insert call may write (4-byte words; 32M by
+// default, COVERTREEX_HIP_CONFLICT_LOG_WORDS overrides it).
+static int64_t conflict_log_words()
+{
+    if (const char *e = getenv("COVERTREEX_HIP_CONFLICT_LOG_WORDS")) {
+        const long long v = atoll(e);
+        if (v > 0)
+            return (int64_t)v;
+    }
+    return (int64_t)1 << 25;
+}
+
 // Runs the batch insert of rows [s_begin, s_end) (default: to t.n) and
 // appends the per-batch telemetry to t.batch_stats.
 // Runs of at least this many full batches replay a recorded graph.
@@ -1533,6 +1644,15 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
     DevBuf<uint64_t> adj((size_t)kBiMaxChunk * kBiWords), gadj((size_t)kBiMaxChunk * kBiWords);
     if (s_begin > 0)
         bi_root_kids_kernel<<<1, 1, 0, st>>>(links, state.ptr, rk.ptr, t.n);
+    // the conflict record of this run's batches (when turned on)
+    ConflictLog clog{nullptr, 0, (int32_t)(t.batch_stats.size() / kBatchFields)};
+    DevBuf<int32_t> logbuf;
+    if (g_conflict_log.load() != 0) {
+        clog.cap = conflict_log_words();
+        logbuf.alloc((size_t)clog.cap + 2);
+        clog.buf = logbuf.ptr;
+        CX_HIP(hipMemsetAsync(logbuf.ptr, 0, 2 * sizeof(int32_t), st));
+    }
     const int32_t fused_from = fused_build_level(KIND == kMetricEuclidean);
     const int32_t near_bottom = KIND == kMetricEuclidean ? euclid_near_bottom(t, st) : 10;
     CX_HIP(hipMemsetAsync(&state.ptr->bidx, 0, sizeof(int64_t), st));
@@ -1566,12 +1686,12 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
                 gadj.ptr);
             bi_resolve_kernel<KIND><<<1, kBiResThreads, 0, st>>>(
                 links, state.ptr, rk.ptr, dist.ptr, adj.ptr, gadj.ptr, level, r2, bst.ptr,
-                fused_from);
+                fused_from, clog);
         }
         if (fused >= -10)
             bi_levels_kernel<KIND><<<1, kBiResThreads, 0, st>>>(
                 mv, links, state.ptr, rk.ptr, t.raw.ptr, t.d, grid_scale, use_grid, dist.ptr,
-                adj.ptr, gadj.ptr, fused, bst.ptr, t.n);
+                adj.ptr, gadj.ptr, fused, bst.ptr, t.n, clog);
     };
     // Full batches replay one recorded graph of the batch's launches per
     // mode (the host then submits a batch at a time instead of tens of
@@ -1693,6 +1813,20 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
     CX_HIP(hipMemcpyAsync(t.batch_stats.data() + old, bst.ptr, bst.bytes(), hipMemcpyDeviceToHost,
                           st));
     CX_HIP(hipStreamSynchronize(st));
+    if (clog.buf) {
+        int64_t used = 0;
+        CX_HIP(hipMemcpyAsync(&used, logbuf.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        CX_HIP(hipStreamSynchronize(st));
+        CX_REQUIRE(used <= clog.cap, "conflict record overflow: the insert needs " +
+                                         std::to_string(used) +
+                                         " words; raise COVERTREEX_HIP_CONFLICT_LOG_WORDS");
+        const size_t at = t.conflict_log.size();
+        t.conflict_log.resize(at + (size_t)used);
+        if (used)
+            CX_HIP(hipMemcpyAsync(t.conflict_log.data() + at, logbuf.ptr + 2,
+                                  (size_t)used * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        CX_HIP(hipStreamSynchronize(st));
+    }
 }
 
 // Arrays derived from the links after inserts: child CSR, depths, si_cache

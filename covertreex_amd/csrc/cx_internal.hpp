@@ -152,9 +152,10 @@ enum BatchField {
     kBfGridEdges = 9, // conflicts found through the grid
     kBfDistEdges = 10, // conflicts found through d^2 <= r^2
     kBfTicksScan = 11, // wall-clock ticks: near/far scan
-    kBfTicksMis = 12,  // wall-clock ticks: MIS and leader linking
+    kBfTicksMis = 12,  // wall-clock ticks: near/far split, MIS and leader linking
     kBfTicksNl = 13,   // wall-clock ticks: non-leader attachment
-    kBfTicksTotal = 14 // wall-clock ticks: whole batch
+    kBfTicksTotal = 14, // wall-clock ticks: whole batch
+    kBfTicksConflict = 15 // wall-clock ticks: conflict pass (far pairs' distances, grid)
 };
 
 // Device-resident tree and metric data (all arrays in tree order).
@@ -226,6 +227,9 @@ struct TreeState {
     int32_t chunk = 512;
     float grid_scale = 1.0f;
     std::vector<int64_t> batch_stats;
+    // conflict records of the inserts made while they were turned on
+    // (set_conflict_log; entry format at ConflictLog in build.hip)
+    std::vector<int32_t> conflict_log;
     // (start, size) of every batch inserted since the build or the last delete
     // (tree positions): the invariant checker's separation groups
     std::vector<int64_t> live_batches;
@@ -285,6 +289,8 @@ void knn_reference_f32(TreeState &t, const int64_t *d_queries, int64_t nq, int32
 // Candidates per screen segment of the mixed engine for nq queries on this
 // tree, and whether a requested value is acceptable (0 or a multiple of 256
 // in [256, 16384]).
+// Turns the batch inserts' conflict records on or off (process-wide).
+void set_conflict_log(bool enable);
 int32_t screen_segment(const TreeState &t, int64_t nq);
 bool screen_segment_valid(int32_t seg);
 void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k,

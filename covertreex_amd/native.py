@@ -195,6 +195,18 @@ class HipTree:
         return [{k: int(row[i]) for i, k in enumerate(_ffi.BATCH_FIELDS)}
                 for row in out[: nb.value]]
 
+    def conflict_log(self) -> np.ndarray:
+        """The conflict records of the batches inserted while
+        :func:`set_conflict_log` was on (int32 words; layout in
+        ``include/covertreex_hip.h``; ``telemetry.batch_payloads`` turns them
+        into the reference's per-batch payload)."""
+        nw = C.c_int64(0)
+        L = _ffi.load()
+        check(L.cx_tree_conflict_log(self._h, None, 0, C.byref(nw)))
+        out = np.zeros(max(nw.value, 1), np.int32)
+        check(L.cx_tree_conflict_log(self._h, ptr(out), nw.value, C.byref(nw)))
+        return out[: nw.value]
+
     # -- incremental insert (src/lib.rs:254-300) ----------------------------------
     def insert_residual(self, v_new, p_new, coords_new, *, chunk_size: int | None = None):
         """Append rows (dataset indices n, n+1, ...) and batch-insert them in
@@ -298,6 +310,13 @@ class HipTree:
         mx = np.empty(self.n, np.int64)
         check(_ffi.load().cx_tree_get_subtree_bounds(self._h, ptr(mn), ptr(mx)))
         return mn, mx
+
+
+def set_conflict_log(enable: bool = True) -> None:
+    """Record the conflict graph of every later build and insert in this
+    process (the reference collects it when a log writer is given,
+    src/algo/batch/mod.rs:802-956); off by default."""
+    check(_ffi.load().cx_set_conflict_log(int(bool(enable))))
 
 
 def compute_subtree_bounds(parents, node_to_dataset):

@@ -12,12 +12,19 @@ here writes records with the same identity fields (``schema_id``,
 quantity; :func:`record_build_batches` emits one record per device batch from
 ``HipTree.build_telemetry()``.
 
-Device-build specifics, recorded under ``hip_*`` keys: the conflict pass
-(every far pair's distance and conflict bit) and the greedy MIS run back to
-back on the device and are timed as one figure (``conflict_graph_ms``;
-``mis_ms`` is 0 and ``hip_conflict_mis_fused`` is 1); ``conflict_adj_pairs``
-counts the conflict edges (grid and distance) and ``hip_mis_evals`` the
-far pairs scored, n_far * (n_far - 1) / 2 per level as the reference counts.
+Device-build specifics, recorded under ``hip_*`` keys: ``conflict_graph_ms``
+is the conflict pass (every far pair's distance, grid cells and conflict
+bits) and ``mis_ms`` the near/far split, greedy MIS and leader linking, each
+from the device wall clock; ``conflict_adj_pairs`` counts the conflict edges
+(grid and distance) and ``hip_mis_evals`` the far pairs scored,
+n_far * (n_far - 1) / 2 per level as the reference counts.
+
+:func:`batch_payloads` rebuilds, from the device's conflict records
+(``set_conflict_log``), the per-batch payload the reference's Rust path sends
+to ``record_rust_batch`` (``emit_rust_batch``, src/lib.rs:1274-1371, filled
+by ``batch_insert_collect``, src/algo/batch/mod.rs:802-956): parents and
+levels of the batch's rows, the selected and dominated rows, and the conflict
+CSR in batch-local indices.
 """
 
 from __future__ import annotations
@@ -142,18 +149,17 @@ def batch_record_fields(b: Mapping[str, int], *, batch_order: str = "hilbert-mor
         "dominated": size - int(b["leaders"]),
         "mis_iterations": 1,
         "traversal_ms": ms(b["scan_ns"]),
-        "conflict_graph_ms": ms(b["mis_ns"]),
-        "mis_ms": 0.0,
+        "conflict_graph_ms": ms(b["conflict_ns"]),
+        "mis_ms": ms(b["mis_ns"]),
         "traversal_pairwise_ms": ms(b["scan_ns"]),
         "traversal_mask_ms": 0.0,
         "traversal_semisort_ms": 0.0,
-        "conflict_pairwise_ms": ms(b["mis_ns"]),
+        "conflict_pairwise_ms": ms(b["conflict_ns"]),
         "conflict_adj_pairs": int(b["grid_edges"]) + int(b["distance_edges"]),
         "conflict_grid_local_edges": int(b["grid_edges"]),
         "batch_order_strategy": batch_order,
         "traversal_engine": "hip",
         "engine": "hip",
-        "hip_conflict_mis_fused": 1,
         "hip_batch_start": int(b["start"]),
         "hip_leaders": int(b["leaders"]),
         "hip_survivors": int(b["survivors"]),
@@ -178,5 +184,134 @@ def record_build_batches(writer: BenchmarkLogWriter, tree, *, first_batch: int =
     return len(batches) - first_batch
 
 
+def parse_conflict_log(words):
+    """Split a conflict record (int32 words, the layout of
+    ``cx_tree_conflict_log``) into per-batch lists of level entries
+    ``(level, far ids, leader mask, covered mask, distance bits, grid bits)``
+    (bool arrays; the bit matrices nf x nf, upper triangle) and bottom
+    survivors."""
+    import numpy as np
+
+    w = np.asarray(words, np.int64)
+    out: Dict[int, Dict[str, list]] = {}
+    pos = 0
+
+    def unpack(row_words, nf):
+        u = (row_words.astype(np.int64) & 0xFFFFFFFF).astype(np.uint32)
+        bits = (u[..., :, None] >> np.arange(32, dtype=np.uint32)) & 1
+        return bits.reshape(*u.shape[:-1], -1)[..., :nf].astype(bool)
+
+    while pos < w.size:
+        kind, batch = int(w[pos]), int(w[pos + 1])
+        rec = out.setdefault(batch, {"levels": [], "survivors": []})
+        if kind == 1:
+            level, nf = int(w[pos + 2]), int(w[pos + 3])
+            W = (nf + 31) // 32
+            p = pos + 4
+            far = w[p:p + nf]
+            p += nf
+            sel = unpack(w[p:p + W], nf)
+            cov = unpack(w[p + W:p + 2 * W], nf)
+            p += 2 * W
+            dist = unpack(w[p:p + nf * W].reshape(nf, W), nf)
+            grid = unpack(w[p + nf * W:p + 2 * nf * W].reshape(nf, W), nf)
+            rec["levels"].append((level, far, sel, cov, dist, grid))
+            pos = p + 2 * nf * W
+        elif kind == 2:
+            cnt = int(w[pos + 2])
+            rec["survivors"].extend(int(x) for x in w[pos + 3:pos + 3 + cnt])
+            pos += 3 + cnt
+        else:
+            raise ValueError(f"conflict record: unknown entry kind {kind} at word {pos}")
+    return out
+
+
+def batch_payloads(tree, *, coords=None, grid_scale: float = 1.0, words=None):
+    """The reference's per-batch payload (``emit_rust_batch``) for every batch
+    with a conflict record: ``batch_index``, ``batch_size``, ``parents``,
+    ``levels`` (the batch's rows, tree order), ``selected`` (leaders at any
+    level and bottom survivors) and ``dominated`` (far non-leaders covered by a
+    leader), batch-local and ascending, ``conflict_indptr`` /
+    ``conflict_indices`` (row u lists, level by level, its distance
+    conflicts in far-list order, then each grid neighbour twice -- the
+    reference pushes both directions of every directed grid edge; the order
+    among grid neighbours follows a hash set there and ascending far-list
+    order here), ``timings`` (device wall clock) and the grid statistics.
+    ``coords`` (dataset order, as given to the build) adds
+    ``conflict_grid_cells``: the most occupied grid cells of one level's far
+    set."""
+    return payloads_from_record(
+        tree.conflict_log() if words is None else words, tree.build_telemetry(),
+        tree.get_parents(), tree.get_levels(),
+        order=tree.get_order() if coords is not None else None, coords=coords,
+        grid_scale=grid_scale)
+
+
+def payloads_from_record(words, batches, parents, levels, *, order=None, coords=None,
+                         grid_scale: float = 1.0):
+    """:func:`batch_payloads` from its parts: the conflict record, the
+    per-batch telemetry rows (``start``, ``size`` and, when present, the
+    ``*_ns`` timings), the tree's parents and levels (tree order) and, for
+    the grid statistics, the node-to-dataset order and dataset coordinates."""
+    import numpy as np
+
+    recs = parse_conflict_log(words)
+    parents = np.asarray(parents)
+    levels = np.asarray(levels)
+    out = []
+    for b, rec in sorted(recs.items()):
+        st = batches[b]
+        s0, nb = int(st["start"]), int(st["size"])
+        rows = [[] for _ in range(nb)]
+        sel = np.zeros(nb, bool)
+        dom = np.zeros(nb, bool)
+        cells = leaders_raw = local_edges = 0
+        for level, far, lsel, lcov, dist, grid in rec["levels"]:
+            loc = far - s0
+            sel[loc[lsel]] = True
+            dom[loc[lcov]] = True
+            nf = far.size
+            leaders_raw = max(leaders_raw, nf)
+            local_edges += int(grid.sum())
+            if coords is not None and nf:
+                keys = np.floor(np.asarray(coords, np.float32)[order[far]] * np.float32(grid_scale))
+                cells = max(cells, len({tuple(k) for k in keys.tolist()}))
+            dsym = dist | dist.T
+            gsym = grid | grid.T
+            for i in range(nf):
+                r = rows[int(loc[i])]
+                r.extend(int(x) for x in loc[np.nonzero(dsym[i])[0]])
+                for j in np.nonzero(gsym[i])[0]:
+                    r.extend((int(loc[j]), int(loc[j])))
+        for node in rec["survivors"]:
+            sel[node - s0] = True
+        indptr = np.zeros(nb + 1, np.int64)
+        indptr[1:] = np.cumsum([len(r) for r in rows])
+        indices = np.array([x for r in rows for x in r], np.int64)
+        out.append({
+            "batch_index": b,
+            "batch_size": nb,
+            "parents": parents[s0:s0 + nb].tolist(),
+            "levels": levels[s0:s0 + nb].astype(np.int64).tolist(),
+            "selected": np.nonzero(sel)[0].tolist(),
+            "dominated": np.nonzero(dom)[0].tolist(),
+            "conflict_indptr": indptr.tolist(),
+            "conflict_indices": indices.tolist(),
+            "timings": {"traversal_seconds": st.get("scan_ns", 0) * 1e-9,
+                        "conflict_graph_seconds": st.get("conflict_ns", 0) * 1e-9,
+                        "mis_seconds": st.get("mis_ns", 0) * 1e-9,
+                        "pairwise_seconds": st.get("conflict_ns", 0) * 1e-9,
+                        "mis_iterations": 1},
+            "batch_order_strategy": "hilbert-morton",
+            "engine": "hip",
+            "conflict_grid_cells": cells if coords is not None else None,
+            "conflict_grid_leaders_raw": leaders_raw,
+            "conflict_grid_leaders_after": cells if coords is not None else None,
+            "conflict_grid_local_edges": local_edges,
+        })
+    return out
+
+
 __all__ = ["SCHEMA_ID", "SCHEMA_VERSION", "REQUIRED_KEYS", "BenchmarkLogWriter",
-           "batch_record_fields", "record_build_batches"]
+           "batch_record_fields", "record_build_batches", "parse_conflict_log",
+           "batch_payloads", "payloads_from_record"]

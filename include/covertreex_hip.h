@@ -246,12 +246,31 @@ cx_status cx_tree_delete(cx_tree *t, const int64_t *indices, int64_t n_remove);
 #define CX_BF_GRID_EDGES 9 /* conflicts found through the grid */
 #define CX_BF_DIST_EDGES 10 /* conflicts found through d^2 <= r^2 */
 #define CX_BF_NS_SCAN 11   /* device wall time (ns): near/far scan */
-#define CX_BF_NS_MIS 12    /* device wall time (ns): conflicts, MIS, leaders */
+#define CX_BF_NS_MIS 12    /* device wall time (ns): near/far split, MIS, leaders */
 #define CX_BF_NS_NL 13     /* device wall time (ns): non-leader attachment */
 #define CX_BF_NS_TOTAL 14  /* device wall time (ns): whole batch */
+#define CX_BF_NS_CONFLICT 15 /* device wall time (ns): conflict pass (far pairs) */
 /* Writes min(cap, batches) rows of CX_BATCH_FIELDS; *n_batches = batches. */
 cx_status cx_tree_build_telemetry(const cx_tree *t, int64_t *out, int64_t cap_batches,
                                   int64_t *n_batches);
+
+/* Conflict records of batch inserts, for the reference's per-batch payload
+ * (BatchInsertTelemetry's parents/levels/selected/dominated/conflict_indptr/
+ * conflict_indices, src/algo/batch/mod.rs:802-956, sent by emit_rust_batch,
+ * src/lib.rs:1274-1371, when a log writer is given).  Off by default:
+ * cx_set_conflict_log(1) turns them on for every later build and insert in
+ * the process.  cx_tree_conflict_log copies a tree's record (int32 words)
+ * into out (up to cap words; *n_words = its length).  Entries, in insert
+ * order:
+ *   1, batch, level, nf, far node ids [nf], leader bits [W], covered
+ *   non-leader bits [W], distance-conflict bits [nf][W] (d^2 <= r^2, j > i),
+ *   grid-conflict bits [nf][W]   (one per level with a far set; W =
+ *   ceil(nf / 32), bit j of a row in word j / 32)
+ *   2, batch, count, bottom survivor node ids [count]   (one per batch)
+ * COVERTREEX_HIP_CONFLICT_LOG_WORDS sets how many words one insert call may
+ * record (32M by default; more fails with CX_ERR_INVALID). */
+cx_status cx_set_conflict_log(int32_t enable);
+cx_status cx_tree_conflict_log(const cx_tree *t, int32_t *out, int64_t cap, int64_t *n_words);
 
 /* Query telemetry.  Replaces the COVERTREEX_RUST_QUERY_TELEMETRY switch and
  * CoverTreeWrapper.last_query_telemetry / clear_last_query_telemetry

@@ -101,6 +101,42 @@ void ox_grid_edges_dense(int64_t n, int32_t d, const float *coords, float scale,
     free(keys);
 }
 
+/* Conflict record of the batches inserted while a buffer is set (the
+ * reference's BatchInsertTelemetry conflict CSR, selected and dominated
+ * marks, src/algo/batch/mod.rs:696-800, 802-956), in the layout the device
+ * library writes: per level with a far set
+ *   1, batch, level, nf, far ids [nf], leader bits [W], covered non-leader
+ *   bits [W], distance-conflict bits [nf][W] (d^2 <= r^2, columns j > i),
+ *   grid-conflict bits [nf][W]                     (W = ceil(nf / 32))
+ * and per batch 2, batch, count, bottom survivors [count].  Words past cap
+ * are counted in *used but not written. */
+static int32_t *g_log = NULL;
+static int64_t g_log_cap = 0;
+static int64_t *g_log_used = NULL;
+static int32_t g_log_batch = 0;
+
+void ox_set_conflict_log(int32_t *buf, int64_t cap, int64_t *used, int32_t batch)
+{
+    g_log = buf;
+    g_log_cap = cap;
+    g_log_used = used;
+    g_log_batch = batch;
+}
+
+static int32_t *log_reserve(int64_t words)
+{
+    if (!g_log)
+        return NULL;
+    int64_t at = *g_log_used;
+    *g_log_used += words;
+    return at + words <= g_log_cap ? g_log + at : NULL;
+}
+
+static void log_set_bit(int32_t *words, int64_t j)
+{
+    words[j >> 5] = (int32_t)((uint32_t)words[j >> 5] | (1u << (j & 31)));
+}
+
 /* stats: [0] filter distance evals, [1] conflict pairs (reference count
  * n_far*(n_far-1)/2), [2] non-leader distance evals, [3] leaders,
  * [4] orphans dropped, [5] levels with a far set */
@@ -247,6 +283,36 @@ int64_t ox_batch_insert(ox_tree *t, const ox_metric *mt, int64_t start, int64_t 
                     }
                 }
             }
+            if (g_log) {
+                const int64_t W = (nf + 31) / 32;
+                int32_t *e = log_reserve(4 + nf + 2 * W + 2 * nf * W);
+                if (e) {
+                    memset(e, 0, sizeof(int32_t) * (size_t)(4 + nf + 2 * W + 2 * nf * W));
+                    e[0] = 1;
+                    e[1] = g_log_batch;
+                    e[2] = level;
+                    e[3] = (int32_t)nf;
+                    int32_t *sel = e + 4 + nf, *cov = sel + W, *db = cov + W, *gb = db + nf * W;
+                    for (int64_t i = 0; i < nf; i++) {
+                        e[4 + i] = (int32_t)far[i];
+                        if (state[i] == 1)
+                            log_set_bit(sel, i);
+                        else if (nl_best[i] >= 0)
+                            log_set_bit(cov, i);
+                    }
+                    /* every far pair, as compute_candidate_conflicts (370-400)
+                     * and grid_conflicts_for_candidates (212-327) list them */
+#pragma omp parallel for schedule(dynamic, 8)
+                    for (int64_t i = 0; i < nf; i++)
+                        for (int64_t j = i + 1; j < nf; j++) {
+                            if (ox_dist_sq(mt, far[i], far[j]) <= r2)
+                                log_set_bit(db + i * W, j);
+                            if (use_grid && grid_coords && gd > 0 &&
+                                grid_adjacent(keys + i * gd, keys + j * gd, gd))
+                                log_set_bit(gb + i * W, j);
+                        }
+                }
+            }
             for (int64_t i = 0; i < nf; i++) {
                 if (state[i] == 1)
                     continue;
@@ -275,6 +341,16 @@ int64_t ox_batch_insert(ox_tree *t, const ox_metric *mt, int64_t start, int64_t 
     for (int64_t i = 0; i < nc; i++) {
         set_level(t, cands[i], min_level);
         set_parent(t, cands[i], covers[i]);
+    }
+    if (g_log) {
+        int32_t *e = log_reserve(3 + nc);
+        if (e) {
+            e[0] = 2;
+            e[1] = g_log_batch;
+            e[2] = (int32_t)nc;
+            for (int64_t i = 0; i < nc; i++)
+                e[3 + i] = (int32_t)cands[i];
+        }
     }
     if (stats)
         for (int64_t i = 0; i < nb; i++)

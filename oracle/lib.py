@@ -102,6 +102,7 @@ def lib():
         L.ox_build.argtypes = [C.POINTER(_Tree), _p, _i64, _p, _i32, _f32, _i32, _p]
         L.ox_batch_insert.argtypes = [C.POINTER(_Tree), _p, _i64, _i64, _p, _i32, _f32, _i32, _p]
         L.ox_batch_insert.restype = _i64
+        L.ox_set_conflict_log.argtypes = [_p, _i64, _p, _i32]
         L.ox_si_cache.argtypes = [C.POINTER(_Tree), _p, _p]
         L.ox_subtree_bounds.argtypes = [_i64, _p, _p, _p, _p]
         L.ox_greedy_mis.argtypes = [_i64, _p, _p, _p]
@@ -270,29 +271,52 @@ _STAT_KEYS = ["filter_evals", "conflict_pairs", "nonleader_evals", "leaders", "o
               "far_levels"]
 
 
-def _insert_segments(t, metric: _Metric, segments, grid_coords, grid_scale, use_grid):
+_CONFLICT_LOG = False
+
+
+def set_conflict_log(enable: bool) -> None:
+    """Record the conflict graph of later builds and inserts
+    (``OracleTree.conflict_log``, the device library's layout)."""
+    global _CONFLICT_LOG
+    _CONFLICT_LOG = bool(enable)
+
+
+def _insert_segments(t, metric: _Metric, segments, grid_coords, grid_scale, use_grid,
+                     batch0: int = 0):
     """Chunked batch inserts over tree rows: ``segments`` is a list of
-    (start, end, chunk); batches run in order and each records its stats."""
+    (start, end, chunk); batches run in order and each records its stats
+    (and, when turned on, its conflict record; ``batch0`` numbers the first)."""
     L = lib()
     gc = None if grid_coords is None else _c(grid_coords, np.float32)
     gd = 0 if gc is None else gc.shape[1]
-    batches = []
+    batches, logs = [], []
     total = np.zeros(8, np.int64)
     for start, end, chunk in segments:
         chunk = int(chunk) if chunk and chunk > 0 else max(end - start, 1)
         for s in range(start, end, chunk):
             e = min(s + chunk, end)
             st = np.zeros(8, np.int64)
+            if _CONFLICT_LOG:
+                nb, w = e - s, (e - s + 31) // 32
+                cap = 21 * (4 + nb + 2 * w + 2 * nb * w) + 3 + nb
+                buf = np.zeros(cap, np.int32)
+                used = np.zeros(1, np.int64)
+                L.ox_set_conflict_log(_ptr(buf), cap, _ptr(used), batch0 + len(batches))
             L.ox_batch_insert(t, metric.h, s, e, _ptr(gc), gd, float(grid_scale), int(use_grid),
                               _ptr(st))
+            if _CONFLICT_LOG:
+                L.ox_set_conflict_log(None, 0, None, 0)
+                assert used[0] <= cap
+                logs.append(buf[: used[0]].copy())
             total += st
             b = {k: int(v) for k, v in zip(_STAT_KEYS, st)}
             b.update(start=s, size=e - s)
             batches.append(b)
-    return batches, total
+    log = np.concatenate(logs) if logs else np.zeros(0, np.int32)
+    return batches, total, log
 
 
-def _finish(t, metric: _Metric, n: int, order, batches, total) -> "OracleTree":
+def _finish(t, metric: _Metric, n: int, order, batches, total, log=None) -> "OracleTree":
     L = lib()
     si = np.zeros(max(n, 1), np.float32)
     L.ox_si_cache(t, metric.h, _ptr(si))
@@ -308,14 +332,16 @@ def _finish(t, metric: _Metric, n: int, order, batches, total) -> "OracleTree":
                       stats={k: int(v) for k, v in zip(_STAT_KEYS, total)}, _tree=t,
                       _metric=metric)
     tree.batch_stats = batches
+    tree.conflict_log = log if log is not None else np.zeros(0, np.int32)
     return tree
 
 
 def _build(metric: _Metric, n: int, order, segments, grid_coords, grid_scale, use_grid):
     """The build's chunk loop (src/lib.rs:1502-1539) and later inserts."""
     t = lib().ox_tree_new(max(n, 1))
-    batches, total = _insert_segments(t, metric, segments, grid_coords, grid_scale, use_grid)
-    return _finish(t, metric, n, order, batches, total)
+    batches, total, log = _insert_segments(t, metric, segments, grid_coords, grid_scale,
+                                           use_grid)
+    return _finish(t, metric, n, order, batches, total, log)
 
 
 def _child_walk(children, nxt, u, n):
@@ -412,9 +438,9 @@ def delete_rows(tree: "OracleTree", data, remove, *, v=None, p=None, var=1.0, ls
     for g1 in ends:
         segs.append((g0, g1, chunk))
         g0 = g1
-    batches, total = _insert_segments(t, metric, segs, gc, grid_scale, use_grid)
+    batches, total, log = _insert_segments(t, metric, segs, gc, grid_scale, use_grid)
     tt.n = n1
-    return _finish(t, metric, n1, order1, batches, total)
+    return _finish(t, metric, n1, order1, batches, total, log)
 
 
 def build_residual_tree(coords, v, p, *, var=1.0, ls=1.0, kernel_type=0, chunk=512,

@@ -88,6 +88,9 @@ int64_t ox_batch_insert(ox_tree *t, const ox_metric *mt, int64_t start, int64_t 
                         int32_t use_grid, int64_t *stats);
 void ox_build(ox_tree *t, const ox_metric *mt, int64_t chunk, const float *grid_coords,
               int32_t grid_d, float grid_scale, int32_t use_grid, int64_t *stats);
+/* Record the conflict graph of later ox_batch_insert calls into buf (cap
+ * words, *used counts words; layout in build.c) as batch `batch`; NULL stops. */
+void ox_set_conflict_log(int32_t *buf, int64_t cap, int64_t *used, int32_t batch);
 void ox_si_cache(const ox_tree *t, const ox_metric *mt, float *out);
 void ox_subtree_bounds(int64_t n, const int64_t *parents, const int64_t *n2d,
                        int64_t *min_out, int64_t *max_out);

@@ -9,7 +9,7 @@ from covertreex_amd.telemetry import (REQUIRED_KEYS, SCHEMA_ID, BenchmarkLogWrit
 BATCH = {"start": 512, "size": 512, "leaders": 7, "survivors": 3, "orphans": 480,
          "far_levels": 12, "scan_evals": 9000, "mis_evals": 800, "nonleader_evals": 3000,
          "grid_edges": 250, "distance_edges": 40, "scan_ns": 2_000_000, "mis_ns": 500_000,
-         "nonleader_ns": 250_000, "total_ns": 3_000_000}
+         "nonleader_ns": 250_000, "total_ns": 3_000_000, "conflict_ns": 400_000}
 
 
 def test_records_carry_schema_identity(tmp_path):
@@ -25,7 +25,8 @@ def test_records_carry_schema_identity(tmp_path):
     assert a["batch_event_index"] == 0 and b["batch_event_index"] == 1
     assert a["seed_pack"] == {"mis": 3} and a["runtime_engine"] == "hip" and a["tag"] == "x"
     assert a["selected"] == 7 and a["dominated"] == 505 and a["candidates"] == 512
-    assert abs(a["traversal_ms"] - 2.0) < 1e-12 and abs(a["conflict_graph_ms"] - 0.5) < 1e-12
+    assert abs(a["traversal_ms"] - 2.0) < 1e-12 and abs(a["conflict_graph_ms"] - 0.4) < 1e-12
+    assert abs(a["mis_ms"] - 0.5) < 1e-12
     assert a["conflict_adj_pairs"] == 290 and a["conflict_grid_local_edges"] == 250
     assert b["event"] == "done" and b["queries"] == 5 and b["conflict_pairwise_reused"] == 1
 
