@@ -247,6 +247,37 @@ def batch_payloads(tree, *, coords=None, grid_scale: float = 1.0, words=None):
         grid_scale=grid_scale)
 
 
+def _scopes_from_rows(rows, nb):
+    """build_scopes_from_edges (src/algo/batch/mod.rs:16-200) with the
+    reference's defaults (scope cap = batch length, no budget schedule, no
+    segment limit, masked append, pair merge): each row's scope is its
+    distinct conflict neighbours in ascending order, and the chunk
+    statistics follow."""
+    indptr, indices = [0], []
+    max_members = points = scans = dedupe = pairs_after = 0
+    for r in rows:
+        members = sorted(set(r))
+        scans += len(r)
+        dedupe += len(r) - len(members)
+        max_members = max(max_members, len(members))
+        points += len(members)
+        pairs_after += min(len(members) * (len(members) - 1) // 2, nb * nb)
+        indices.extend(members)
+        indptr.append(len(indices))
+    pairs_before = sum(len(r) for r in rows) // 2
+    pair_cap = max_members * max_members
+    pairs_after = min(pairs_after, pair_cap)
+    return {"scope_indptr": indptr, "scope_indices": indices,
+            "scope_chunk_segments": nb, "scope_chunk_emitted": nb,
+            "scope_chunk_max_members": max_members, "scope_chunk_points": points,
+            "scope_chunk_scans": scans, "scope_chunk_dedupe": dedupe,
+            "scope_chunk_saturated": 0, "scope_budget_start": nb, "scope_budget_final": nb,
+            "scope_budget_escalations": 0, "scope_budget_early_terminate": 0,
+            "scope_chunk_pair_cap": pair_cap, "scope_chunk_pairs_before": pairs_before,
+            "scope_chunk_pairs_after": pairs_after,
+            "scope_chunk_pair_merges": max(pairs_before - pairs_after, 0)}
+
+
 def payloads_from_record(words, batches, parents, levels, *, order=None, coords=None,
                          grid_scale: float = 1.0):
     """:func:`batch_payloads` from its parts: the conflict record, the
@@ -288,6 +319,7 @@ def payloads_from_record(words, batches, parents, levels, *, order=None, coords=
         indptr = np.zeros(nb + 1, np.int64)
         indptr[1:] = np.cumsum([len(r) for r in rows])
         indices = np.array([x for r in rows for x in r], np.int64)
+        scopes = _scopes_from_rows(rows, nb)
         out.append({
             "batch_index": b,
             "batch_size": nb,
@@ -297,11 +329,15 @@ def payloads_from_record(words, batches, parents, levels, *, order=None, coords=
             "dominated": np.nonzero(dom)[0].tolist(),
             "conflict_indptr": indptr.tolist(),
             "conflict_indices": indices.tolist(),
-            "timings": {"traversal_seconds": st.get("scan_ns", 0) * 1e-9,
-                        "conflict_graph_seconds": st.get("conflict_ns", 0) * 1e-9,
-                        "mis_seconds": st.get("mis_ns", 0) * 1e-9,
-                        "pairwise_seconds": st.get("conflict_ns", 0) * 1e-9,
-                        "mis_iterations": 1},
+            "scope_indptr": scopes.pop("scope_indptr"),
+            "scope_indices": scopes.pop("scope_indices"),
+            "timings": dict({"traversal_seconds": st.get("scan_ns", 0) * 1e-9,
+                             "conflict_graph_seconds": st.get("conflict_ns", 0) * 1e-9,
+                             "mis_seconds": st.get("mis_ns", 0) * 1e-9,
+                             "pairwise_seconds": st.get("conflict_ns", 0) * 1e-9,
+                             "mask_seconds": 0.0, "semisort_seconds": 0.0,
+                             "degree_cap": 0, "degree_pruned_pairs": 0,
+                             "mis_iterations": 1}, **scopes),
             "batch_order_strategy": "hilbert-morton",
             "engine": "hip",
             "conflict_grid_cells": cells if coords is not None else None,

@@ -57,6 +57,15 @@ def _check_expected(pl, levels):
         assert got["levels"] == levels[4 * b:4 * b + 4]
         assert got["conflict_grid_local_edges"] == [1, 2][b]
         assert got["conflict_grid_leaders_raw"] == 2
+        # scopes: each row's distinct neighbours (build_scopes_from_edges,
+        # src/algo/batch/mod.rs:16-200, reference defaults)
+        assert got["scope_indptr"] == [[0, 0, 1, 2, 2], [0, 1, 2, 3, 4]][b]
+        assert got["scope_indices"] == [[2, 1], [2, 3, 0, 1]][b]
+        tm = got["timings"]
+        assert tm["scope_chunk_scans"] == len(want["conflict_indices"])
+        assert tm["scope_chunk_dedupe"] == len(want["conflict_indices"]) - [2, 4][b]
+        assert tm["scope_chunk_pairs_before"] == len(want["conflict_indices"]) // 2
+        assert tm["scope_chunk_max_members"] == 1 and tm["scope_chunk_pairs_after"] == 0
 
 
 def test_euclidean_known_tree_payload(record_on):
