@@ -3,10 +3,13 @@
 //   K_mm = k(U, U) + jitter * var * I           (208-210)
 //   L    = cholesky(K_mm)                       (211), blocked, one workgroup
 //   V    = (L^-1 K_xm^T)^T                      (213-215), blocked forward
-//          substitution in column blocks of 32, one row per thread
+//          substitution in column blocks of 32 (rows resident in LDS up to
+//          m = 256, one row per thread above)
 //   p    = max(var - |V_i|^2, 1e-9)             (217-218)
 // then FP32 storage of V and p (236-240).  Lengthscales may be per dimension
 // (ARD); the reference supports a scalar only.
+#include <mutex>
+
 #include "cx_internal.hpp"
 
 namespace cx {
@@ -217,6 +220,112 @@ __global__ void __launch_bounds__(64)
     }
 }
 
+// V = K_xm L^-T for a block of R rows per workgroup, the rows' FP64 values
+// resident in LDS for the whole solve (written once, as FP32, at the end):
+//   1. sX[r][c] = k(x_{r0+r}, u_c) for every column (rows padded by one
+//      double, so a wavefront reading one column of many rows spreads over
+//      the banks);
+//   2. per column block of kNb: every (row, column) entry subtracts the
+//      earlier columns' terms sum_{l < jb} X[r][l] L[c][l], l ascending,
+//      with L's rows staged through LDS in 32 x 32 tiles; then L's diagonal
+//      block goes to LDS and one thread per row solves against it;
+//   3. V rows out in FP32 and p = max(var - |V_i|^2, 1e-9) (the squared norm
+//      summed in column order).
+// Every value is the one column-by-column forward substitution gives: the
+// same operations in the same order for each entry.
+template <int R>
+__global__ void __launch_bounds__(256)
+    trsm_rows_kernel(const double *__restrict__ pts, int64_t n, int32_t d,
+                     const int64_t *__restrict__ ind, int32_t m,
+                     const double *__restrict__ inv_ls, double var, int32_t kernel,
+                     const double *__restrict__ lt, float *__restrict__ v_out,
+                     float *__restrict__ p_out)
+{
+    extern __shared__ double sX[]; // R x (m + 1)
+    __shared__ double sL[kNb * kNb]; // staged L tile: sL[l * kNb + c] = L[jb + c][l0 + l]
+    const int tid = threadIdx.x;
+    const int ld = m + 1;
+    const int64_t r0 = (int64_t)blockIdx.x * R;
+    const int rows = n - r0 < R ? (int)(n - r0) : R;
+    for (int t = tid; t < R * m; t += 256) {
+        const int r = t / m, c = t % m;
+        sX[r * ld + c] = r < rows ? kernel_f64(pts + (r0 + r) * d, pts + ind[c] * d, inv_ls,
+                                               d, var, kernel)
+                                  : 0.0;
+    }
+    __syncthreads();
+    constexpr int kRg = 256 / kNb;      // row groups
+    constexpr int kPer = R / kRg;       // rows per thread in the update
+    const int c = tid % kNb, rg = tid / kNb;
+    for (int32_t jb = 0; jb < m; jb += kNb) {
+        const int nb = m - jb < kNb ? m - jb : kNb;
+        double acc[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; i++)
+            acc[i] = c < nb ? sX[(rg + kRg * i) * ld + jb + c] : 0.0;
+        for (int32_t l0 = 0; l0 < jb; l0 += kNb) {
+            // L[jb + c][l0 + l] for l < 32 (l0 + 32 <= jb: blocks are whole)
+            for (int t = tid; t < kNb * kNb; t += 256) {
+                const int l = t / kNb, cc = t % kNb;
+                sL[t] = cc < nb ? lt[(int64_t)(l0 + l) * m + jb + cc] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll 8
+            for (int l = 0; l < kNb; l++) {
+                const double lv = sL[l * kNb + c];
+#pragma unroll
+                for (int i = 0; i < kPer; i++)
+                    acc[i] -= sX[(rg + kRg * i) * ld + l0 + l] * lv;
+            }
+            __syncthreads();
+        }
+        if (c < nb)
+#pragma unroll
+            for (int i = 0; i < kPer; i++)
+                sX[(rg + kRg * i) * ld + jb + c] = acc[i];
+        // L's diagonal block: sL[k * kNb + cc] = L[jb + cc][jb + k]
+        for (int t = tid; t < kNb * kNb; t += 256) {
+            const int k = t / kNb, cc = t % kNb;
+            sL[t] = (k < nb && cc < nb) ? lt[(int64_t)(jb + k) * m + jb + cc] : 0.0;
+        }
+        __syncthreads();
+        if (tid < rows) {
+            double x[kNb];
+            double *xr = sX + tid * ld + jb;
+#pragma unroll
+            for (int cc = 0; cc < kNb; cc++)
+                x[cc] = cc < nb ? xr[cc] : 0.0;
+#pragma unroll
+            for (int cc = 0; cc < kNb; cc++) {
+                if (cc < nb) {
+                    double s = x[cc];
+                    for (int k = 0; k < cc; k++)
+                        s -= x[k] * sL[k * kNb + cc];
+                    x[cc] = s / sL[cc * kNb + cc];
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < kNb; cc++)
+                if (cc < nb)
+                    xr[cc] = x[cc];
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < rows * m; t += 256) {
+        const int r = t / m, cc = t % m;
+        v_out[(r0 + r) * m + cc] = (float)sX[r * ld + cc];
+    }
+    if (tid < rows) {
+        double s2 = 0.0;
+        for (int cc = 0; cc < m; cc++) {
+            const double x = sX[tid * ld + cc];
+            s2 += x * x;
+        }
+        const double pv = var - s2;
+        p_out[r0 + tid] = (float)(pv > 1e-9 ? pv : 1e-9);
+    }
+}
+
 void nystrom_device(const double *pts_host, int64_t n, int32_t d, const int64_t *ind_host,
                     int32_t m, double var, const double *ls_host, int32_t n_ls, int32_t kernel,
                     double jitter, float *v_host, float *p_host)
@@ -243,8 +352,7 @@ void nystrom_device(const double *pts_host, int64_t n, int32_t d, const int64_t 
         inv_ls[c] = 1.0 / (l > 1e-12 ? l : 1e-12); // max(lengthscale, 1e-12), 23-24
     }
     CX_REQUIRE(m <= kCholMaxM, "the Nystrom factor supports at most 512 inducing points");
-    DevBuf<double> pts((size_t)n * d), il(d), kmm((size_t)m * m), lt((size_t)m * m),
-        vt((size_t)n * m), norm2(n);
+    DevBuf<double> pts((size_t)n * d), il(d), kmm((size_t)m * m), lt((size_t)m * m);
     DevBuf<int64_t> ind(m);
     DevBuf<float> v((size_t)n * m), p(n);
     DevBuf<int32_t> status(1);
@@ -257,10 +365,31 @@ void nystrom_device(const double *pts_host, int64_t n, int32_t d, const int64_t 
                                                             kernel, jitter * var, kmm.ptr);
     cholesky_kernel<<<1, kCholThreads, 0, st>>>(kmm.ptr, m, status.ptr);
     transpose_lower_kernel<<<(unsigned)((mm + 255) / 256), 256, 0, st>>>(kmm.ptr, m, lt.ptr);
-    for (int32_t jb = 0; jb < m; jb += kNb)
-        trsm_block_kernel<<<(unsigned)((n + 63) / 64), 64, 0, st>>>(
-            pts.ptr, n, d, ind.ptr, m, il.ptr, var, kernel, lt.ptr, jb, vt.ptr, norm2.ptr, v.ptr,
-            p.ptr);
+    // V and p.  Up to m = 256 the rows' values stay in LDS (32 rows per
+    // workgroup, two workgroups per CU): C2 (n = 32,768, m = 256) 0.42 ms
+    // against 1.34 ms for the column-block launches.  Above, a workgroup's
+    // rows no longer leave room for a second workgroup per CU and the
+    // column-block launches are faster (m = 512, n = 262,144: 9.8 ms against
+    // 14.8-15.9 ms with 16 or 32 resident rows).  Both give the same bits.
+    DevBuf<double> vt, norm2;
+    if (m <= 256) {
+        static std::once_flag once; // more than 64 KiB of dynamic LDS
+        std::call_once(once, [] {
+            CX_HIP(hipFuncSetAttribute((const void *)trsm_rows_kernel<32>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(sizeof(double) * 32 * (256 + 1))));
+        });
+        trsm_rows_kernel<32><<<(unsigned)((n + 31) / 32), 256, sizeof(double) * 32 * (m + 1),
+                               st>>>(pts.ptr, n, d, ind.ptr, m, il.ptr, var, kernel, lt.ptr,
+                                     v.ptr, p.ptr);
+    } else {
+        vt.alloc((size_t)n * m);
+        norm2.alloc(n);
+        for (int32_t jb = 0; jb < m; jb += kNb)
+            trsm_block_kernel<<<(unsigned)((n + 63) / 64), 64, 0, st>>>(
+                pts.ptr, n, d, ind.ptr, m, il.ptr, var, kernel, lt.ptr, jb, vt.ptr, norm2.ptr,
+                v.ptr, p.ptr);
+    }
     CX_HIP(hipGetLastError());
     int32_t bad = 0;
     CX_HIP(hipMemcpyAsync(&bad, status.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, st));
