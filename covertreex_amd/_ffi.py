@@ -92,6 +92,7 @@ SIGNATURES = {
     "cx_tree_build_telemetry": (_i32, [_p, _p, _i64, _p]),
     "cx_tree_check_invariants": (_i32, [_p, _p]),
     "cx_tree_debug_set_node": (_i32, [_p, _i64, _i64, _i32]),
+    "cx_expf_cr_range": (_i32, [C.c_uint32, _i64, _p]),
     "cx_tree_set_query_telemetry": (_i32, [_p, _i32]),
     "cx_tree_last_query_telemetry": (_i32, [_p, _p, _i32, _p]),
     "cx_tree_clear_last_query_telemetry": (_i32, [_p]),

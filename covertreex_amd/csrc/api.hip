@@ -534,6 +534,19 @@ cx_status cx_nystrom_residual(const double *points, int64_t n, int32_t d,
     });
 }
 
+cx_status cx_expf_cr_range(uint32_t first_bits, int64_t count, float *out)
+{
+    return guarded([&] {
+        CX_REQUIRE(count >= 0 && count <= ((int64_t)1 << 32) - (int64_t)first_bits,
+                   "count must be in [0, 2^32 - first_bits]");
+        if (count == 0)
+            return;
+        CX_REQUIRE(out != nullptr, "NULL pointer");
+        require_device();
+        expf_cr_range(first_bits, count, out);
+    });
+}
+
 cx_status cx_tree_set_screen_segment(cx_tree *t, int32_t seg)
 {
     return guarded([&] {

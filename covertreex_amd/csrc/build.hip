@@ -2377,4 +2377,21 @@ void check_invariants(TreeState &t, int64_t *out)
         out[f] = h[f];
 }
 
+// expf_cr of the floats whose bit patterns follow first_bits (the exp of
+// the build and the reference-form metric, exposed for its exhaustive check)
+__global__ void expf_cr_range_kernel(uint32_t first_bits, int64_t count, float *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        out[i] = expf_cr(__builtin_bit_cast(float, first_bits + (uint32_t)i));
+}
+
+void expf_cr_range(uint32_t first_bits, int64_t count, float *out)
+{
+    DevBuf<float> d((size_t)count);
+    expf_cr_range_kernel<<<nblocks(count, 256), 256>>>(first_bits, count, d.ptr);
+    CX_HIP(hipGetLastError());
+    CX_HIP(hipMemcpy(out, d.ptr, sizeof(float) * count, hipMemcpyDeviceToHost));
+}
+
 } // namespace cx

@@ -272,6 +272,9 @@ void insert_residual_rows(TreeState &t, const float *V, const float *p, const fl
                           int64_t n_new, int32_t chunk);
 void insert_euclidean_rows(TreeState &t, const float *points, int64_t n_new, int32_t chunk);
 void delete_rows(TreeState &t, const int64_t *remove, int64_t n_remove, bool by_node);
+// expf_cr of count consecutive bit patterns, computed on the device, into
+// host memory
+void expf_cr_range(uint32_t first_bits, int64_t count, float *out);
 // Cover-tree invariants of the tree (CX_INV_* counts into out).
 void check_invariants(TreeState &t, int64_t *out);
 void morton_order_device(const float *coords_host, int64_t n, int32_t d, int64_t *order_host);

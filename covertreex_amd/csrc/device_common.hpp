@@ -9,49 +9,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "exp_cr.hpp"
+
 #pragma clang fp contract(off)
 
 namespace cx {
 
 constexpr int32_t kLevelUnset = INT32_MIN;
-
-// Double-evaluated expf: Cody-Waite reduction x = k*ln2 + r with fdlibm's
-// ln2 hi/lo split (k*ln2_hi exact for |k| < 2^21), exp(r) by the degree-13
-// Taylor polynomial in Horner form (truncation < 5e-18 for |r| <= ln2/2),
-// scaling by 2^k and a single rounding to float.  Fixed IEEE double
-// operations: identical bits on any IEEE machine without contraction, and
-// equal to (float)exp((double)x) for every float in [-110, 88.5].  The
-// reference's expf (glibc) also evaluates in double internally.
-__device__ __forceinline__ float expf_dd(float x)
-{
-    if (x != x)
-        return x;
-    if (x > 89.0f)
-        return __builtin_huge_valf();
-    if (x < -110.0f)
-        return 0.0f;
-    const double inv_ln2 = 1.4426950408889634074;
-    const double ln2_hi = 6.93147180369123816490e-01;
-    const double ln2_lo = 1.90821492927058770002e-10;
-    double xd = (double)x;
-    double k = __builtin_rint(xd * inv_ln2);
-    double r = (xd - k * ln2_hi) - k * ln2_lo;
-    double p = 1.0 / 6227020800.0;
-    p = p * r + 1.0 / 479001600.0;
-    p = p * r + 1.0 / 39916800.0;
-    p = p * r + 1.0 / 3628800.0;
-    p = p * r + 1.0 / 362880.0;
-    p = p * r + 1.0 / 40320.0;
-    p = p * r + 1.0 / 5040.0;
-    p = p * r + 1.0 / 720.0;
-    p = p * r + 1.0 / 120.0;
-    p = p * r + 1.0 / 24.0;
-    p = p * r + 1.0 / 6.0;
-    p = p * r + 0.5;
-    p = p * r + 1.0;
-    p = p * r + 1.0;
-    return (float)__builtin_ldexp(p, (int)k);
-}
 
 // Read-only metric data in tree order.
 struct MetricView {
@@ -170,12 +134,12 @@ __device__ __forceinline__ float dist_sq_ref_dot(const MetricView &mv, int64_t i
         d2 = 0.0f;
     float kval;
     if (mv.kernel == 0) {
-        kval = mv.var * expf_dd(-0.5f * d2);
+        kval = mv.var * expf_cr(-0.5f * d2);
     } else {
         float r = __builtin_sqrtf(d2);
         float a = 2.2360679775f * r;
         float poly = 1.0f + a + (a * a) / 3.0f;
-        kval = mv.var * poly * expf_dd(-a);
+        kval = mv.var * poly * expf_cr(-a);
     }
     float den2 = mv.p[i] * mv.p[j];
     const float eps = 1e-9f;
@@ -222,12 +186,12 @@ __device__ __forceinline__ float dist_batch_ref(const MetricView &mv, int64_t q,
         d2 = 0.0f;
     float kval;
     if (mv.kernel == 0) {
-        kval = mv.var * expf_dd(-0.5f * d2);
+        kval = mv.var * expf_cr(-0.5f * d2);
     } else {
         float r = __builtin_sqrtf(d2);
         float a = 2.2360679775f * r;
         float poly = 1.0f + a + (a * a) * 0.3333333333f;
-        kval = mv.var * poly * expf_dd(-a);
+        kval = mv.var * poly * expf_cr(-a);
     }
     float prod = mv.p[q] * mv.p[j];
     const float eps = 1e-9f;

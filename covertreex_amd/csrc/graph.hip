@@ -173,15 +173,17 @@ void hilbert_order_noalloc(const double *d_points, int64_t n, int32_t d, int64_t
 // ---- Luby MIS ----------------------------------------------------------------
 
 // xorshift priority (_mis_numba.py:19-28) with 32-bit wrapping as in the JAX
-// path (covertreex/algo/mis.py:146-168)
-__device__ __forceinline__ double luby_priority(int64_t node, uint32_t seed, int64_t iteration)
+// path (covertreex/algo/mis.py:146-168).  The reference's priority is
+// (x + 1) / 2^32 in float64, exact and strictly increasing in x, so the
+// 32-bit word x orders the nodes the same way.
+__device__ __forceinline__ uint32_t luby_priority(int64_t node, uint32_t seed, int64_t iteration)
 {
     uint32_t x = (uint32_t)node ^ seed;
     x ^= (uint32_t)((uint64_t)iteration * 0x9E3779B9ull);
     x ^= x << 13;
     x ^= x >> 17;
     x ^= x << 5;
-    return ((double)x + 1.0) / 4294967296.0;
+    return x;
 }
 
 __global__ void luby_round_kernel(const int64_t *__restrict__ indptr,
@@ -195,12 +197,12 @@ __global__ void luby_round_kernel(const int64_t *__restrict__ indptr,
     win[v] = 0;
     if (!active[v])
         return;
-    const double pv = luby_priority(v, seed, iteration);
+    const uint32_t pv = luby_priority(v, seed, iteration);
     for (int64_t e = indptr[v]; e < indptr[v + 1]; e++) {
         const int64_t u = indices[e];
         if (!active[u])
             continue;
-        const double pu = luby_priority(u, seed, iteration);
+        const uint32_t pu = luby_priority(u, seed, iteration);
         if (pu > pv || (pu == pv && u < v))
             return;
     }

@@ -327,6 +327,15 @@ cx_status cx_tree_check_invariants(cx_tree *t, int64_t *out);
  * meaningless. */
 cx_status cx_tree_debug_set_node(cx_tree *t, int64_t node, int64_t parent, int32_t level);
 
+/* The exp of the build's and the reference form's kernel values, which
+ * replaces the platform expf the reference's metric calls
+ * (src/metric.rs:183,196,458,464,534,541): binary32 exp correctly rounded,
+ * computed in FP32 arithmetic.  Evaluates it on the device for the count
+ * floats whose bit patterns follow first_bits (first_bits + count <= 2^32)
+ * and writes the results to host memory out.  For checking the routine
+ * against an oracle. */
+cx_status cx_expf_cr_range(uint32_t first_bits, int64_t count, float *out);
+
 /* *n_out = CX_QT_SIZE when the last call recorded telemetry, else 0. */
 cx_status cx_tree_last_query_telemetry(const cx_tree *t, int64_t *out, int32_t cap,
                                        int32_t *n_out);

@@ -92,6 +92,8 @@ def lib():
         L.ox_metric_free.argtypes = [_p]
         L.ox_expf.restype = _f32
         L.ox_expf.argtypes = [_i32, _f32]
+        L.ox_expf_range.argtypes = [_i32, C.c_uint32, _i64, _p]
+        L.ox_expf_array.argtypes = [_i32, _p, _i64, _p]
         L.ox_dist_sq_pairs.argtypes = [_p, _p, _p, _i64, _p]
         L.ox_dist_batch_pairs.argtypes = [_p, _p, _p, _i64, _p]
         L.ox_morton_order.argtypes = [_i64, _i32, _p, _p, _p]
@@ -156,6 +158,21 @@ def euclidean_metric(points) -> _Metric:
 
 def expf(x: float, mode: int = EXP_LIBM) -> float:
     return float(lib().ox_expf(mode, float(x)))
+
+
+def expf_array(x, mode: int = EXP_CR) -> np.ndarray:
+    """expf of every element of a float32 array."""
+    x = _c(x, np.float32)
+    out = np.empty(x.shape, np.float32)
+    lib().ox_expf_array(mode, _ptr(x), x.size, _ptr(out))
+    return out
+
+
+def expf_range(first_bits: int, count: int, mode: int = EXP_CR) -> np.ndarray:
+    """expf of the `count` floats whose bit patterns follow `first_bits`."""
+    out = np.empty(count, np.float32)
+    lib().ox_expf_range(mode, first_bits, count, _ptr(out))
+    return out
 
 
 def dist_sq_pairs(metric: _Metric, a, b) -> np.ndarray:

@@ -49,6 +49,24 @@ float ox_expf(int32_t mode, float x)
     return expf(x);
 }
 
+void ox_expf_range(int32_t mode, uint32_t first_bits, int64_t count, float *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < count; i++) {
+        const uint32_t b = first_bits + (uint32_t)i;
+        float x;
+        memcpy(&x, &b, sizeof x);
+        out[i] = ox_expf(mode, x);
+    }
+}
+
+void ox_expf_array(int32_t mode, const float *x, int64_t n, float *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++)
+        out[i] = ox_expf(mode, x[i]);
+}
+
 /* ResidualMetric::new, src/metric.rs:84-177 (f32 instantiation). */
 ox_metric *ox_metric_residual(int64_t n, int32_t m, int32_t d, const float *coords,
                               const float *v, const float *p, float var,

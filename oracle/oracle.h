@@ -68,6 +68,9 @@ ox_metric *ox_metric_residual(int64_t n, int32_t m, int32_t d, const float *coor
 ox_metric *ox_metric_euclidean(int64_t n, int32_t d, const float *points);
 void ox_metric_free(ox_metric *mt);
 float ox_expf(int32_t mode, float x);
+/* ox_expf of the count floats whose bit patterns follow first_bits */
+void ox_expf_range(int32_t mode, uint32_t first_bits, int64_t count, float *out);
+void ox_expf_array(int32_t mode, const float *x, int64_t n, float *out);
 float ox_dist_sq(const ox_metric *mt, int64_t i, int64_t j);     /* build / roots */
 float ox_dist_batch(const ox_metric *mt, int64_t q, int64_t j);  /* query tiles */
 void ox_dist_sq_pairs(const ox_metric *mt, const int64_t *a, const int64_t *b,
