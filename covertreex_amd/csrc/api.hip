@@ -139,6 +139,117 @@ void release_event(hipEvent_t e)
     g_events.push_back(e);
 }
 
+namespace {
+struct BigBlock {
+    void *p;
+    size_t bytes;
+    int device;
+};
+std::mutex g_big_mu;
+std::vector<BigBlock> g_big_free;
+std::vector<BigBlock> g_big_live;
+size_t g_big_free_bytes = 0;
+// cached free bytes kept per process; beyond it the largest free blocks are
+// released
+constexpr size_t kBigCacheCap = (size_t)48 << 30;
+} // namespace
+
+void *big_alloc(size_t bytes, hipStream_t)
+{
+    int dev = 0;
+    CX_HIP(hipGetDevice(&dev));
+    const size_t grain = (size_t)2 << 20;
+    bytes = (bytes + grain - 1) & ~(grain - 1);
+    // best fit among blocks up to half again (+64 MiB) the request
+    const size_t slack = bytes / 2 + ((size_t)64 << 20);
+    BigBlock b{};
+    bool found = false;
+    {
+        std::lock_guard<std::mutex> lock(g_big_mu);
+        size_t best = 0;
+        for (size_t i = 0; i < g_big_free.size(); i++) {
+            const BigBlock &c = g_big_free[i];
+            if (c.device == dev && c.bytes >= bytes && c.bytes <= bytes + slack &&
+                (!found || c.bytes < g_big_free[best].bytes)) {
+                best = i;
+                found = true;
+            }
+        }
+        if (found) {
+            b = g_big_free[best];
+            g_big_free.erase(g_big_free.begin() + (ptrdiff_t)best);
+            g_big_free_bytes -= b.bytes;
+        }
+    }
+    if (!found) {
+        b.bytes = bytes;
+        b.device = dev;
+        hipError_t e = hipMalloc(&b.p, bytes);
+        if (e == hipErrorOutOfMemory) {
+            // give the cached blocks of this device back and try once more
+            (void)hipGetLastError();
+            std::vector<BigBlock> drop;
+            {
+                std::lock_guard<std::mutex> lock(g_big_mu);
+                for (size_t i = 0; i < g_big_free.size();)
+                    if (g_big_free[i].device == dev) {
+                        drop.push_back(g_big_free[i]);
+                        g_big_free_bytes -= g_big_free[i].bytes;
+                        g_big_free.erase(g_big_free.begin() + (ptrdiff_t)i);
+                    } else {
+                        i++;
+                    }
+            }
+            for (const BigBlock &c : drop)
+                (void)hipFree(c.p);
+            e = hipMalloc(&b.p, bytes);
+        }
+        CX_HIP(e);
+    }
+    std::lock_guard<std::mutex> lock(g_big_mu);
+    g_big_live.push_back(b);
+    return b.p;
+}
+
+bool big_free(void *p, hipStream_t s)
+{
+    BigBlock b{};
+    {
+        std::lock_guard<std::mutex> lock(g_big_mu);
+        size_t i = 0;
+        while (i < g_big_live.size() && g_big_live[i].p != p)
+            i++;
+        if (i == g_big_live.size())
+            return false;
+        b = g_big_live[i];
+        g_big_live.erase(g_big_live.begin() + (ptrdiff_t)i);
+    }
+    // the freeing stream's work on the block ends before it is cached, so
+    // any stream may take it at once (no cross-stream event, which a stream
+    // captured into a graph elsewhere could not wait on); the frees come at
+    // a tree's end or when a buffer grows, where the stream is drained or
+    // about to be
+    (void)hipStreamSynchronize(s);
+    std::vector<BigBlock> drop;
+    {
+        std::lock_guard<std::mutex> lock(g_big_mu);
+        g_big_free.push_back(b);
+        g_big_free_bytes += b.bytes;
+        while (g_big_free_bytes > kBigCacheCap) {
+            size_t big = 0;
+            for (size_t i = 1; i < g_big_free.size(); i++)
+                if (g_big_free[i].bytes > g_big_free[big].bytes)
+                    big = i;
+            drop.push_back(g_big_free[big]);
+            g_big_free_bytes -= g_big_free[big].bytes;
+            g_big_free.erase(g_big_free.begin() + (ptrdiff_t)big);
+        }
+    }
+    for (const BigBlock &c : drop)
+        (void)hipFree(c.p);
+    return true;
+}
+
 void ensure_stream_pool(int device)
 {
     static std::mutex mu;

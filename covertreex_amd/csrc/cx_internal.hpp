@@ -53,6 +53,21 @@ struct StreamScope {
 // per device.
 void ensure_stream_pool(int device);
 
+// Stream-ordered buffers of kBigBlock bytes or more come from a process-wide
+// cache of device blocks instead of the stream-ordered pool: a block is
+// cached once the freeing stream has drained, and the next allocation of
+// about its size on the same device (any stream) takes it.  Trees of a
+// serving loop allocate and free the same large transient buffers (pair
+// codes, candidate lists) request after request; a fresh allocation of that
+// size from the stream-ordered pool has blocked the host for up to a second,
+// a cached block costs a lookup.  Blocks stay cached for the process, up to
+// 48 GiB of free blocks (the largest go first beyond that, and all of a
+// device's when an allocation would otherwise fail).
+constexpr size_t kBigBlock = (size_t)32 << 20;
+void *big_alloc(size_t bytes, hipStream_t s);
+// returns false when p is not a cached-block allocation
+bool big_free(void *p, hipStream_t s);
+
 // Process-wide pools of the per-tree runtime objects whose creation or
 // destruction can wait on other work on the device (pinned host memory, and
 // for symmetry streams and events): a tree takes them at first use and gives
@@ -94,10 +109,13 @@ template <typename T> struct DevBuf {
     static T *raw_alloc(size_t n, hipStream_t s)
     {
         T *q = nullptr;
-        if (s)
-            CX_HIP(hipMallocAsync((void **)&q, n * sizeof(T) + 256, s));
+        const size_t bytes = n * sizeof(T) + 256;
+        if (s && bytes >= kBigBlock)
+            q = (T *)big_alloc(bytes, s);
+        else if (s)
+            CX_HIP(hipMallocAsync((void **)&q, bytes, s));
         else
-            CX_HIP(hipMalloc((void **)&q, n * sizeof(T) + 256));
+            CX_HIP(hipMalloc((void **)&q, bytes));
         return q;
     }
     void alloc(size_t n)
@@ -111,10 +129,12 @@ template <typename T> struct DevBuf {
     void release()
     {
         if (ptr) {
-            if (st)
-                (void)hipFreeAsync(ptr, st);
-            else
+            if (st) {
+                if (!big_free(ptr, st))
+                    (void)hipFreeAsync(ptr, st);
+            } else {
                 (void)hipFree(ptr);
+            }
         }
         ptr = nullptr;
         count = 0;
@@ -126,10 +146,14 @@ template <typename T> struct DevBuf {
     {
         T *np = n ? raw_alloc(n, s) : nullptr;
         const size_t keep = count < n ? count : n;
+        if (st && st != s)
+            CX_HIP(hipStreamSynchronize(st)); // its work on the old buffer
         if (keep)
             CX_HIP(hipMemcpyAsync(np, ptr, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
         if (!st)
             CX_HIP(hipStreamSynchronize(s));
+        else
+            st = s; // the old buffer is freed after the copy
         release();
         ptr = np;
         count = n;
