@@ -1457,8 +1457,10 @@ constexpr int64_t kHsBudget = (int64_t)512 << 20; // bytes of stored bounds per 
 // list counters one 128-byte line apart: each takes one atomic per listed
 // pair, and counters sharing a line would serialize on one L2 channel
 constexpr int kHsCntStride = 32;
-// overflowed queries above which hs_bounds_kernel prefetches its next K step
-constexpr int32_t kHsPrefetchMin = 1024;
+// overflowed queries above which the exact-product screen takes the wide
+// tiles (hs_bounds_wide_kernel, next K step prefetched); fewer run beside the
+// rescore in the small tiles, which slow it less
+constexpr int32_t kHsWideMin = 1024;
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
@@ -1528,7 +1530,6 @@ __device__ __forceinline__ df df_of_int(int32_t g)
 // Pass 1a.  Workgroup (segment x, query tile y) = 32 overflowed queries (MFMA
 // columns) x a segment of 256 candidates in tiles of 128 (four wavefronts x
 // 32 MFMA rows); stores each valid pair's dot as a double-float.
-template <bool kPf>
 __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
 {
     __shared__ __attribute__((aligned(16))) int8_t sA[kSlices * kHsTile * kHsRow];
@@ -1574,76 +1575,26 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
 #pragma unroll
         for (int t = 0; t < 6; t++)
             G[t] = v16i{};
-        // candidate slices: 128 rows x 6 slices x 2 halves of 16 bytes (six
-        // per thread), query slices: 32 rows x 6 x 2 (one or two per thread).
-        // kPf: the next K step's loads fly while this one is multiplied.  That
-        // pays when many queries overflowed; a few run beside the rescore,
-        // which the form without prefetch slows less.
-        constexpr int kAU = kHsTile * kSlices * 2 / 256, kBU = (32 * kSlices * 2 + 255) / 256;
-        static_assert(kAU * 256 == kHsTile * kSlices * 2, "candidate staging splits evenly");
-        // byte offsets of staged segment i of this thread: global (without
-        // the K offset) and LDS; the query slices are 32 x 12 = 384 segments,
-        // so a thread's second query segment past them reloads a valid one
-        // and is not stored
-#define HS_GA(i) ((((c0 + (tid + 256 * (i)) / (kSlices * 2)) < n                                   \
-                        ? c0 + (tid + 256 * (i)) / (kSlices * 2)                                    \
-                        : (int64_t)0) * kSlices + ((tid + 256 * (i)) % (kSlices * 2) >> 1)) * mp +  \
-                  16 * ((tid + 256 * (i)) & 1))
-#define HS_LA(i) ((((tid + 256 * (i)) % (kSlices * 2) >> 1) * kHsTile +                            \
-                   (tid + 256 * (i)) / (kSlices * 2)) * kHsRow + 16 * ((tid + 256 * (i)) & 1))
-#define HS_BU(i) (tid + 256 * (i) < 32 * kSlices * 2 ? tid + 256 * (i) : tid)
-#define HS_GB(i) (((int64_t)sQrow[HS_BU(i) / (kSlices * 2)] * kSlices +                            \
-                   (HS_BU(i) % (kSlices * 2) >> 1)) * mp + 16 * (HS_BU(i) & 1))
-#define HS_LB(i) (((HS_BU(i) % (kSlices * 2) >> 1) * 32 + HS_BU(i) / (kSlices * 2)) * kHsRow +   \
-                  16 * (HS_BU(i) & 1))
-        // named registers, not arrays: a loop-carried array lands in scratch
-        static_assert(kAU == 6 && kBU == 2, "six candidate and two query segments per thread");
-        [[maybe_unused]] uint4 pa0, pa1, pa2, pa3, pa4, pa5, pb0, pb1;
-#define HS_LOAD(o)                                                                                \
-    do {                                                                                          \
-        pa0 = *(const uint4 *)(H.sl + HS_GA(0) + (o));                                            \
-        pa1 = *(const uint4 *)(H.sl + HS_GA(1) + (o));                                            \
-        pa2 = *(const uint4 *)(H.sl + HS_GA(2) + (o));                                            \
-        pa3 = *(const uint4 *)(H.sl + HS_GA(3) + (o));                                            \
-        pa4 = *(const uint4 *)(H.sl + HS_GA(4) + (o));                                            \
-        pa5 = *(const uint4 *)(H.sl + HS_GA(5) + (o));                                            \
-        pb0 = *(const uint4 *)(H.sl + HS_GB(0) + (o));                                            \
-        pb1 = *(const uint4 *)(H.sl + HS_GB(1) + (o));                                            \
-    } while (0)
-        if constexpr (kPf)
-            HS_LOAD(0);
+        // candidate slices: 128 rows x 6 slices x 2 halves of 16 bytes,
+        // query slices: 32 rows x 6 x 2
         for (int32_t kc = 0; kc < mp; kc += 32) {
             __syncthreads();
-            if constexpr (kPf) {
-                *(uint4 *)(sA + HS_LA(0)) = pa0;
-                *(uint4 *)(sA + HS_LA(1)) = pa1;
-                *(uint4 *)(sA + HS_LA(2)) = pa2;
-                *(uint4 *)(sA + HS_LA(3)) = pa3;
-                *(uint4 *)(sA + HS_LA(4)) = pa4;
-                *(uint4 *)(sA + HS_LA(5)) = pa5;
-                *(uint4 *)(sB + HS_LB(0)) = pb0; // tid < 256 <= 32 * 12: always a segment
-                if (tid + 256 < 32 * kSlices * 2)
-                    *(uint4 *)(sB + HS_LB(1)) = pb1;
-                if (kc + 32 < mp)
-                    HS_LOAD(kc + 32);
-            } else {
-                for (int u = tid; u < kHsTile * kSlices * 2; u += 256) {
-                    const int row = u / (kSlices * 2), rem = u % (kSlices * 2);
-                    const int a = rem >> 1, half = rem & 1;
-                    const int64_t j = c0 + row;
-                    const int64_t jr = j < n ? j : 0;
-                    const uint4 val =
-                        *(const uint4 *)(H.sl + (jr * kSlices + a) * mp + kc + 16 * half);
-                    *(uint4 *)(sA + (a * kHsTile + row) * kHsRow + 16 * half) = val;
-                }
-                for (int u = tid; u < 32 * kSlices * 2; u += 256) {
-                    const int row = u / (kSlices * 2), rem = u % (kSlices * 2);
-                    const int a = rem >> 1, half = rem & 1;
-                    const uint4 val =
-                        *(const uint4 *)(H.sl + ((int64_t)sQrow[row] * kSlices + a) * mp + kc +
-                                         16 * half);
-                    *(uint4 *)(sB + (a * 32 + row) * kHsRow + 16 * half) = val;
-                }
+            for (int u = tid; u < kHsTile * kSlices * 2; u += 256) {
+                const int row = u / (kSlices * 2), rem = u % (kSlices * 2);
+                const int a = rem >> 1, half = rem & 1;
+                const int64_t j = c0 + row;
+                const int64_t jr = j < n ? j : 0;
+                const uint4 val =
+                    *(const uint4 *)(H.sl + (jr * kSlices + a) * mp + kc + 16 * half);
+                *(uint4 *)(sA + (a * kHsTile + row) * kHsRow + 16 * half) = val;
+            }
+            for (int u = tid; u < 32 * kSlices * 2; u += 256) {
+                const int row = u / (kSlices * 2), rem = u % (kSlices * 2);
+                const int a = rem >> 1, half = rem & 1;
+                const uint4 val =
+                    *(const uint4 *)(H.sl + ((int64_t)sQrow[row] * kSlices + a) * mp + kc +
+                                     16 * half);
+                *(uint4 *)(sB + (a * 32 + row) * kHsRow + 16 * half) = val;
             }
             __syncthreads();
             // any consistent K order works for A and B: the sums are exact
@@ -1664,12 +1615,6 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
                                                                          G[t - 2], 0, 0, 0);
                 }
         }
-#undef HS_LOAD
-#undef HS_GA
-#undef HS_LA
-#undef HS_BU
-#undef HS_GB
-#undef HS_LB
         // register r of lane l: candidate row 8(r/4) + 4(l/32) + r%4 of the
         // wavefront's 32, query column l%32
 #pragma unroll
@@ -1679,6 +1624,137 @@ __global__ void __launch_bounds__(256, 2) hs_bounds_kernel(HsArgs H)
             if (!qlive || j >= lim)
                 continue;
             const int32_t ej = sEj[32 * w + rw];
+            const int es = eq + ej;
+            df dot = {0.0f, 0.0f};
+#pragma unroll
+            for (int t = 0; t < 6; t++) {
+                const df g = df_of_int(G[t][r]);
+                const int sh = es - (12 + 7 * t);
+                dot = df_add(dot, df{__builtin_ldexpf(g.hi, sh), __builtin_ldexpf(g.lo, sh)});
+            }
+            H.bounds[(int64_t)hq * n + j] = make_float2(dot.hi, dot.lo);
+        }
+    }
+}
+
+// Pass 1a for many overflowed queries: the same pairs and sums with larger
+// workgroup tiles, 128 queries x 64 candidates (eight wavefronts, two
+// candidate rows of 32 by four query columns of 32), so each staged
+// candidate slice serves 128 queries instead of 32 and a K step stages 4.5
+// bytes per pair instead of 7.5.  Segments of 256 candidates.
+constexpr int kHwQ = 128, kHwC = 64, kHwSeg = 256;
+
+__global__ void __launch_bounds__(512, 1) hs_bounds_wide_kernel(HsArgs H)
+{
+    __shared__ __attribute__((aligned(16))) int8_t sA[kSlices * kHwC * kHsRow];
+    __shared__ __attribute__((aligned(16))) int8_t sB[kSlices * kHwQ * kHsRow];
+    __shared__ int32_t sQrow[kHwQ], sEj[kHwC];
+    __shared__ int64_t s_lmax[8];
+    const RescoreArgs &A = H.r;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int cb = w & 1, qb = w >> 1; // candidate row block, query column block
+    const int64_t n = A.n;
+    const int32_t mp = H.mp;
+    const int hq = (int)blockIdx.y * kHwQ + 32 * qb + (lane & 31);
+    const bool qlive = hq < H.hn;
+    const int64_t s = qlive ? H.hard[H.h0 + hq] : 0;
+    const int64_t lim = qlive ? hard_limit(A, s) : 0;
+    const int32_t qrow = qlive ? A.qrow[s] : 0;
+    const int32_t eq = H.sexp[qrow];
+    if (tid < kHwQ) {
+        const int h = (int)blockIdx.y * kHwQ + tid;
+        sQrow[tid] = h < H.hn ? A.qrow[H.hard[H.h0 + h]] : 0;
+    }
+    // the segment ends at the largest limit of the 128 queries
+    int64_t lmax = lim;
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) {
+        const int64_t u = __shfl_xor(lmax, o, 64);
+        lmax = u > lmax ? u : lmax;
+    }
+    if (lane == 0)
+        s_lmax[w] = lmax;
+    __syncthreads();
+    lmax = s_lmax[0];
+    for (int i = 1; i < 8; i++)
+        lmax = s_lmax[i] > lmax ? s_lmax[i] : lmax;
+    const int64_t seg0 = (int64_t)blockIdx.x * kHwSeg;
+    const int64_t seg1 = seg0 + kHwSeg < lmax ? seg0 + kHwSeg : lmax;
+    for (int64_t c0 = seg0; c0 < seg1; c0 += kHwC) {
+        __syncthreads(); // the previous tile's epilogue still reads sEj
+        if (tid < kHwC) {
+            const int64_t j = c0 + tid < n ? c0 + tid : 0;
+            sEj[tid] = H.sexp[j];
+        }
+        v16i G[6];
+#pragma unroll
+        for (int t = 0; t < 6; t++)
+            G[t] = v16i{};
+        // staged 16-byte pieces: candidates 64 rows x 6 slices x 2 halves
+        // (768: pieces tid and tid + 512 when below), queries 128 x 12 (1536:
+        // tid, tid + 512, tid + 1024); the next K step's loads fly while this
+        // one is multiplied
+#define HW_GA(u) ((((c0 + (u) / (kSlices * 2)) < n ? c0 + (u) / (kSlices * 2) : (int64_t)0) *      \
+                       kSlices + ((u) % (kSlices * 2) >> 1)) * mp + 16 * ((u) & 1))
+#define HW_LA(u) ((((u) % (kSlices * 2) >> 1) * kHwC + (u) / (kSlices * 2)) * kHsRow + 16 * ((u) & 1))
+#define HW_GB(u) (((int64_t)sQrow[(u) / (kSlices * 2)] * kSlices + ((u) % (kSlices * 2) >> 1)) * mp + \
+                  16 * ((u) & 1))
+#define HW_LB(u) ((((u) % (kSlices * 2) >> 1) * kHwQ + (u) / (kSlices * 2)) * kHsRow + 16 * ((u) & 1))
+        const bool a1 = tid + 512 < kHwC * kSlices * 2;
+        uint4 pa0, pa1 = make_uint4(0u, 0u, 0u, 0u), pb0, pb1, pb2;
+#define HW_LOAD(o)                                                                                \
+    do {                                                                                          \
+        pa0 = *(const uint4 *)(H.sl + HW_GA(tid) + (o));                                          \
+        if (a1)                                                                                   \
+            pa1 = *(const uint4 *)(H.sl + HW_GA(tid + 512) + (o));                                \
+        pb0 = *(const uint4 *)(H.sl + HW_GB(tid) + (o));                                          \
+        pb1 = *(const uint4 *)(H.sl + HW_GB(tid + 512) + (o));                                    \
+        pb2 = *(const uint4 *)(H.sl + HW_GB(tid + 1024) + (o));                                   \
+    } while (0)
+        HW_LOAD(0);
+        for (int32_t kc = 0; kc < mp; kc += 32) {
+            __syncthreads();
+            *(uint4 *)(sA + HW_LA(tid)) = pa0;
+            if (a1)
+                *(uint4 *)(sA + HW_LA(tid + 512)) = pa1;
+            *(uint4 *)(sB + HW_LB(tid)) = pb0;
+            *(uint4 *)(sB + HW_LB(tid + 512)) = pb1;
+            *(uint4 *)(sB + HW_LB(tid + 1024)) = pb2;
+            if (kc + 32 < mp)
+                HW_LOAD(kc + 32);
+            __syncthreads();
+            // any consistent K order works for A and B: the sums are exact
+            const int koff = 16 * (lane >> 5);
+            v4i av[kSlices], bv[kSlices];
+#pragma unroll
+            for (int a = 0; a < kSlices; a++) {
+                av[a] = *(const v4i *)(sA + (a * kHwC + 32 * cb + (lane & 31)) * kHsRow + koff);
+                bv[a] = *(const v4i *)(sB + (a * kHwQ + 32 * qb + (lane & 31)) * kHsRow + koff);
+            }
+#pragma unroll
+            for (int t = 2; t <= 7; t++)
+#pragma unroll
+                for (int a = 1; a <= 6; a++) {
+                    const int b = t - a;
+                    if (b >= 1 && b <= 6)
+                        G[t - 2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a - 1], bv[b - 1],
+                                                                         G[t - 2], 0, 0, 0);
+                }
+        }
+#undef HW_LOAD
+#undef HW_GA
+#undef HW_LA
+#undef HW_GB
+#undef HW_LB
+        // register r of lane l: candidate row 8(r/4) + 4(l/32) + r%4 of the
+        // wavefront's 32, query column l%32
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int rw = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int64_t j = c0 + 32 * cb + rw;
+            if (!qlive || j >= lim)
+                continue;
+            const int32_t ej = sEj[32 * cb + rw];
             const int es = eq + ej;
             df dot = {0.0f, 0.0f};
 #pragma unroll
@@ -1906,10 +1982,13 @@ static void rescreen_hard(TreeState &t, const RescoreArgs &ra, int32_t *hard, in
         CX_HIP(hipMemsetAsync(H.hcnt, 0, (size_t)H.hn * kHsCntStride * sizeof(int32_t), st));
         // x: candidate segments (up to 2^31 - 1), y: tiles of 32 queries
         const dim3 g1((unsigned)((n + kHsSeg - 1) / kHsSeg), (unsigned)((H.hn + 31) / 32));
-        if (n_hard > kHsPrefetchMin)
-            hs_bounds_kernel<true><<<g1, 256, 0, st>>>(H);
-        else
-            hs_bounds_kernel<false><<<g1, 256, 0, st>>>(H);
+        if (n_hard > kHsWideMin) {
+            const dim3 gw((unsigned)((n + kHwSeg - 1) / kHwSeg),
+                          (unsigned)((H.hn + kHwQ - 1) / kHwQ));
+            hs_bounds_wide_kernel<<<gw, 512, 0, st>>>(H);
+        } else {
+            hs_bounds_kernel<<<g1, 256, 0, st>>>(H);
+        }
         if (h0 == 0)
             after_first();
         const dim3 g2((unsigned)std::min<int64_t>((n + 255) / 256, 64), (unsigned)H.hn);
