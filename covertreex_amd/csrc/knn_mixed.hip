@@ -549,13 +549,16 @@ __global__ void __launch_bounds__(kScreenThreads, 2) screen_kernel(ScreenArgs A)
                             // or not at all: tau stays an upper bound
                             const float binf = __builtin_fmaf(rlo, -(float)kBins,
                                                               (float)kBins * (1.0f + 1.5e-7f));
-                            if (valid && binf < (float)kBins * kHistCap) {
+                            // (one atomic per pair, adding zero when not
+                            // counted: no branch between the pairs)
+                            {
+                                const bool in = valid && binf < (float)kBins * kHistCap;
                                 const int bin = __builtin_amdgcn_fmed3f(binf, 0.0f, 127.0f);
                                 if (kWide)
-                                    atomicAdd(&sHist[qcol[b] * kRow + bin], 1u);
+                                    atomicAdd(&sHist[qcol[b] * kRow + bin], in ? 1u : 0u);
                                 else
                                     atomicAdd(&sHist[qcol[b] * kRow + (bin >> 1)],
-                                              (bin & 1) ? 65536u : 1u);
+                                              in ? ((bin & 1) ? 65536u : 1u) : 0u);
                             }
                             // code = floor(256 * squared lower bound) with the
                             // bound 1 - rhi - 2.3e-7 (the 2.3e-7 covers the
