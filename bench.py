@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per screen_kernel<3, 1> launch on this workload: rocprofv3
 # FETCH_SIZE (x2.000, calibrated on a 1 GiB copy) + WRITE_SIZE (x1.000),
 # recorded in profiles/mixed_knn_c2_n32768_m256_k50.md
-SCREEN_PASS1_HBM_BYTES = 2.933e9
+SCREEN_PASS1_HBM_BYTES = 2.894e9
 
 
 def make_inputs():
