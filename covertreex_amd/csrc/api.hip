@@ -48,6 +48,13 @@ TreeState::~TreeState()
     list_big.release();
     work.release();
     telem_buf.release();
+    eu_pts.release();
+    eu_idx.release();
+    eu_lo.release();
+    eu_hi.release();
+    eu_min.release();
+    eu_codes.release();
+    eu_bounds.release();
     if (host_word)
         release_host_word(host_word);
     if (host_word2)
