@@ -1751,12 +1751,18 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
             const int32_t dim = mv.m > 0 ? mv.m : mv.d;
             int64_t *word[2] = {(int64_t *)acquire_host_word(), (int64_t *)acquire_host_word()};
             hipEvent_t ev[2] = {acquire_event(), acquire_event()};
+            // A pooled event stays tied to the stream it was last recorded
+            // on, which may by now be another thread's, under graph capture:
+            // waiting on it there invalidates that capture.  Only events
+            // recorded here are waited on.
+            bool recorded[2] = {false, false};
             int64_t seg_end[2] = {-1, -1};
             int64_t seen_scan = 0, seen_pairs = 0, seen_b = 0;
             int slot = 0;
             auto give_back = [&]() {
                 for (int i = 0; i < 2; i++) {
-                    (void)hipEventSynchronize(ev[i]);
+                    if (recorded[i])
+                        (void)hipEventSynchronize(ev[i]);
                     release_event(ev[i]);
                     release_host_word((int32_t *)word[i]);
                 }
@@ -1782,6 +1788,7 @@ static void run_insert(TreeState &t, const MetricView &mv, int32_t use_grid, flo
                     CX_HIP(hipMemcpyAsync(word[slot], &state.ptr->ev_fused, 2 * sizeof(int64_t),
                                           hipMemcpyDeviceToHost, st));
                     CX_HIP(hipEventRecord(ev[slot], st));
+                    recorded[slot] = true;
                     seg_end[slot] = b;
                     if (b == len) {
                         decide(slot);
