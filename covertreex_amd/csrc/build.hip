@@ -56,6 +56,75 @@ __device__ __forceinline__ float build_dist_sq(const MetricView &mv, int64_t i, 
         return dist_sq_ref(mv, i, j);
 }
 
+// LDS rows of per-chunk dot sums (32 chunks of eight, padded by a word)
+constexpr int kCoopRow = 33;
+__device__ __forceinline__ bool coop_rank(int32_t m) { return (m & 7) == 0 && m >= 64 && m <= 256; }
+
+// orders the wavefront's own LDS accesses (no workgroup barrier)
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Residual distances d(q, node) of a wavefront's lanes (q the same on every
+// lane, node per lane, wanted where live; every lane calls it) with the V-row
+// dots eight lanes per pair: pass p gives lane group gr the pair of lane
+// 8p + gr, whose chunk sums (reduce8 of eight products each) go to that
+// pair's LDS row (cs: 64 rows of kCoopRow floats, this wavefront's); each
+// lane then adds its own row in chunk order from 0.0 -- dot_slice's
+// operations, so the result has build_dist_sq's bits.  Needs coop_rank(m).
+__device__ __forceinline__ float coop_wave_dist_sq(const MetricView &mv, int64_t q,
+                                                   int64_t node, bool live, float *cs)
+{
+    const int lane = threadIdx.x & 63, sub = lane & 7, gr = lane >> 3, nch = mv.m >> 3;
+    const float *va = mv.v + q * mv.m;
+    const uint64_t ball = __ballot(live);
+    for (int p = 0; p < 8; p++) {
+        const int jj = 8 * p + gr;
+        const int64_t nd = __shfl(node, jj, 64);
+        if ((ball >> jj) & 1ull) {
+            const float *vb = mv.v + nd * mv.m;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int c = sub + 8 * u;
+                if (c < nch) {
+                    const float4 a0 = *(const float4 *)(va + 8 * c);
+                    const float4 a1 = *(const float4 *)(va + 8 * c + 4);
+                    const float4 b0 = *(const float4 *)(vb + 8 * c);
+                    const float4 b1 = *(const float4 *)(vb + 8 * c + 4);
+                    const float pr[8] = {a0.x * b0.x, a0.y * b0.y, a0.z * b0.z, a0.w * b0.w,
+                                         a1.x * b1.x, a1.y * b1.y, a1.z * b1.z, a1.w * b1.w};
+                    cs[jj * kCoopRow + c] = reduce8(pr);
+                }
+            }
+        }
+    }
+    wave_lds_sync();
+    float d = 0.0f;
+    if (live) {
+        float acc = 0.0f;
+        for (int c = 0; c < nch; c++)
+            acc += cs[lane * kCoopRow + c];
+        d = dist_sq_ref_dot(mv, q, node, acc);
+    }
+    wave_lds_sync(); // the next call rewrites the rows
+    return d;
+}
+
+// build_dist_sq(q, node) where live (0 elsewhere), q the same on every lane;
+// every lane calls it.  With an LDS area (cs) and a rank coop_rank takes,
+// the dots go eight lanes per pair (coop_wave_dist_sq).
+template <int KIND>
+__device__ __forceinline__ float wave_dist_sq(const MetricView &mv, int64_t q, int64_t node,
+                                              bool live, float *cs)
+{
+    if (KIND == kMetricResidual && cs != nullptr && coop_rank(mv.m))
+        return coop_wave_dist_sq(mv, q, node, live, cs);
+    return live ? build_dist_sq<KIND>(mv, q, node) : 0.0f;
+}
+
 __device__ __forceinline__ void link_child(const TreeLinks &t, int64_t parent, int64_t child)
 {
     // CoverTreeData::link_child, src/tree.rs:98-103: prepend.
@@ -283,7 +352,8 @@ __global__ void __launch_bounds__(kBiResThreads)
 template <int KIND>
 __device__ __forceinline__ void bi_scan_body(const MetricView &mv, const TreeLinks &t, BiState *S,
                                              const int32_t *__restrict__ rk, float r2,
-                                             int64_t n_nodes, int64_t wave0, int64_t nw)
+                                             int64_t n_nodes, int64_t wave0, int64_t nw,
+                                             float *cs = nullptr)
 {
     const int nc = S->nc;
     if (nc == 0)
@@ -303,13 +373,14 @@ __device__ __forceinline__ void bi_scan_body(const MetricView &mv, const TreeLin
                 continue;
             const int pos = g * 64 + lane;
             uint64_t key = kBiFar;
+            const bool cached = pos == 0 && st;
+            const int64_t node = (pos == 0 || pos > kn) ? 0 : rk[nrk - pos];
+            float dc = wave_dist_sq<KIND>(mv, q, node, pos <= kn && !cached, cs);
             if (pos <= kn) {
-                float dc;
                 if (pos == 0) {
-                    dc = st ? S->d0[i] : build_dist_sq<KIND>(mv, q, 0);
+                    if (cached)
+                        dc = S->d0[i];
                     S->sd0[i] = dc;
-                } else {
-                    dc = build_dist_sq<KIND>(mv, q, rk[nrk - pos]);
                 }
                 if (dc <= r2)
                     key = ((uint64_t)__float_as_uint(dc) << 32) | (uint32_t)pos;
@@ -342,13 +413,13 @@ __device__ __forceinline__ void bi_scan_body(const MetricView &mv, const TreeLin
                 c = nx;
             }
             uint64_t key = kBiFar;
+            const bool cached = base == 0 && lane == 0 && st;
+            float dc = wave_dist_sq<KIND>(mv, q, mine >= 0 ? mine : q, mine >= 0 && !cached, cs);
             if (mine >= 0) {
-                float dc;
                 if (base == 0 && lane == 0) {
-                    dc = st ? S->d0[i] : build_dist_sq<KIND>(mv, q, mine);
+                    if (cached)
+                        dc = S->d0[i];
                     d0 = dc;
-                } else {
-                    dc = build_dist_sq<KIND>(mv, q, mine);
                 }
                 if (dc <= r2)
                     key = ((uint64_t)__float_as_uint(dc) << 32) | (uint32_t)(base + lane);
@@ -377,9 +448,11 @@ __global__ void __launch_bounds__(kBiParThreads)
     bi_scan_kernel(MetricView mv, TreeLinks t, BiState *S, const int32_t *__restrict__ rk,
                    float r2, int64_t n_nodes)
 {
+    __shared__ float s_cs[kBiParWaves * 64 * kCoopRow];
     bi_scan_body<KIND>(mv, t, S, rk, r2, n_nodes,
                        (int64_t)blockIdx.x * kBiParWaves + (threadIdx.x >> 6),
-                       (int64_t)gridDim.x * kBiParWaves);
+                       (int64_t)gridDim.x * kBiParWaves,
+                       s_cs + (threadIdx.x >> 6) * 64 * kCoopRow);
 }
 
 // Level 9 of a residual batch: every candidate starts at the root, so the
@@ -519,6 +592,7 @@ __global__ void __launch_bounds__(kBiParThreads)
     __shared__ int64_t s_cover, s_len, s_head;
     __shared__ int32_t s_pos;
     __shared__ float s_d0;
+    __shared__ float s_cs[kBiParWaves * 64 * kCoopRow];
     const int i = (int)blockIdx.x;
     if (i >= S->nc)
         return;
@@ -550,12 +624,15 @@ __global__ void __launch_bounds__(kBiParThreads)
         const float r = ldexpf(1.0f, level), r2 = r * r;
         if (p == 0) {
             uint64_t best = kBiFar;
-            for (int pos = tid; pos <= nrk; pos += kBiParThreads) {
-                const int64_t c = pos == 0 ? 0 : rk[nrk - pos];
-                const float dc = build_dist_sq<KIND>(mv, q, c);
+            // wave-uniform trips: every lane calls wave_dist_sq
+            for (int base = wid * 64; base <= nrk; base += kBiParThreads) {
+                const int pos = base + lane;
+                const bool in = pos <= nrk;
+                const int64_t c = (pos == 0 || !in) ? 0 : rk[nrk - pos];
+                const float dc = wave_dist_sq<KIND>(mv, q, c, in, s_cs + wid * 64 * kCoopRow);
                 if (pos == 0)
                     s_d0 = dc;
-                if (dc <= r2) {
+                if (in && dc <= r2) {
                     const uint64_t key = ((uint64_t)__float_as_uint(dc) << 32) | (uint32_t)pos;
                     best = key < best ? key : best;
                 }
@@ -593,8 +670,8 @@ __global__ void __launch_bounds__(kBiParThreads)
                     c = nx;
                 }
                 uint64_t key = kBiFar;
+                const float dc = wave_dist_sq<KIND>(mv, q, mine >= 0 ? mine : q, mine >= 0, s_cs);
                 if (mine >= 0) {
-                    const float dc = build_dist_sq<KIND>(mv, q, mine);
                     if (base == 0 && lane == 0)
                         s_d0 = dc;
                     if (dc <= r2)
@@ -669,7 +746,8 @@ __device__ __forceinline__ void bi_pairs_body(const MetricView &mv, BiState *S,
                                               float *__restrict__ dist,
                                               uint64_t *__restrict__ adj,
                                               uint64_t *__restrict__ gadj, int32_t *s_far,
-                                              int *s_cnt, int wave0, int nw, bool stamp)
+                                              int *s_cnt, int wave0, int nw, bool stamp,
+                                              float *cs = nullptr)
 {
     if (stamp && threadIdx.x == 0)
         S->tk_pairs = wall_clock64();
@@ -688,11 +766,12 @@ __device__ __forceinline__ void bi_pairs_body(const MetricView &mv, BiState *S,
             continue;
         const int j = w * 64 + lane;
         bool a = false, g = false;
-        if (j > i && j < nf) {
-            const int64_t x = s_far[i], y = s_far[j];
+        const bool live = j > i && j < nf;
+        const int64_t x = s_far[i], y = live ? s_far[j] : x;
+        const float d = wave_dist_sq<KIND>(mv, x, y, live, cs);
+        if (live) {
             if (use_grid)
                 g = grid_adjacent(raw, gd, gs, x, y);
-            const float d = build_dist_sq<KIND>(mv, x, y);
             a = g || d <= r2;
             dist[(size_t)i * kBiMaxChunk + j] = d;
         }
@@ -712,9 +791,11 @@ __global__ void __launch_bounds__(kBiParThreads)
 {
     __shared__ int32_t s_far[kBiMaxChunk];
     __shared__ int s_cnt[kBiParWaves];
+    __shared__ float s_cs[kBiParWaves * 64 * kCoopRow];
     bi_pairs_body<KIND, kBiParWaves>(mv, S, raw, gd, gs, use_grid, r2, dist, adj, gadj, s_far,
                                      s_cnt, (int)blockIdx.x * kBiParWaves + (int)(threadIdx.x >> 6),
-                                     (int)gridDim.x * kBiParWaves, blockIdx.x == 0);
+                                     (int)gridDim.x * kBiParWaves, blockIdx.x == 0,
+                                     s_cs + (threadIdx.x >> 6) * 64 * kCoopRow);
 }
 
 // *acc (LDS) += the sum of v over the workgroup.
