@@ -2055,8 +2055,19 @@ static void prepare_mixed(TreeState &t)
     t.mixed_ready = true;
 }
 
-// pair codes kept per launch group of query tiles (bytes)
+// pair codes kept per launch group of query tiles (bytes);
+// COVERTREEX_HIP_CODE_BUDGET overrides it (the tests split small calls into
+// several groups with it)
 constexpr int64_t kCodeBudget = (int64_t)4 << 30;
+static int64_t code_budget()
+{
+    if (const char *e = getenv("COVERTREEX_HIP_CODE_BUDGET")) {
+        const long long v = atoll(e);
+        if (v > 0)
+            return (int64_t)v;
+    }
+    return kCodeBudget;
+}
 // segment top-k lists of the hard queries per launch batch (bytes)
 constexpr int64_t kPartBudget = (int64_t)512 << 20;
 
@@ -2217,7 +2228,7 @@ void knn_mixed(TreeState &t, const int64_t *d_queries, int64_t nq, int32_t k, in
     // launch groups of query tiles whose pair codes fit the code budget
     const int64_t tile_bytes = (int64_t)sa.n_seg * seg_len * kQT;
     const int32_t group =
-        (int32_t)std::max<int64_t>(1, std::min<int64_t>(n_tiles, kCodeBudget / tile_bytes));
+        (int32_t)std::max<int64_t>(1, std::min<int64_t>(n_tiles, code_budget() / tile_bytes));
     if (t.codes.count < (size_t)(group * tile_bytes)) {
         CX_HIP(hipStreamSynchronize(st));
         t.codes.alloc((size_t)(group * tile_bytes));

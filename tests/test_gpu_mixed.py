@@ -317,3 +317,28 @@ def test_screen_segment_lengths_same_results(hip, pred):
     _check(outs[0][0][qs], outs[0][1][qs], outs[0][2][qs], x, v, p, qs, k, pred, 1)
     with pytest.raises(ValueError):
         tree.set_screen_segment(300)
+
+
+@pytest.mark.parametrize("pred", [True, False])
+def test_launch_groups_same_results(hip, monkeypatch, pred):
+    # a small code budget splits the screen into launch groups (each
+    # group's screen, tau and filter over its query tiles, one code buffer
+    # reused); the results are those of one group
+    from covertreex_amd.native import HipTree
+
+    n, d, m, k = 5000, 3, 128, 20
+    x = np.random.default_rng(5).standard_normal((n, d)).astype(np.float32)
+    v, p, *_ = fp64.nystrom(x, inducing_count=m, seed=0, kernel_type=1)
+    tree = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
+    q = np.arange(n)
+    one = tree.knn_residual(q, k, predecessor_mode=pred, mode="mixed", return_error=True)
+    # 79 query tiles of 64 queries; a tile's codes take n_seg * seg * 64 bytes
+    tile_bytes = ((n + 511) // 512) * 512 * 64
+    for tiles in (1, 2, 5, 40):
+        monkeypatch.setenv("COVERTREEX_HIP_CODE_BUDGET", str(tiles * tile_bytes))
+        got = tree.knn_residual(q, k, predecessor_mode=pred, mode="mixed", return_error=True)
+        assert np.array_equal(got[0], one[0])
+        assert np.array_equal(got[1].view(np.uint32), one[1].view(np.uint32))
+        assert np.array_equal(got[2].view(np.uint32), one[2].view(np.uint32))
+    qs = q[::41]
+    _check(one[0][qs], one[1][qs], one[2][qs], x, v, p, qs, k, pred, 1)
