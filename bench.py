@@ -221,6 +221,23 @@ def other_workload(name: str, steps: int, warmup: int) -> dict:
                                     f"{'Matern52 l=1' if kt == 1 else 'ARD RBF'}")}}
 
 
+def build_timing(reps: int) -> dict:
+    """Wall time of the C2 build call (host V, p and coords in; the tree
+    resident in HBM out), one untimed build first."""
+    from covertreex_amd.native import HipTree
+
+    x, v, p = make_inputs()
+    ms = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        tree = HipTree.build_residual(v, p, x, 1.0, np.ones(1, np.float32), kernel_type=1)
+        if r:
+            ms.append(round((time.perf_counter() - t0) * 1e3, 3))
+        tree.close()
+    return {"build_ms": ms, "median_ms": float(np.median(ms)),
+            "config": {"workload": f"C2 build: N={N} D={D} m={M}, chunk 512, Morton order"}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +253,9 @@ def main() -> int:
                     help="c2 (default) is the headline line; c1/c3/c4 print their own")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--build-reps", type=int, default=0,
+                    help="only time R builds of the C2 tree from host V, p and coords "
+                         "(after one untimed build)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
@@ -255,6 +275,10 @@ def main() -> int:
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}))
+        return 0
+
+    if args.build_reps > 0:
+        print(json.dumps(build_timing(args.build_reps)))
         return 0
 
     if args.workload != "c2":

@@ -1261,40 +1261,52 @@ __global__ void __launch_bounds__(kBiResThreads)
 
 // ---- Morton order ---------------------------------------------------------
 
-__global__ void minmax3_kernel(const float *__restrict__ x, int64_t n, float *out)
+constexpr int kMinmaxThreads = 1024;
+__global__ void __launch_bounds__(kMinmaxThreads)
+    minmax3_kernel(const float *__restrict__ x, int64_t n, float *out)
 {
-    // per-axis min/max; NaN never updates (matches `if v < mins[i]`)
-    __shared__ float smin[3][256], smax[3][256];
+    // per-axis min/max; NaN never updates (matches `if v < mins[i]`).  The
+    // threads read consecutive elements (element e is axis e % 3); min and
+    // max of the values taken do not depend on the order they are seen in.
+    __shared__ float smin[3][kMinmaxThreads], smax[3][kMinmaxThreads];
     float mn[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
     float mx[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
-        for (int c = 0; c < 3; c++) {
-            float v = x[i * 3 + c];
-            if (v < mn[c])
-                mn[c] = v;
-            if (v > mx[c])
-                mx[c] = v;
-        }
-    for (int c = 0; c < 3; c++) {
-        smin[c][threadIdx.x] = mn[c];
-        smax[c][threadIdx.x] = mx[c];
+    const int64_t total = n * 3;
+    int c = (int)(threadIdx.x % 3);
+    constexpr int kStep = kMinmaxThreads % 3; // axis advance per stride
+    for (int64_t e = threadIdx.x; e < total; e += kMinmaxThreads) {
+        const float v = x[e];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            if (a == c) {
+                if (v < mn[a])
+                    mn[a] = v;
+                if (v > mx[a])
+                    mx[a] = v;
+            }
+        c += kStep;
+        c = c >= 3 ? c - 3 : c;
+    }
+    for (int a = 0; a < 3; a++) {
+        smin[a][threadIdx.x] = mn[a];
+        smax[a][threadIdx.x] = mx[a];
     }
     __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    for (int o = kMinmaxThreads / 2; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o)
-            for (int c = 0; c < 3; c++) {
-                float a = smin[c][threadIdx.x + o], b = smax[c][threadIdx.x + o];
-                if (a < smin[c][threadIdx.x])
-                    smin[c][threadIdx.x] = a;
-                if (b > smax[c][threadIdx.x])
-                    smax[c][threadIdx.x] = b;
+            for (int a = 0; a < 3; a++) {
+                const float lo = smin[a][threadIdx.x + o], hi = smax[a][threadIdx.x + o];
+                if (lo < smin[a][threadIdx.x])
+                    smin[a][threadIdx.x] = lo;
+                if (hi > smax[a][threadIdx.x])
+                    smax[a][threadIdx.x] = hi;
             }
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int c = 0; c < 3; c++) {
-            out[c] = smin[c][0];
-            out[3 + c] = smax[c][0];
+        for (int a = 0; a < 3; a++) {
+            out[a] = smin[a][0];
+            out[3 + a] = smax[a][0];
         }
 }
 
@@ -1339,8 +1351,10 @@ __global__ void iota_kernel(int64_t *out, int64_t n)
 }
 
 // Morton permutation on the device; order_dev receives n entries.
+// (sync: wait for the sort before returning; with stream-ordered work
+// buffers, tls_alloc_stream == st, the caller may leave it queued)
 static void morton_order_on_stream(const float *coords_dev, int64_t n, int32_t d,
-                                   int64_t *order_dev, hipStream_t st)
+                                   int64_t *order_dev, hipStream_t st, bool sync = true)
 {
     const int threads = 256;
     const unsigned blocks = (unsigned)((n + threads - 1) / threads);
@@ -1354,7 +1368,7 @@ static void morton_order_on_stream(const float *coords_dev, int64_t n, int32_t d
     DevBuf<float> mm(6);
     DevBuf<uint64_t> codes(n), codes_sorted(n);
     DevBuf<int64_t> idx(n);
-    minmax3_kernel<<<1, 256, 0, st>>>(coords_dev, n, mm.ptr);
+    minmax3_kernel<<<1, kMinmaxThreads, 0, st>>>(coords_dev, n, mm.ptr);
     morton_codes_kernel<<<blocks, threads, 0, st>>>(coords_dev, n, mm.ptr, codes.ptr, idx.ptr);
     CX_HIP(hipGetLastError());
     // radix sort is stable: ties keep index order, as slice::sort_by_key does
@@ -1364,7 +1378,8 @@ static void morton_order_on_stream(const float *coords_dev, int64_t n, int32_t d
     DevBuf<uint8_t> tmp(tmp_bytes);
     CX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, codes.ptr, codes_sorted.ptr,
                                               idx.ptr, order_dev, (int)n, 0, 64, st));
-    CX_HIP(hipStreamSynchronize(st));
+    if (sync || tls_alloc_stream != st)
+        CX_HIP(hipStreamSynchronize(st));
 }
 
 // ---- reorder + metric precompute -------------------------------------------
@@ -1964,6 +1979,27 @@ static MetricView residual_view(const TreeState &t)
 // Rows [n0, n0 + nn) of the tree arrays from host V, p, coords (nn rows):
 // tree row n0 + i takes staged row sel[i]; coordinates are also scaled by
 // 1/lengthscale.
+// Rows [n0, n0 + nn) of the tree arrays from device staging buffers of V,
+// coords and p (nn rows each, ordered on st): tree row n0 + i takes staged
+// row sel[i]; coordinates are also scaled by 1/lengthscale.
+static void gather_staged_rows(TreeState &t, const float *v_in, const float *c_in,
+                               const float *p_in, int64_t n0, int64_t nn, const int64_t *sel,
+                               hipStream_t st)
+{
+    const int32_t m = t.m, d = t.d;
+    gather_rows_kernel<<<(unsigned)nn, 64, 0, st>>>(v_in, sel, nn, m, t.v.ptr + n0 * m);
+    gather_rows_kernel<<<(unsigned)nn, 64, 0, st>>>(c_in, sel, nn, d, t.raw.ptr + n0 * d);
+    gather_rows_kernel<<<(unsigned)nn, 64, 0, st>>>(p_in, sel, nn, 1, t.p.ptr + n0);
+    DevBuf<float> inv_ls(d);
+    CX_HIP(hipMemcpyAsync(inv_ls.ptr, t.inv_ls.data(), sizeof(float) * d, hipMemcpyHostToDevice,
+                          st));
+    scale_rows_kernel<<<nblocks(nn, 256), 256, 0, st>>>(t.raw.ptr + n0 * d, nn, d, inv_ls.ptr,
+                                                        t.scaled.ptr + n0 * d, t.norms.ptr + n0);
+    CX_HIP(hipGetLastError());
+    CX_HIP(hipStreamSynchronize(st)); // the staging buffers go out of scope
+}
+
+// The same rows from host V, p, coords (nn rows).
 static void load_residual_rows(TreeState &t, const float *V, const float *p, const float *coords,
                                int64_t n0, int64_t nn, const int64_t *sel, hipStream_t st)
 {
@@ -1972,16 +2008,7 @@ static void load_residual_rows(TreeState &t, const float *V, const float *p, con
     CX_HIP(hipMemcpyAsync(v_in.ptr, V, sizeof(float) * nn * m, hipMemcpyHostToDevice, st));
     CX_HIP(hipMemcpyAsync(c_in.ptr, coords, sizeof(float) * nn * d, hipMemcpyHostToDevice, st));
     CX_HIP(hipMemcpyAsync(p_in.ptr, p, sizeof(float) * nn, hipMemcpyHostToDevice, st));
-    gather_rows_kernel<<<(unsigned)nn, 64, 0, st>>>(v_in.ptr, sel, nn, m, t.v.ptr + n0 * m);
-    gather_rows_kernel<<<(unsigned)nn, 64, 0, st>>>(c_in.ptr, sel, nn, d, t.raw.ptr + n0 * d);
-    gather_rows_kernel<<<(unsigned)nn, 64, 0, st>>>(p_in.ptr, sel, nn, 1, t.p.ptr + n0);
-    DevBuf<float> inv_ls(d);
-    CX_HIP(hipMemcpyAsync(inv_ls.ptr, t.inv_ls.data(), sizeof(float) * d, hipMemcpyHostToDevice,
-                          st));
-    scale_rows_kernel<<<nblocks(nn, 256), 256, 0, st>>>(t.raw.ptr + n0 * d, nn, d, inv_ls.ptr,
-                                                        t.scaled.ptr + n0 * d, t.norms.ptr + n0);
-    CX_HIP(hipGetLastError());
-    CX_HIP(hipStreamSynchronize(st)); // the staging buffers go out of scope
+    gather_staged_rows(t, v_in.ptr, c_in.ptr, p_in.ptr, n0, nn, sel, st);
 }
 
 void build_residual_tree(TreeState &t, const float *V, const float *p, const float *coords,
@@ -1996,24 +2023,52 @@ void build_residual_tree(TreeState &t, const float *V, const float *p, const flo
 
     t.order.alloc(n);
     t.inv_order.alloc(n);
-    if (order_mode == CX_ORDER_NATURAL || d != 3) {
-        iota_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.order.ptr, n);
-        CX_HIP(hipGetLastError());
-    } else {
-        DevBuf<float> c_in((size_t)n * d);
-        CX_HIP(hipMemcpyAsync(c_in.ptr, coords, sizeof(float) * n * d, hipMemcpyHostToDevice,
-                              st));
-        morton_order_on_stream(c_in.ptr, n, d, t.order.ptr, st);
-        CX_HIP(hipStreamSynchronize(st));
-    }
-    inverse_perm_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.order.ptr, n, t.inv_order.ptr);
-
     t.v.alloc((size_t)n * m);
     t.raw.alloc((size_t)n * d);
     t.scaled.alloc((size_t)n * d);
     t.norms.alloc(n);
     t.p.alloc(n);
-    load_residual_rows(t, V, p, coords, 0, n, t.order.ptr, st);
+    if (order_mode == CX_ORDER_NATURAL || d != 3) {
+        iota_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.order.ptr, n);
+        CX_HIP(hipGetLastError());
+        inverse_perm_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.order.ptr, n, t.inv_order.ptr);
+        load_residual_rows(t, V, p, coords, 0, n, t.order.ptr, st);
+    } else {
+        // The staged V and p go up on a second stream while the Morton
+        // order is computed: the host copies from pageable memory (blocking)
+        // only after the sort is queued, and the second stream waits only
+        // for the staging buffers' allocation.
+        DevBuf<float> v_in((size_t)n * m), c_in((size_t)n * d), p_in(n);
+        struct Upload {
+            int dev;
+            hipStream_t s = nullptr;
+            hipEvent_t alloc_done = nullptr, copied = nullptr;
+            ~Upload()
+            {
+                if (s)
+                    release_stream(dev, s); // synchronizes it
+                if (alloc_done)
+                    release_event(alloc_done);
+                if (copied)
+                    release_event(copied);
+            }
+        } up{t.device};
+        up.s = acquire_stream(t.device);
+        up.alloc_done = acquire_event();
+        up.copied = acquire_event();
+        CX_HIP(hipEventRecord(up.alloc_done, st));
+        CX_HIP(hipMemcpyAsync(c_in.ptr, coords, sizeof(float) * n * d, hipMemcpyHostToDevice,
+                              st));
+        morton_order_on_stream(c_in.ptr, n, d, t.order.ptr, st, false);
+        inverse_perm_kernel<<<nblocks(n, 256), 256, 0, st>>>(t.order.ptr, n, t.inv_order.ptr);
+        CX_HIP(hipGetLastError());
+        CX_HIP(hipStreamWaitEvent(up.s, up.alloc_done, 0));
+        CX_HIP(hipMemcpyAsync(v_in.ptr, V, sizeof(float) * n * m, hipMemcpyHostToDevice, up.s));
+        CX_HIP(hipMemcpyAsync(p_in.ptr, p, sizeof(float) * n, hipMemcpyHostToDevice, up.s));
+        CX_HIP(hipEventRecord(up.copied, up.s));
+        CX_HIP(hipStreamWaitEvent(st, up.copied, 0));
+        gather_staged_rows(t, v_in.ptr, c_in.ptr, p_in.ptr, 0, n, t.order.ptr, st);
+    }
 
     t.parents.alloc(n);
     t.children.alloc(n);
