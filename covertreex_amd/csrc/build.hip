@@ -71,19 +71,26 @@ __device__ __forceinline__ void wave_lds_sync()
 // Residual distances d(q, node) of a wavefront's lanes (q the same on every
 // lane, node per lane, wanted where live; every lane calls it) with the V-row
 // dots eight lanes per pair: pass p gives lane group gr the pair of lane
-// 8p + gr, whose chunk sums (reduce8 of eight products each) go to that
-// pair's LDS row (cs: 64 rows of kCoopRow floats, this wavefront's); each
-// lane then adds its own row in chunk order from 0.0 -- dot_slice's
-// operations, so the result has build_dist_sq's bits.  Needs coop_rank(m).
+// 8p + gr, whose chunk sums (reduce8 of eight products each) go to an LDS
+// row of kCoopRow floats; each lane then adds its own row in chunk order
+// from 0.0 -- dot_slice's operations, so the result has build_dist_sq's
+// bits.  CR = 64: cs holds a row per lane (64 rows) and every lane folds
+// after the eight passes; CR = 8: cs holds the eight rows of one pass, which
+// its owner lanes (8p .. 8p + 7) fold before the next pass.  Needs
+// coop_rank(m).
+template <int CR>
 __device__ __forceinline__ float coop_wave_dist_sq(const MetricView &mv, int64_t q,
                                                    int64_t node, bool live, float *cs)
 {
+    static_assert(CR == 64 || CR == 8, "64 rows, or the eight of one pass");
     const int lane = threadIdx.x & 63, sub = lane & 7, gr = lane >> 3, nch = mv.m >> 3;
     const float *va = mv.v + q * mv.m;
     const uint64_t ball = __ballot(live);
+    float acc = 0.0f;
     for (int p = 0; p < 8; p++) {
         const int jj = 8 * p + gr;
         const int64_t nd = __shfl(node, jj, 64);
+        float *row = cs + (CR == 64 ? jj : gr) * kCoopRow;
         if ((ball >> jj) & 1ull) {
             const float *vb = mv.v + nd * mv.m;
 #pragma unroll
@@ -96,32 +103,40 @@ __device__ __forceinline__ float coop_wave_dist_sq(const MetricView &mv, int64_t
                     const float4 b1 = *(const float4 *)(vb + 8 * c + 4);
                     const float pr[8] = {a0.x * b0.x, a0.y * b0.y, a0.z * b0.z, a0.w * b0.w,
                                          a1.x * b1.x, a1.y * b1.y, a1.z * b1.z, a1.w * b1.w};
-                    cs[jj * kCoopRow + c] = reduce8(pr);
+                    row[c] = reduce8(pr);
                 }
             }
         }
+        if (CR == 8) {
+            wave_lds_sync();
+            if (gr == p && live) {
+                const float *own = cs + sub * kCoopRow;
+                for (int c = 0; c < nch; c++)
+                    acc += own[c];
+            }
+            wave_lds_sync(); // the next pass rewrites the rows
+        }
     }
-    wave_lds_sync();
-    float d = 0.0f;
-    if (live) {
-        float acc = 0.0f;
-        for (int c = 0; c < nch; c++)
-            acc += cs[lane * kCoopRow + c];
-        d = dist_sq_ref_dot(mv, q, node, acc);
+    if (CR == 64) {
+        wave_lds_sync();
+        if (live)
+            for (int c = 0; c < nch; c++)
+                acc += cs[lane * kCoopRow + c];
+        wave_lds_sync(); // the next call rewrites the rows
     }
-    wave_lds_sync(); // the next call rewrites the rows
-    return d;
+    return live ? dist_sq_ref_dot(mv, q, node, acc) : 0.0f;
 }
 
 // build_dist_sq(q, node) where live (0 elsewhere), q the same on every lane;
-// every lane calls it.  With an LDS area (cs) and a rank coop_rank takes,
-// the dots go eight lanes per pair (coop_wave_dist_sq).
-template <int KIND>
+// every lane calls it.  With an LDS area (cs: CR rows of kCoopRow floats)
+// and a rank coop_rank takes, the dots go eight lanes per pair
+// (coop_wave_dist_sq).
+template <int KIND, int CR = 64>
 __device__ __forceinline__ float wave_dist_sq(const MetricView &mv, int64_t q, int64_t node,
                                               bool live, float *cs)
 {
     if (KIND == kMetricResidual && cs != nullptr && coop_rank(mv.m))
-        return coop_wave_dist_sq(mv, q, node, live, cs);
+        return coop_wave_dist_sq<CR>(mv, q, node, live, cs);
     return live ? build_dist_sq<KIND>(mv, q, node) : 0.0f;
 }
 
@@ -349,7 +364,7 @@ __global__ void __launch_bounds__(kBiResThreads)
 // its previous scan (BiState::stay).
 // wave0 / nw: this wavefront's first work item and the stride (the number of
 // wavefronts sharing the scan).
-template <int KIND>
+template <int KIND, int CR = 64>
 __device__ __forceinline__ void bi_scan_body(const MetricView &mv, const TreeLinks &t, BiState *S,
                                              const int32_t *__restrict__ rk, float r2,
                                              int64_t n_nodes, int64_t wave0, int64_t nw,
@@ -375,7 +390,7 @@ __device__ __forceinline__ void bi_scan_body(const MetricView &mv, const TreeLin
             uint64_t key = kBiFar;
             const bool cached = pos == 0 && st;
             const int64_t node = (pos == 0 || pos > kn) ? 0 : rk[nrk - pos];
-            float dc = wave_dist_sq<KIND>(mv, q, node, pos <= kn && !cached, cs);
+            float dc = wave_dist_sq<KIND, CR>(mv, q, node, pos <= kn && !cached, cs);
             if (pos <= kn) {
                 if (pos == 0) {
                     if (cached)
@@ -414,7 +429,8 @@ __device__ __forceinline__ void bi_scan_body(const MetricView &mv, const TreeLin
             }
             uint64_t key = kBiFar;
             const bool cached = base == 0 && lane == 0 && st;
-            float dc = wave_dist_sq<KIND>(mv, q, mine >= 0 ? mine : q, mine >= 0 && !cached, cs);
+            float dc =
+                wave_dist_sq<KIND, CR>(mv, q, mine >= 0 ? mine : q, mine >= 0 && !cached, cs);
             if (mine >= 0) {
                 if (base == 0 && lane == 0) {
                     if (cached)
@@ -739,7 +755,7 @@ __device__ __forceinline__ int far_list(const BiState *S, int nc, int32_t *s_far
 // grid bits alone in gadj.  Work items are (i, word w >= i / 64).
 // A workgroup of WAVES wavefronts; wave0 / nw as in bi_scan_body; stamp: this
 // workgroup records the pass's start tick.
-template <int KIND, int WAVES>
+template <int KIND, int WAVES, int CR = 64>
 __device__ __forceinline__ void bi_pairs_body(const MetricView &mv, BiState *S,
                                               const float *__restrict__ raw, int32_t gd,
                                               float gs, int32_t use_grid, float r2,
@@ -768,7 +784,7 @@ __device__ __forceinline__ void bi_pairs_body(const MetricView &mv, BiState *S,
         bool a = false, g = false;
         const bool live = j > i && j < nf;
         const int64_t x = s_far[i], y = live ? s_far[j] : x;
-        const float d = wave_dist_sq<KIND>(mv, x, y, live, cs);
+        const float d = wave_dist_sq<KIND, CR>(mv, x, y, live, cs);
         if (live) {
             if (use_grid)
                 g = grid_adjacent(raw, gd, gs, x, y);
@@ -1236,7 +1252,11 @@ __global__ void __launch_bounds__(kBiResThreads)
 {
     __shared__ int32_t s_pfar[kBiMaxChunk];
     __shared__ int s_pcnt[kBiResWaves];
+    // per wavefront the eight chunk-sum rows of one cooperative pass (the
+    // resolve's arrays leave no room for 64)
+    __shared__ float s_cs8[kBiResWaves * 8 * kCoopRow];
     const int wave = (int)(threadIdx.x >> 6);
+    float *cs = s_cs8 + wave * 8 * kCoopRow;
     // thread 0 updates ev_scan in every resolve, so its own read is current
     const int64_t ev0 = threadIdx.x == 0 ? S->ev_scan : 0;
     const int64_t evp0 = threadIdx.x == 0 ? S->ev_mis + S->ev_nl : 0;
@@ -1244,11 +1264,11 @@ __global__ void __launch_bounds__(kBiResThreads)
         if (level > -10 && S->nc == 0)
             continue;
         const float r = ldexpf(1.0f, level), r2 = r * r;
-        bi_scan_body<KIND>(mv, t, S, rk, r2, n_nodes, wave, kBiResWaves);
+        bi_scan_body<KIND, 8>(mv, t, S, rk, r2, n_nodes, wave, kBiResWaves, cs);
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        bi_pairs_body<KIND, kBiResWaves>(mv, S, raw, gd, gs, use_grid, r2, dist, adj, gadj,
-                                         s_pfar, s_pcnt, wave, kBiResWaves, true);
+        bi_pairs_body<KIND, kBiResWaves, 8>(mv, S, raw, gd, gs, use_grid, r2, dist, adj, gadj,
+                                            s_pfar, s_pcnt, wave, kBiResWaves, true, cs);
         __syncthreads();
         bi_resolve_body<KIND>(t, S, rk, dist, adj, gadj, level, r2, bs, CL);
         __syncthreads();
