@@ -119,6 +119,104 @@ int32_t *acquire_host_word()
     return w;
 }
 
+namespace {
+constexpr size_t kStageChunk = (size_t)8 << 20;
+constexpr size_t kStageMin = (size_t)4 << 20;
+struct Stager {
+    int device;
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false}; // a DMA recorded on ev[i] not yet waited for
+};
+std::mutex g_stage_mu;
+std::vector<Stager *> g_stagers;
+
+Stager *acquire_stager()
+{
+    int dev = 0;
+    CX_HIP(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> lock(g_stage_mu);
+        for (size_t i = 0; i < g_stagers.size(); i++)
+            if (g_stagers[i]->device == dev) {
+                Stager *g = g_stagers[i];
+                g_stagers.erase(g_stagers.begin() + (ptrdiff_t)i);
+                return g;
+            }
+    }
+    Stager *g = new Stager{dev};
+    try {
+        for (int i = 0; i < 2; i++) {
+            CX_HIP(hipHostMalloc(&g->buf[i], kStageChunk, hipHostMallocPortable));
+            CX_HIP(hipEventCreateWithFlags(&g->ev[i], hipEventDisableTiming));
+        }
+    } catch (...) {
+        for (int i = 0; i < 2; i++) {
+            if (g->buf[i])
+                (void)hipHostFree(g->buf[i]);
+            if (g->ev[i])
+                (void)hipEventDestroy(g->ev[i]);
+        }
+        delete g;
+        throw;
+    }
+    return g;
+}
+
+// Waits for the stager's DMAs (recorded on the caller's stream, which the
+// caller still owns), so the next user never waits on a stale record.
+struct StagerLease {
+    Stager *g;
+    explicit StagerLease(Stager *s) : g(s) {}
+    ~StagerLease()
+    {
+        for (int i = 0; i < 2; i++)
+            if (g->pending[i]) {
+                (void)hipEventSynchronize(g->ev[i]);
+                g->pending[i] = false;
+            }
+        std::lock_guard<std::mutex> lock(g_stage_mu);
+        g_stagers.push_back(g);
+    }
+    StagerLease(const StagerLease &) = delete;
+    StagerLease &operator=(const StagerLease &) = delete;
+};
+} // namespace
+
+void copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    if (bytes < kStageMin) {
+        CX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        CX_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    StagerLease lease(acquire_stager());
+    Stager *g = lease.g;
+    const size_t n = (bytes + kStageChunk - 1) / kStageChunk;
+    auto issue = [&](size_t i) {
+        const int slot = (int)(i & 1);
+        const size_t off = i * kStageChunk;
+        const size_t len = bytes - off < kStageChunk ? bytes - off : kStageChunk;
+        CX_HIP(hipMemcpyAsync(g->buf[slot], (const char *)src + off, len, hipMemcpyDeviceToHost,
+                              s));
+        CX_HIP(hipEventRecord(g->ev[slot], s));
+        g->pending[slot] = true;
+    };
+    issue(0);
+    for (size_t i = 0; i < n; i++) {
+        // chunk i + 1 comes down while chunk i is copied out (its buffer's
+        // last chunk, i - 1, was copied out in the previous step)
+        if (i + 1 < n)
+            issue(i + 1);
+        const int slot = (int)(i & 1);
+        CX_HIP(hipEventSynchronize(g->ev[slot]));
+        g->pending[slot] = false;
+        const size_t off = i * kStageChunk;
+        const size_t len = bytes - off < kStageChunk ? bytes - off : kStageChunk;
+        std::memcpy((char *)dst + off, g->buf[slot], len);
+    }
+}
+
 void release_host_word(int32_t *w)
 {
     std::lock_guard<std::mutex> lock(g_pool_mu);
@@ -491,14 +589,10 @@ cx_status cx_tree_knn_residual(cx_tree *t, const int64_t *queries, int64_t nq, i
                                                   od.ptr, out_err ? oe.ptr : nullptr);
         if (s != CX_OK)
             throw Error(s, g_last_error);
-        CX_HIP(hipMemcpyAsync(out_idx, oi.ptr, sizeof(int64_t) * nq * k, hipMemcpyDeviceToHost,
-                              st));
-        CX_HIP(hipMemcpyAsync(out_dist, od.ptr, sizeof(float) * nq * k, hipMemcpyDeviceToHost,
-                              st));
+        copy_d2h(out_idx, oi.ptr, sizeof(int64_t) * nq * k, st);
+        copy_d2h(out_dist, od.ptr, sizeof(float) * nq * k, st);
         if (out_err)
-            CX_HIP(hipMemcpyAsync(out_err, oe.ptr, sizeof(float) * nq * k,
-                                  hipMemcpyDeviceToHost, st));
-        CX_HIP(hipStreamSynchronize(st));
+            copy_d2h(out_err, oe.ptr, sizeof(float) * nq * k, st);
     });
 }
 
@@ -556,11 +650,8 @@ cx_status cx_tree_knn_euclidean(cx_tree *t, const float *queries, int64_t nq, in
         CX_HIP(hipMemcpyAsync(q.ptr, queries, sizeof(float) * nq * s.d, hipMemcpyHostToDevice,
                               st));
         knn_euclidean(s, q.ptr, nq, k, predecessor, mode, oi.ptr, od.ptr);
-        CX_HIP(hipMemcpyAsync(out_idx, oi.ptr, sizeof(int64_t) * nq * k, hipMemcpyDeviceToHost,
-                              st));
-        CX_HIP(hipMemcpyAsync(out_dist, od.ptr, sizeof(float) * nq * k, hipMemcpyDeviceToHost,
-                              st));
-        CX_HIP(hipStreamSynchronize(st));
+        copy_d2h(out_idx, oi.ptr, sizeof(int64_t) * nq * k, st);
+        copy_d2h(out_dist, od.ptr, sizeof(float) * nq * k, st);
     });
 }
 

@@ -1286,26 +1286,36 @@ __global__ void __launch_bounds__(kMinmaxThreads)
     minmax3_kernel(const float *__restrict__ x, int64_t n, float *out)
 {
     // per-axis min/max; NaN never updates (matches `if v < mins[i]`).  The
-    // threads read consecutive elements (element e is axis e % 3); min and
-    // max of the values taken do not depend on the order they are seen in.
+    // threads read consecutive elements (element e is axis e % 3), eight
+    // loads in flight per thread; min and max of the values taken do not
+    // depend on the order they are seen in.
     __shared__ float smin[3][kMinmaxThreads], smax[3][kMinmaxThreads];
     float mn[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
     float mx[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
     const int64_t total = n * 3;
-    int c = (int)(threadIdx.x % 3);
-    constexpr int kStep = kMinmaxThreads % 3; // axis advance per stride
-    for (int64_t e = threadIdx.x; e < total; e += kMinmaxThreads) {
-        const float v = x[e];
+    constexpr int kDepth = 8;
+    for (int64_t base = threadIdx.x; base < total; base += kDepth * kMinmaxThreads) {
+        float v[kDepth];
 #pragma unroll
-        for (int a = 0; a < 3; a++)
-            if (a == c) {
-                if (v < mn[a])
-                    mn[a] = v;
-                if (v > mx[a])
-                    mx[a] = v;
-            }
-        c += kStep;
-        c = c >= 3 ? c - 3 : c;
+        for (int u = 0; u < kDepth; u++) {
+            const int64_t e = base + (int64_t)u * kMinmaxThreads;
+            v[u] = e < total ? x[e] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kDepth; u++) {
+            const int64_t e = base + (int64_t)u * kMinmaxThreads;
+            if (e >= total)
+                break;
+            const int c = (int)(e % 3);
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                if (a == c) {
+                    if (v[u] < mn[a])
+                        mn[a] = v[u];
+                    if (v[u] > mx[a])
+                        mx[a] = v[u];
+                }
+        }
     }
     for (int a = 0; a < 3; a++) {
         smin[a][threadIdx.x] = mn[a];

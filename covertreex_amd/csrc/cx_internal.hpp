@@ -79,6 +79,14 @@ void release_host_word(int32_t *w);
 hipEvent_t acquire_event();
 void release_event(hipEvent_t e);
 
+// Device-to-host copies into pageable memory through pinned staging chunks
+// (a pool of two-chunk stagers, one per concurrent caller): the DMA of one
+// chunk runs while the other is copied out on the host.  Copies below a few
+// MiB go straight to hipMemcpyAsync.  Returns with the data in dst (s
+// drained up to the copy).  (Uploads stay on HIP's own pageable path, which
+// measured faster here than staging them the same way.)
+void copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t s);
+
 // Owning device buffer.
 template <typename T> struct DevBuf {
     T *ptr = nullptr;

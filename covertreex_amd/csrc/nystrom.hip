@@ -396,9 +396,8 @@ void nystrom_device(const double *pts_host, int64_t n, int32_t d, const int64_t 
     CX_HIP(hipStreamSynchronize(st));
     if (bad)
         throw Error(CX_ERR_INVALID, "inducing kernel matrix is not positive definite");
-    CX_HIP(hipMemcpyAsync(v_host, v.ptr, sizeof(float) * n * m, hipMemcpyDeviceToHost, st));
-    CX_HIP(hipMemcpyAsync(p_host, p.ptr, sizeof(float) * n, hipMemcpyDeviceToHost, st));
-    CX_HIP(hipStreamSynchronize(st));
+    copy_d2h(v_host, v.ptr, sizeof(float) * n * m, st);
+    copy_d2h(p_host, p.ptr, sizeof(float) * n, st);
 }
 
 } // namespace cx
