@@ -56,9 +56,14 @@ __device__ __forceinline__ float build_dist_sq(const MetricView &mv, int64_t i, 
         return dist_sq_ref(mv, i, j);
 }
 
-// LDS rows of per-chunk dot sums (32 chunks of eight, padded by a word)
-constexpr int kCoopRow = 33;
-__device__ __forceinline__ bool coop_rank(int32_t m) { return (m & 7) == 0 && m >= 64 && m <= 256; }
+// LDS rows of per-chunk dot sums, padded by a word: 64 chunks of eight
+// (m <= 512) in the 64-row areas, 32 (m <= 256) in the eight-row areas of
+// the fused kernel, whose LDS is nearly full
+template <int CR> constexpr int coop_row() { return CR == 64 ? 65 : 33; }
+template <int CR> __device__ __forceinline__ bool coop_rank(int32_t m)
+{
+    return (m & 7) == 0 && m >= 64 && m <= 8 * (coop_row<CR>() - 1);
+}
 
 // orders the wavefront's own LDS accesses (no workgroup barrier)
 __device__ __forceinline__ void wave_lds_sync()
@@ -72,17 +77,18 @@ __device__ __forceinline__ void wave_lds_sync()
 // lane, node per lane, wanted where live; every lane calls it) with the V-row
 // dots eight lanes per pair: pass p gives lane group gr the pair of lane
 // 8p + gr, whose chunk sums (reduce8 of eight products each) go to an LDS
-// row of kCoopRow floats; each lane then adds its own row in chunk order
+// row of coop_row<CR>() floats; each lane then adds its own row in chunk order
 // from 0.0 -- dot_slice's operations, so the result has build_dist_sq's
 // bits.  CR = 64: cs holds a row per lane (64 rows) and every lane folds
 // after the eight passes; CR = 8: cs holds the eight rows of one pass, which
 // its owner lanes (8p .. 8p + 7) fold before the next pass.  Needs
-// coop_rank(m).
+// coop_rank<CR>(m).
 template <int CR>
 __device__ __forceinline__ float coop_wave_dist_sq(const MetricView &mv, int64_t q,
                                                    int64_t node, bool live, float *cs)
 {
     static_assert(CR == 64 || CR == 8, "64 rows, or the eight of one pass");
+    constexpr int kRow = coop_row<CR>(), kU = (kRow - 1) / 8; // chunks per lane
     const int lane = threadIdx.x & 63, sub = lane & 7, gr = lane >> 3, nch = mv.m >> 3;
     const float *va = mv.v + q * mv.m;
     const uint64_t ball = __ballot(live);
@@ -90,11 +96,11 @@ __device__ __forceinline__ float coop_wave_dist_sq(const MetricView &mv, int64_t
     for (int p = 0; p < 8; p++) {
         const int jj = 8 * p + gr;
         const int64_t nd = __shfl(node, jj, 64);
-        float *row = cs + (CR == 64 ? jj : gr) * kCoopRow;
+        float *row = cs + (CR == 64 ? jj : gr) * kRow;
         if ((ball >> jj) & 1ull) {
             const float *vb = mv.v + nd * mv.m;
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < kU; u++) {
                 const int c = sub + 8 * u;
                 if (c < nch) {
                     const float4 a0 = *(const float4 *)(va + 8 * c);
@@ -110,7 +116,7 @@ __device__ __forceinline__ float coop_wave_dist_sq(const MetricView &mv, int64_t
         if (CR == 8) {
             wave_lds_sync();
             if (gr == p && live) {
-                const float *own = cs + sub * kCoopRow;
+                const float *own = cs + sub * kRow;
                 for (int c = 0; c < nch; c++)
                     acc += own[c];
             }
@@ -121,21 +127,21 @@ __device__ __forceinline__ float coop_wave_dist_sq(const MetricView &mv, int64_t
         wave_lds_sync();
         if (live)
             for (int c = 0; c < nch; c++)
-                acc += cs[lane * kCoopRow + c];
+                acc += cs[lane * kRow + c];
         wave_lds_sync(); // the next call rewrites the rows
     }
     return live ? dist_sq_ref_dot(mv, q, node, acc) : 0.0f;
 }
 
 // build_dist_sq(q, node) where live (0 elsewhere), q the same on every lane;
-// every lane calls it.  With an LDS area (cs: CR rows of kCoopRow floats)
-// and a rank coop_rank takes, the dots go eight lanes per pair
+// every lane calls it.  With an LDS area (cs: CR rows of coop_row<CR>()
+// floats) and a rank coop_rank<CR> takes, the dots go eight lanes per pair
 // (coop_wave_dist_sq).
 template <int KIND, int CR = 64>
 __device__ __forceinline__ float wave_dist_sq(const MetricView &mv, int64_t q, int64_t node,
                                               bool live, float *cs)
 {
-    if (KIND == kMetricResidual && cs != nullptr && coop_rank(mv.m))
+    if (KIND == kMetricResidual && cs != nullptr && coop_rank<CR>(mv.m))
         return coop_wave_dist_sq<CR>(mv, q, node, live, cs);
     return live ? build_dist_sq<KIND>(mv, q, node) : 0.0f;
 }
@@ -464,11 +470,11 @@ __global__ void __launch_bounds__(kBiParThreads)
     bi_scan_kernel(MetricView mv, TreeLinks t, BiState *S, const int32_t *__restrict__ rk,
                    float r2, int64_t n_nodes)
 {
-    __shared__ float s_cs[kBiParWaves * 64 * kCoopRow];
+    __shared__ float s_cs[kBiParWaves * 64 * coop_row<64>()];
     bi_scan_body<KIND>(mv, t, S, rk, r2, n_nodes,
                        (int64_t)blockIdx.x * kBiParWaves + (threadIdx.x >> 6),
                        (int64_t)gridDim.x * kBiParWaves,
-                       s_cs + (threadIdx.x >> 6) * 64 * kCoopRow);
+                       s_cs + (threadIdx.x >> 6) * 64 * coop_row<64>());
 }
 
 // Level 9 of a residual batch: every candidate starts at the root, so the
@@ -608,7 +614,7 @@ __global__ void __launch_bounds__(kBiParThreads)
     __shared__ int64_t s_cover, s_len, s_head;
     __shared__ int32_t s_pos;
     __shared__ float s_d0;
-    __shared__ float s_cs[kBiParWaves * 64 * kCoopRow];
+    __shared__ float s_cs[kBiParWaves * 64 * coop_row<64>()];
     const int i = (int)blockIdx.x;
     if (i >= S->nc)
         return;
@@ -645,7 +651,8 @@ __global__ void __launch_bounds__(kBiParThreads)
                 const int pos = base + lane;
                 const bool in = pos <= nrk;
                 const int64_t c = (pos == 0 || !in) ? 0 : rk[nrk - pos];
-                const float dc = wave_dist_sq<KIND>(mv, q, c, in, s_cs + wid * 64 * kCoopRow);
+                const float dc =
+                    wave_dist_sq<KIND>(mv, q, c, in, s_cs + wid * 64 * coop_row<64>());
                 if (pos == 0)
                     s_d0 = dc;
                 if (in && dc <= r2) {
@@ -807,11 +814,11 @@ __global__ void __launch_bounds__(kBiParThreads)
 {
     __shared__ int32_t s_far[kBiMaxChunk];
     __shared__ int s_cnt[kBiParWaves];
-    __shared__ float s_cs[kBiParWaves * 64 * kCoopRow];
+    __shared__ float s_cs[kBiParWaves * 64 * coop_row<64>()];
     bi_pairs_body<KIND, kBiParWaves>(mv, S, raw, gd, gs, use_grid, r2, dist, adj, gadj, s_far,
                                      s_cnt, (int)blockIdx.x * kBiParWaves + (int)(threadIdx.x >> 6),
                                      (int)gridDim.x * kBiParWaves, blockIdx.x == 0,
-                                     s_cs + (threadIdx.x >> 6) * 64 * kCoopRow);
+                                     s_cs + (threadIdx.x >> 6) * 64 * coop_row<64>());
 }
 
 // *acc (LDS) += the sum of v over the workgroup.
@@ -1254,9 +1261,9 @@ __global__ void __launch_bounds__(kBiResThreads)
     __shared__ int s_pcnt[kBiResWaves];
     // per wavefront the eight chunk-sum rows of one cooperative pass (the
     // resolve's arrays leave no room for 64)
-    __shared__ float s_cs8[kBiResWaves * 8 * kCoopRow];
+    __shared__ float s_cs8[kBiResWaves * 8 * coop_row<8>()];
     const int wave = (int)(threadIdx.x >> 6);
-    float *cs = s_cs8 + wave * 8 * kCoopRow;
+    float *cs = s_cs8 + wave * 8 * coop_row<8>();
     // thread 0 updates ev_scan in every resolve, so its own read is current
     const int64_t ev0 = threadIdx.x == 0 ? S->ev_scan : 0;
     const int64_t evp0 = threadIdx.x == 0 ? S->ev_mis + S->ev_nl : 0;

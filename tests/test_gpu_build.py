@@ -35,6 +35,11 @@ BUILD_CASES = [
     # level, many candidates kept at their cover level after level (the
     # incremental-scan path) and a root with thousands of children
     dict(n=8000, d=2, m=256, kt=0, chunk=512, gs=1.0, ls=1.5),
+    # ranks past 256 and between the powers of two: the cooperative dots of
+    # the descent, wide scans and conflict pairs (64 chunks of eight at
+    # m = 512), the fused levels' per-lane dots at m = 512
+    dict(n=3000, d=5, m=512, kt=0, chunk=512, gs=1.0, dist="uniform"),
+    dict(n=2500, d=3, m=200, kt=1, chunk=512, gs=1.0),
 ]
 
 
