@@ -1021,24 +1021,39 @@ __device__ __forceinline__ void bi_resolve_body(const TreeLinks &t, BiState *S,
         ev_mis = (int64_t)nf * (nf - 1) / 2;
         // greedy MIS in far order: take the first undecided candidate, drop
         // every later candidate in conflict with it
+        // Word by word (64 candidates): the block's own choices run in
+        // registers (lane b holds candidate 64 f + b's conflict word f, read
+        // by the chosen lane), then every later word drops the conflicts of
+        // the block's leaders; the same leaders as one candidate at a time.
         if (wid == 0) {
             uint64_t U = 0, sel = 0;
             if (lane < W)
                 U = (lane < W - 1 || (nf & 63) == 0) ? ~0ull : ((1ull << (nf & 63)) - 1);
-            for (;;) {
-                const uint64_t nz = __ballot(U != 0);
-                if (nz == 0)
-                    break;
-                const int f = __ffsll((unsigned long long)nz) - 1;
-                const uint64_t wf = __shfl(U, f, 64);
-                const int bit = __ffsll((unsigned long long)wf) - 1;
-                const int istar = f * 64 + bit;
-                if (lane == f) {
-                    U &= ~(1ull << bit);
-                    sel |= 1ull << bit;
+            for (int f = 0; f < W; f++) {
+                uint64_t u = __shfl(U, f, 64); // undecided after the earlier words
+                const int i = f * 64 + lane;
+                const uint64_t A = i < nf ? s_adj[adj_off(i, W)] : 0ull;
+                const uint32_t a_lo = (uint32_t)A, a_hi = (uint32_t)(A >> 32);
+                uint64_t sf = 0;
+                while (u != 0) {
+                    const int b = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)u) - 1);
+                    sf |= 1ull << b;
+                    const uint64_t row =
+                        (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)a_lo, b) |
+                        ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)a_hi, b) << 32);
+                    u &= ~((1ull << b) | row);
                 }
-                if (lane >= f && lane < W)
-                    U &= ~s_adj[adj_off(istar, W) + (lane - f)];
+                if (lane == f) {
+                    sel = sf;
+                    U = 0;
+                } else if (lane > f && lane < W) {
+                    uint64_t drop = 0;
+                    for (uint64_t t = sf; t != 0; t &= t - 1) {
+                        const int b = __ffsll((unsigned long long)t) - 1;
+                        drop |= s_adj[adj_off(f * 64 + b, W) + (lane - f)];
+                    }
+                    U &= ~drop;
+                }
             }
             if (lane < W)
                 s_sel[lane] = sel;
