@@ -1,6 +1,7 @@
 """Write Hilbert-order golden vectors from the reference's own ordering code.
 
-Run in the build container only (the reference is not on the GPU box):
+Run only where the reference's sources are present (the tests read the
+committed vectors, never the reference):
 
     python tests/golden/make_hilbert_golden.py
 
